@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the GraSPH2 derivative + leapfrog step on one MI355X.
+
+A step is the reference's main-loop body (src/main.cu:256-257):
+computeDerivatives(pb) then integrate(pb, true), here one
+grasph_compute_derivatives + one grasph_integrate on device-resident
+particles.  Default workload: the two-sphere solid collision, N = 131072
+(SPH solid + von Mises + Balsara + artificial viscosity + self-gravity,
+FP64 position drift).  Prints one JSON line.
+
+value = pair interactions per second = N (N - 1) / (seconds per step): every
+step evaluates softened gravity on all ordered pairs i != j.
+roofline: the all-pairs sweep kernel (the dominant one), FP32 vector ALU
+bound; its algorithmic FLOPs per launch are 20 per gravity pair (FMA = 2,
+rsqrt = 1) over N (N - 1) pairs plus 9 per neighbour pair for the density
+sum, divided by its mean duration from HIP events recorded around every
+launch on the launch stream over the timed steps.
+cpu_baseline: the CPU oracle in FP32 (the same op structure as the
+reference's host path) on a bounded sample of target particles i against
+all j, all three passes, OpenMP threads as reported.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (AMD published)
+PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (AMD published)
+FLOP_GRAVITY_PAIR = 20
+FLOP_DENSITY_NBR = 9
+FLOP_STRAIN_NBR = 63
+FLOP_ACCEL_NBR_SOLID = 74
+FLOP_ACCEL_NBR_LIQUID = 54
+INTEGRATOR_BYTES_SOLID = 168   # x64 24 + v 12 + a 12 + S 24 + dS 24 read; x64 24 + xf 12 + v 12 + S 24 written
+INTEGRATOR_BYTES_NBODY = 96    # x64 24 + v 12 + a 12 read; x64 24 + xf 12 + v 12 written
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="collision", choices=["collision", "plummer", "liquid", "large"])
+    ap.add_argument("--n", type=int, default=0, help="override the particle count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the sweep would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import grasph2_amd as g
+    from grasph2_amd import ics
+
+    kw = {"n": args.n} if args.n else {}
+    h, prm = ics.workload(args.workload, **kw)
+    n = h["posm"].shape[0]
+    solid = bool(prm.flags & g.SOLIDS)
+    sph = bool(prm.flags & g.SPH)
+
+    ctx = g.Context(0)
+    d = g.DeviceParticles.from_host(h)
+    g.hip.synchronize()
+
+    def step():
+        ctx.compute_derivatives(d, prm)
+        ctx.integrate(d, prm, True)
+
+    for _ in range(args.warmup):
+        step()
+    g.hip.synchronize()
+    ctx.set_option("profile", 1)
+    ctx.kernel_times(reset=True)
+    g.hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    g.hip.synchronize()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times(reset=True)
+    ctx.set_option("profile", 0)
+    nb_pairs, nb_max, nb_over = ctx.neighbour_stats() if sph else (0, 0, 0)
+
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    pairs = float(n) * (n - 1)
+    value = pairs / (ms_step * 1e-3)
+
+    sweep_ms = kt["sweep"][0] / max(kt["sweep"][1], 1)
+    sweep_flops = FLOP_GRAVITY_PAIR * pairs + (FLOP_DENSITY_NBR * nb_pairs if sph else 0)
+    achieved = sweep_flops / (sweep_ms * 1e-3) / 1e12
+    integ_ms = kt["integrate"][0] / max(kt["integrate"][1], 1)
+    integ_bytes = n * (INTEGRATOR_BYTES_SOLID if solid else INTEGRATOR_BYTES_NBODY)
+    step_flops = sweep_flops
+    if sph:
+        step_flops += nb_pairs * (FLOP_STRAIN_NBR + (FLOP_ACCEL_NBR_SOLID if solid else FLOP_ACCEL_NBR_LIQUID))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sweep_%s_%d_traffic.json" % (args.workload, n))
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "pair-interactions/s & %FP32 VALU peak at N=131k; batched-ensemble steps/s",
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded initial conditions of the workload)",
+        "config": {"workload": args.workload, "n": n, "flags": int(prm.flags), "H": prm.H,
+                   "dt": prm.fixed_timestep, "fp64_drift": True},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": "sweep", "kernel_ms": sweep_ms, "flops_per_launch": sweep_flops},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1]},
+        "step_tflops": step_flops / (ms_step * 1e-3) / 1e12,
+        "neighbour_pairs": nb_pairs,
+        "max_neighbours_seen": nb_max,
+        "list_overflows": nb_over,
+        "integrator": {"ms": integ_ms, "bytes_per_particle": INTEGRATOR_BYTES_SOLID if solid else INTEGRATOR_BYTES_NBODY,
+                       "gbps": integ_bytes / (integ_ms * 1e-3) / 1e9 if integ_ms > 0 else None,
+                       "hbm_frac": integ_bytes / (integ_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS if integ_ms > 0 else None},
+    }
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(d, prm, n, args.cpu_seconds)
+
+    d.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(d, prm, n, target_s):
+    """Time the FP32 oracle's three passes for a sample of i against all j."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    state = d.download()
+    rng = np.random.default_rng(0)
+
+    def run(count):
+        sample = np.sort(rng.choice(n, size=count, replace=False))
+        t = time.perf_counter()
+        O.calc_density(prm, state, "f32", sample)
+        O.calc_balsara(prm, state, "f32", sample)
+        O.calc_acceleration(prm, state, "f32", sample)
+        return time.perf_counter() - t
+
+    probe = min(n, max(64, threads * 8))
+    tp = run(probe)
+    count = int(min(n, max(probe, probe * target_s / max(tp, 1e-6))))
+    t = run(count)
+    return {"value": count * (n - 1) / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+            "sample": "FP32 oracle, calcDensity + calcBalsaraDensityDTDStressDT + calcAcceleration for %d of %d "
+                      "target particles against all %d partners, %.1f s" % (count, n, n, t)}
+
+
+if __name__ == "__main__":
+    main()

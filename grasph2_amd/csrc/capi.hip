@@ -1,0 +1,597 @@
+// capi.hip -- the C ABI (include/grasph2.h): argument checks, scratch
+// management, pass orchestration and per-kernel timing.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "internal.hpp"
+
+struct grasph_ctx {
+    grasph::Ctx c;
+};
+
+namespace grasph {
+
+static thread_local std::string g_create_err = "no error";
+
+ProfScope::ProfScope(Ctx* ctx, hipStream_t st, int k) : c(ctx), s(st), cls(k)
+{
+    if (!c || !c->profile) return;
+    for (int q = 0; q < 2; ++q) {
+        hipEvent_t e = nullptr;
+        if (!c->ev_free.empty()) {
+            e = c->ev_free.back();
+            c->ev_free.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        (q == 0 ? a : b) = e;
+    }
+    if (a) hipEventRecord(a, s);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!a || !b) {
+        if (a) c->ev_free.push_back(a);
+        if (b) c->ev_free.push_back(b);
+        return;
+    }
+    hipEventRecord(b, s);
+    c->ev_pending.push_back(Ctx::Ev{a, b, cls});
+}
+
+namespace {
+
+int fail(Ctx& c, int code, const std::string& msg)
+{
+    c.err = msg;
+    return code;
+}
+
+int hip_fail(Ctx& c, hipError_t e, const char* where)
+{
+    c.err = std::string(where) + ": " + hipGetErrorString(e);
+    return GRASPH_E_HIP;
+}
+
+template <typename T>
+int grow(Ctx& c, T*& ptr, int64_t& cap, int64_t need, const char* what)
+{
+    if (need <= cap && ptr) return 0;
+    if (ptr) {
+        hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    hipError_t e = hipMalloc((void**)&ptr, (size_t)need * sizeof(T));
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        (void)hipGetLastError();
+        return fail(c, GRASPH_E_NOMEM, std::string("device allocation of ") + what + " failed");
+    }
+    cap = need;
+    return 0;
+}
+
+int check_params(Ctx& c, const grasph_params* prm)
+{
+    if (!prm) return fail(c, GRASPH_E_INVALID, "params is NULL");
+    if (prm->flags & ~GRASPH_ALL_FLAGS) return fail(c, GRASPH_E_INVALID, "unknown flag bits");
+    if (prm->dimension < 1 || prm->dimension > 3) return fail(c, GRASPH_E_INVALID, "dimension must be 1, 2 or 3");
+    if (!(prm->H >= 0.0f) || !std::isfinite(prm->H)) return fail(c, GRASPH_E_INVALID, "H must be finite and >= 0");
+    if ((prm->flags & GRASPH_SPH) && !(prm->H > 0.0f)) return fail(c, GRASPH_E_INVALID, "SPH needs H > 0");
+    if ((prm->flags & GRASPH_SOLIDS) && (prm->flags & GRASPH_ARTIFICIAL_STRESS))
+        return fail(c, GRASPH_E_UNSUPPORTED,
+                    "SOLIDS with ARTIFICIAL_STRESS is not supported (the reference does not compile it, "
+                    "src/computeDerivatives.h:267)");
+    if ((prm->flags & GRASPH_PLASTICITY_MIESE) && (prm->flags & GRASPH_PLASTICITY_MC))
+        return fail(c, GRASPH_E_INVALID, "choose up to one plasticity model (settings.h:203-205)");
+    return 0;
+}
+
+int check_particles(Ctx& c, const grasph_particles* p)
+{
+    if (!p) return fail(c, GRASPH_E_INVALID, "particles is NULL");
+    if (p->n < 0) return fail(c, GRASPH_E_INVALID, "negative particle count");
+    if (p->n > 0x7ffffff0LL) return fail(c, GRASPH_E_INVALID, "particle count exceeds 2^31 - 16");
+    if (p->n > 0 && !p->posm) return fail(c, GRASPH_E_INVALID, "posm is NULL");
+    return 0;
+}
+
+#define NEED(ptr, name)                                                                       \
+    do {                                                                                      \
+        if (!(ptr)) return grasph::fail(c, GRASPH_E_INVALID, std::string(name) + " is NULL but the flags need it"); \
+    } while (0)
+
+int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_eval)
+{
+    const int64_t n = p->n;
+    if (c.reorder_interval <= 0) return 0;
+    if (c.reorder_cap < n || !c.perm) {
+        void* bufs[] = {c.perm, c.keys, c.keys_alt, c.vals};
+        for (void* b : bufs)
+            if (b) hipFree(b);
+        c.perm = nullptr; c.keys = nullptr; c.keys_alt = nullptr; c.vals = nullptr;
+        c.reorder_cap = 0;
+        c.perm_n = 0;
+        if (hipMalloc((void**)&c.perm, n * sizeof(int)) != hipSuccess ||
+            hipMalloc((void**)&c.keys, n * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc((void**)&c.keys_alt, n * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc((void**)&c.vals, n * sizeof(int)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(c, GRASPH_E_NOMEM, "device allocation of the Morton order failed");
+        }
+        c.reorder_cap = n;
+    }
+    if (!c.bbox && hipMalloc((void**)&c.bbox, 257 * 6 * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, GRASPH_E_NOMEM, "device allocation of the bounding box failed");
+    }
+    const bool stale = c.perm_n != n || c.perm_src != (const void*)p->posm ||
+                       c.evals_since_reorder >= c.reorder_interval;
+    if (stale) {
+        const size_t need = reorder_tmp_bytes((int)n);
+        if (need > c.sort_tmp_bytes || !c.sort_tmp) {
+            if (c.sort_tmp) hipFree(c.sort_tmp);
+            c.sort_tmp = nullptr;
+            c.sort_tmp_bytes = 0;
+            if (hipMalloc(&c.sort_tmp, need ? need : 1) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(c, GRASPH_E_NOMEM, "device allocation of sort storage failed");
+            }
+            c.sort_tmp_bytes = need;
+        }
+        ProfScope ps(&c, s, KC_REORDER);
+        hipError_t e = launch_reorder((const float4*)p->posm, (int)n, c, s);
+        if (e != hipSuccess) return hip_fail(c, e, "Morton reorder");
+        c.perm_n = n;
+        c.perm_src = p->posm;
+        c.evals_since_reorder = 0;
+    }
+    if (count_eval) c.evals_since_reorder++;
+    return 0;
+}
+
+const int* order_of(Ctx& c) { return c.reorder_interval > 0 ? c.perm : nullptr; }
+
+int ensure_nbr(Ctx& c, int64_t n)
+{
+    const int k = c.max_neighbours;
+    if (c.nbr && c.nbr_cap_n >= n && c.nbr_cap_k >= k) return 0;
+    if (c.nbr) hipFree(c.nbr);
+    if (c.nbr_cnt) hipFree(c.nbr_cnt);
+    c.nbr = nullptr; c.nbr_cnt = nullptr; c.nbr_cap_n = 0; c.nbr_cap_k = 0;
+    if (hipMalloc((void**)&c.nbr, (size_t)n * (size_t)k * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&c.nbr_cnt, (size_t)n * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, GRASPH_E_NOMEM, "device allocation of the neighbour list failed");
+    }
+    c.nbr_cap_n = n;
+    c.nbr_cap_k = k;
+    return 0;
+}
+
+SweepLaunch sweep_base(Ctx& c, const grasph_particles* p)
+{
+    SweepLaunch L;
+    std::memset(&L, 0, sizeof L);
+    L.posm = (const float4*)p->posm;
+    L.perm = order_of(c);
+    L.n = (int)p->n;
+    L.acc = (float4*)p->acc;
+    L.density = p->density;
+    L.nbr = c.nbr;
+    L.nbr_cnt = c.nbr_cnt;
+    L.pot = c.pot;
+    L.kmax = c.max_neighbours;
+    L.vel = (const float4*)p->vel;
+    L.xvel = (float4*)p->xvel;
+    L.maxvsig = p->maxvsig;
+    return L;
+}
+
+NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
+{
+    NbrLaunch L;
+    std::memset(&L, 0, sizeof L);
+    L.posm = (const float4*)p->posm;
+    L.vel = (const float4*)p->vel;
+    L.density = p->density;
+    L.dstress = p->dstress;
+    L.balsara_in = p->balsara;
+    L.perm = order_of(c);
+    L.nbr = c.nbr;
+    L.nbr_cnt = c.nbr_cnt;
+    L.n = (int)p->n;
+    L.kmax = c.max_neighbours;
+    L.balsara = p->balsara;
+    L.density_dt = p->density_dt;
+    L.dstress_dt = p->dstress_dt;
+    L.jrec = c.jrec;
+    L.acc = (float4*)p->acc;
+    L.xvel = (float4*)p->xvel;
+    L.maxvsig = p->maxvsig;
+    return L;
+}
+
+int run_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s)
+{
+    ProfScope ps(&c, s, KC_SWEEP);
+    hipError_t e = launch_sweep(L, P, s);
+    if (e != hipSuccess) return hip_fail(c, e, "sweep launch");
+    return 0;
+}
+
+// which == 0: full computeDerivatives; 1: calcDensity; 2: calcBalsara...; 3: calcAcceleration
+int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream, int which)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    int r;
+    if ((r = check_params(c, prm)) || (r = check_particles(c, p))) return r;
+    if (p->n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t f = prm->flags;
+    const bool sph = f & GRASPH_SPH, grav = f & GRASPH_SELF_GRAVITY;
+    const bool dens_pass = (which == 0) ? (sph && !(f & GRASPH_INTEGRATE_DENSITY)) : (which == 1);
+    const bool strain_pass =
+        (which == 0) ? (sph && ((f & GRASPH_SOLIDS) || (f & GRASPH_INTEGRATE_DENSITY) || (f & GRASPH_BALSARA_SWITCH)))
+                     : (which == 2);
+    const bool accel_pass = (which == 0) || (which == 3);
+    const bool need_list = (which == 0 || which == 3) ? sph : (which == 2);
+    if ((which == 1 || which == 2) && !(prm->H > 0.0f)) return fail(c, GRASPH_E_INVALID, "SPH passes need H > 0");
+
+    // required arrays
+    if (accel_pass) NEED(p->acc, "acc");
+    if (dens_pass) NEED(p->density, "density");
+    if (strain_pass) {
+        NEED(p->vel, "vel");
+        NEED(p->density, "density");
+        if (f & GRASPH_SOLIDS) { NEED(p->dstress, "dstress"); }
+        if (f & GRASPH_SOLIDS) { NEED(p->dstress_dt, "dstress_dt"); }
+        if (f & GRASPH_BALSARA_SWITCH) { NEED(p->balsara, "balsara"); }
+        if (f & GRASPH_INTEGRATE_DENSITY) { NEED(p->density_dt, "density_dt"); }
+    }
+    if (accel_pass && sph) {
+        NEED(p->vel, "vel");
+        NEED(p->density, "density");
+        if (f & GRASPH_SOLIDS) { NEED(p->dstress, "dstress"); }
+        if ((f & GRASPH_BALSARA_SWITCH) && which == 3) { NEED(p->balsara, "balsara"); }
+        if (f & GRASPH_XSPH) { NEED(p->xvel, "xvel"); }
+    }
+    if (accel_pass && (f & GRASPH_CLOHESSY_WILTSHIRE)) NEED(p->vel, "vel");
+
+    const int64_t n = p->n;
+    if ((r = ensure_order(c, p, s, true))) return r;
+    if (need_list) {
+        if ((r = ensure_nbr(c, n))) return r;
+        c.last_list_n = n;
+    }
+    if ((accel_pass && sph) || strain_pass) {
+        if ((r = grow(c, c.jrec, c.jrec_cap, 2 * n, "pass record"))) return r;
+    }
+    const DevParams P = make_dev_params(prm);
+
+    // one all-pairs sweep
+    SweepLaunch L = sweep_base(c, p);
+    L.grav = accel_pass && grav;
+    L.dens = dens_pass;
+    L.want_nbr = need_list;
+    L.want_pot = false;
+    L.finish = accel_pass && !sph;
+    if ((r = run_sweep(c, L, P, s))) return r;
+
+    NbrLaunch N = nbr_base(c, p);
+    if (strain_pass) {
+        ProfScope ps(&c, s, KC_STRAIN);
+        hipError_t e = launch_strain(N, P, s);
+        if (e != hipSuccess) return hip_fail(c, e, "strain pass launch");
+    }
+    if (accel_pass && sph) {
+        if (!strain_pass) {
+            ProfScope ps(&c, s, KC_OTHER);
+            hipError_t e = launch_prep(N, P, s);
+            if (e != hipSuccess) return hip_fail(c, e, "record pass launch");
+        }
+        N.acc_has_gravity = grav;
+        ProfScope ps(&c, s, KC_ACCEL);
+        hipError_t e = launch_accel(N, P, s);
+        if (e != hipSuccess) return hip_fail(c, e, "acceleration pass launch");
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace grasph
+
+using grasph::Ctx;
+
+extern "C" {
+
+int grasph_version(void) { return GRASPH_VERSION; }
+
+const char* grasph_last_error(const grasph_ctx* ctx)
+{
+    return ctx ? ctx->c.err.c_str() : grasph::g_create_err.c_str();
+}
+
+void grasph_params_default(grasph_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    // settings/settings.h with FIXED_TIMESTEP_LEAPFROG in place of the
+    // variable-step integrator (:49-62)
+    p->flags = GRASPH_SELF_GRAVITY | GRASPH_SPH | GRASPH_INTEGRATE_DENSITY | GRASPH_SOLIDS | GRASPH_PLASTICITY_MC |
+               GRASPH_ARTIFICIAL_VISCOSITY | GRASPH_BALSARA_SWITCH;
+    p->dimension = 3;
+    const double count = 16384.0, radius = 0.1, tmass = 1.0;
+    const double pradius = radius * std::pow(count, -1.0 / 3.0);  // :97-98
+    p->H = (float)(pradius * 2.5);                                // :99
+    const double rho0 = tmass / count / (4.0 / 3.0 * pradius * pradius * pradius * M_PI);  // :105
+    p->rho0 = (float)rho0;
+    p->BULK = 1024.0f;
+    p->dBULKdP = 1.0f;
+    p->SOUNDSPEED = (float)std::sqrt(1024.0 / rho0);  // :108
+    p->shear = 128.0f;
+    p->Y = 0.5f;
+    p->tanfr = (float)std::tan(45.0 * M_PI / 180.0);  // :121-122
+    p->cohesion = 0.0f;
+    p->alpha = 1.0f;
+    p->mateps = 0.4f;
+    p->matexp = 4.0f;
+    p->normalsep = p->H * 0.3f;
+    p->xsph_factor = 0.5f;
+    p->cw_n = (float)std::sqrt(1.0 / 3.0);
+    p->constant_acceleration[1] = -1.0f;
+    for (int d = 0; d < 3; ++d) {
+        p->box_min[d] = -1.2f;
+        p->box_max[d] = 1.2f;
+    }
+    p->box_reflectiveness = 0.5f;
+    p->fixed_timestep = 0.002f;
+}
+
+int grasph_create(int device, grasph_ctx** out)
+{
+    if (!out) {
+        grasph::g_create_err = "out is NULL";
+        return GRASPH_E_INVALID;
+    }
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        grasph::g_create_err = std::string("no HIP device: ") + hipGetErrorString(e);
+        (void)hipGetLastError();
+        return GRASPH_E_HIP;
+    }
+    if (device < 0 || device >= count) {
+        grasph::g_create_err = "device index out of range";
+        return GRASPH_E_INVALID;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        grasph::g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return GRASPH_E_HIP;
+    }
+    grasph_ctx* ctx = new (std::nothrow) grasph_ctx;
+    if (!ctx) {
+        grasph::g_create_err = "out of host memory";
+        return GRASPH_E_NOMEM;
+    }
+    ctx->c.device = device;
+    ctx->c.err = "no error";
+    *out = ctx;
+    return 0;
+}
+
+int grasph_destroy(grasph_ctx* ctx)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    hipDeviceSynchronize();
+    void* bufs[] = {c.perm, c.keys, c.keys_alt, c.vals, c.sort_tmp, c.bbox, c.nbr, c.nbr_cnt, c.jrec, c.pot, c.red};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (auto& ev : c.ev_pending) {
+        hipEventDestroy(ev.a);
+        hipEventDestroy(ev.b);
+    }
+    for (auto e : c.ev_free) hipEventDestroy(e);
+    delete ctx;
+    return 0;
+}
+
+int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    if (!key) return grasph::fail(c, GRASPH_E_INVALID, "key is NULL");
+    const std::string k(key);
+    if (k == "max_neighbours") {
+        if (value < 1 || value > 4096) return grasph::fail(c, GRASPH_E_INVALID, "max_neighbours must be in [1, 4096]");
+        c.max_neighbours = (int)value;
+    } else if (k == "reorder_interval") {
+        if (value < 0) return grasph::fail(c, GRASPH_E_INVALID, "reorder_interval must be >= 0");
+        c.reorder_interval = (int)(value > 1000000000 ? 1000000000 : value);
+        c.perm_n = 0;
+    } else if (k == "profile") {
+        c.profile = value != 0;
+    } else {
+        return grasph::fail(c, GRASPH_E_INVALID, "unknown option '" + k + "'");
+    }
+    return 0;
+}
+
+int grasph_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
+{
+    return grasph::derivatives(ctx, p, prm, stream, 0);
+}
+int grasph_calc_density(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
+{
+    return grasph::derivatives(ctx, p, prm, stream, 1);
+}
+int grasph_calc_balsara_density_dt_dstress_dt(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm,
+                                              void* stream)
+{
+    return grasph::derivatives(ctx, p, prm, stream, 2);
+}
+int grasph_calc_acceleration(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
+{
+    return grasph::derivatives(ctx, p, prm, stream, 3);
+}
+
+int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, int not_first_step,
+                     void* stream)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    int r;
+    if ((r = grasph::check_params(c, prm)) || (r = grasph::check_particles(c, p))) return r;
+    if (p->n == 0) return 0;
+    const uint32_t f = prm->flags;
+    NEED(p->vel, "vel");
+    NEED(p->acc, "acc");
+    if ((f & GRASPH_SPH) && (f & GRASPH_INTEGRATE_DENSITY)) {
+        NEED(p->density, "density");
+        NEED(p->density_dt, "density_dt");
+    }
+    if ((f & GRASPH_SPH) && (f & GRASPH_SOLIDS)) {
+        NEED(p->dstress, "dstress");
+        NEED(p->dstress_dt, "dstress_dt");
+        if (f & GRASPH_PLASTICITY_MC) { NEED(p->density, "density"); }
+    }
+    if ((f & GRASPH_SPH) && (f & GRASPH_XSPH)) NEED(p->xvel, "xvel");
+    grasph::IntegLaunch L;
+    L.posm = (float4*)p->posm;
+    L.pos64 = p->pos64;
+    L.vel = (float4*)p->vel;
+    L.acc = (const float4*)p->acc;
+    L.xvel = (const float4*)p->xvel;
+    L.density = p->density;
+    L.density_dt = p->density_dt;
+    L.dstress = p->dstress;
+    L.dstress_dt = p->dstress_dt;
+    L.n = (int)p->n;
+    L.not_first_step = not_first_step;
+    hipStream_t s = (hipStream_t)stream;
+    grasph::ProfScope ps(&c, s, grasph::KC_INTEGRATE);
+    hipError_t e = grasph::launch_integrate(L, grasph::make_dev_params(prm), s);
+    if (e != hipSuccess) return grasph::hip_fail(c, e, "leapfrog launch");
+    return 0;
+}
+
+double grasph_get_current_timestep(const grasph_ctx* ctx, const grasph_params* prm)
+{
+    (void)ctx;
+    return prm ? (double)prm->fixed_timestep : 0.0;
+}
+
+int grasph_reduce_invariants(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm,
+                             double half_kick_dt, int with_potential, double out[GRASPH_NINV], void* stream)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    int r;
+    if ((r = grasph::check_params(c, prm)) || (r = grasph::check_particles(c, p))) return r;
+    if (!out) return grasph::fail(c, GRASPH_E_INVALID, "out is NULL");
+    for (int k = 0; k < GRASPH_NINV; ++k) out[k] = 0.0;
+    if (p->n == 0) return 0;
+    NEED(p->vel, "vel");
+    if (half_kick_dt != 0.0) NEED(p->acc, "acc");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = p->n;
+    if (with_potential) {
+        if ((r = grasph::grow(c, c.pot, c.pot_cap, n, "potential"))) return r;
+        grasph::SweepLaunch L = grasph::sweep_base(c, p);
+        L.perm = nullptr;
+        L.pot = c.pot;
+        L.want_pot = true;
+        if ((r = grasph::run_sweep(c, L, grasph::make_dev_params(prm), s))) return r;
+    }
+    const int nb = grasph::inv_blocks((int)n);
+    if ((r = grasph::grow(c, c.red, c.red_cap, (int64_t)(nb + 1) * GRASPH_NINV, "reduction partials"))) return r;
+    grasph::InvLaunch I;
+    I.posm = (const float4*)p->posm;
+    I.pos64 = p->pos64;
+    I.vel = (const float4*)p->vel;
+    I.acc = (const float4*)p->acc;
+    I.density = p->density;
+    I.dstress = p->dstress;
+    I.pot = with_potential ? c.pot : nullptr;
+    I.n = (int)n;
+    I.half_kick = half_kick_dt;
+    I.partials = c.red;
+    I.nblocks = nb;
+    {
+        grasph::ProfScope ps(&c, s, grasph::KC_OTHER);
+        hipError_t e = grasph::launch_invariants(I, *prm, s);
+        if (e != hipSuccess) return grasph::hip_fail(c, e, "invariants launch");
+    }
+    hipError_t e = hipMemcpyAsync(out, c.red + (size_t)nb * GRASPH_NINV, GRASPH_NINV * sizeof(double),
+                                  hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return grasph::hip_fail(c, e, "invariants copy");
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return grasph::hip_fail(c, e, "invariants synchronise");
+    return 0;
+}
+
+int grasph_get_kernel_times(grasph_ctx* ctx, double ms[GRASPH_NKERNEL_CLASSES],
+                            int64_t launches[GRASPH_NKERNEL_CLASSES], int reset)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    for (auto& ev : c.ev_pending) {
+        hipError_t e = hipEventSynchronize(ev.b);
+        if (e != hipSuccess) return grasph::hip_fail(c, e, "event synchronise");
+        float t = 0.0f;
+        e = hipEventElapsedTime(&t, ev.a, ev.b);
+        if (e != hipSuccess) return grasph::hip_fail(c, e, "event elapsed time");
+        c.ms[ev.cls] += t;
+        c.launches[ev.cls] += 1;
+        c.ev_free.push_back(ev.a);
+        c.ev_free.push_back(ev.b);
+    }
+    c.ev_pending.clear();
+    for (int k = 0; k < GRASPH_NKERNEL_CLASSES; ++k) {
+        if (ms) ms[k] = c.ms[k];
+        if (launches) launches[k] = c.launches[k];
+        if (reset) {
+            c.ms[k] = 0.0;
+            c.launches[k] = 0;
+        }
+    }
+    return 0;
+}
+
+int grasph_get_neighbour_stats(grasph_ctx* ctx, int64_t* pairs, int32_t* max_count, int64_t* overflowed)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    if (pairs) *pairs = 0;
+    if (max_count) *max_count = 0;
+    if (overflowed) *overflowed = 0;
+    if (!c.nbr_cnt || c.last_list_n <= 0) return 0;
+    std::vector<int> cnt((size_t)c.last_list_n);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(cnt.data(), c.nbr_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return grasph::hip_fail(c, e, "neighbour counts copy");
+    int64_t tot = 0, over = 0;
+    int32_t mx = 0;
+    for (int v : cnt) {
+        tot += v;
+        if (v > mx) mx = v;
+        if (v > c.max_neighbours) ++over;
+    }
+    if (pairs) *pairs = tot;
+    if (max_count) *max_count = mx;
+    if (overflowed) *overflowed = over;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+// integrate.hip -- fixed-step leapfrog kick-drift with plasticity, and the
+// FP64 global invariants (gfx950).
+//
+// leapfrog: fixedTsLeapfrog::do_for_each (src/integration.h:82-125) with
+// doPlasticity (:24-31) and handleBoundaryConditions (:33-73); one particle
+// per lane, every array read and written once, coalesced (HBM-bound).  With
+// FP64 master positions the drift runs in FP64 and the FP32 mirror is
+// rewritten from it; without them it drifts the FP32 positions as the
+// reference does.
+#include "internal.hpp"
+
+namespace grasph {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ posm, double* __restrict__ pos64,
+                                                          float4* __restrict__ vel, const float4* __restrict__ acc,
+                                                          const float4* __restrict__ xvel,
+                                                          float* __restrict__ density,
+                                                          const float* __restrict__ density_dt,
+                                                          float* __restrict__ dstress,
+                                                          const float* __restrict__ dstress_dt, const int n,
+                                                          const int not_first_step, const DevParams P)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const size_t ns = (size_t)n;
+    const float dt = P.dt;
+    const bool sph = P.flags & GRASPH_SPH;
+
+    const float4 a = acc[i];
+    float4 v = vel[i];
+    // integration.h:95
+    const float kick = dt * ((not_first_step) ? 1.0f : 0.5f);
+    v.x = v.x + a.x * kick;
+    v.y = v.y + a.y * kick;
+    v.z = v.z + a.z * kick;
+
+    // integration.h:98-102
+    float ux = v.x, uy = v.y, uz = v.z;
+    if ((P.flags & GRASPH_XSPH) && sph) {
+        const float4 xv = xvel[i];
+        ux = v.x + P.xsph * xv.x;
+        uy = v.y + P.xsph * xv.y;
+        uz = v.z + P.xsph * xv.z;
+    }
+    float4 p = posm[i];
+    double X = 0, Y = 0, Z = 0;
+    if (pos64) {
+        X = pos64[i] + (double)ux * (double)dt;
+        Y = pos64[ns + i] + (double)uy * (double)dt;
+        Z = pos64[2 * ns + i] + (double)uz * (double)dt;
+        p.x = (float)X; p.y = (float)Y; p.z = (float)Z;
+    } else {
+        p.x = p.x + ux * dt;
+        p.y = p.y + uy * dt;
+        p.z = p.z + uz * dt;
+    }
+
+    float rho = 0.0f;
+    if (sph) {
+        if (P.flags & GRASPH_INTEGRATE_DENSITY) {  // :105-110
+            rho = density[i] + density_dt[i] * dt;
+            if (rho < 0.0f) rho = 0.0f;
+            density[i] = rho;
+        } else if (P.flags & GRASPH_PLASTICITY_MC) {
+            rho = density[i];
+        }
+        if (P.flags & GRASPH_SOLIDS) {  // :111-117
+            Sym6 S;
+            S.xx = dstress[0 * ns + i] + dstress_dt[0 * ns + i] * dt;
+            S.yy = dstress[1 * ns + i] + dstress_dt[1 * ns + i] * dt;
+            S.zz = dstress[2 * ns + i] + dstress_dt[2 * ns + i] * dt;
+            S.xy = dstress[3 * ns + i] + dstress_dt[3 * ns + i] * dt;
+            S.xz = dstress[4 * ns + i] + dstress_dt[4 * ns + i] * dt;
+            S.yz = dstress[5 * ns + i] + dstress_dt[5 * ns + i] * dt;
+            if (P.flags & GRASPH_PLASTICITY_MC)
+                plasticity(S, P.tanfr * eos_murnaghan(rho, P.rho0, P.BULK, P.dBULKdP) + P.cohesion);
+            else if (P.flags & GRASPH_PLASTICITY_MIESE)
+                plasticity(S, P.Y);
+            dstress[0 * ns + i] = S.xx; dstress[1 * ns + i] = S.yy; dstress[2 * ns + i] = S.zz;
+            dstress[3 * ns + i] = S.xy; dstress[4 * ns + i] = S.xz; dstress[5 * ns + i] = S.yz;
+        }
+    }
+
+    if (P.flags & GRASPH_SIMPLE_BOX_BOUNDARY) {  // :33-73
+        const float vm = 1.0f + P.refl;
+        bool moved = false;
+        if (p.x > P.bmaxx) { p.x = P.bmaxx; v.x -= vm * v.x; X = p.x; moved = true; }
+        else if (p.x < P.bminx) { p.x = P.bminx; v.x -= vm * v.x; X = p.x; moved = true; }
+        if (p.y > P.bmaxy) { p.y = P.bmaxy; v.y -= vm * v.y; Y = p.y; moved = true; }
+        else if (p.y < P.bminy) { p.y = P.bminy; v.y -= vm * v.y; Y = p.y; moved = true; }
+        if (P.dim == 3) {
+            if (p.z > P.bmaxz) { p.z = P.bmaxz; v.z -= vm * v.z; Z = p.z; moved = true; }
+            else if (p.z < P.bminz) { p.z = P.bminz; v.z -= vm * v.z; Z = p.z; moved = true; }
+        }
+        (void)moved;
+    }
+    if (pos64) {
+        pos64[i] = X; pos64[ns + i] = Y; pos64[2 * ns + i] = Z;
+    }
+    posm[i] = p;
+    vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// internal energy per unit mass of the EOS, u(rho) = int_{rho0}^{rho} p / rho'^2 drho'
+__device__ inline double u_eos(uint32_t flags, double rho, double r0, double K, double n, double c2)
+{
+    if (!(rho > 0)) return 0.0;
+    if (flags & GRASPH_SOLIDS) {
+        if (fabs(n - 1.0) < 1e-12) return K * (log(rho / r0) / r0 + 1.0 / rho - 1.0 / r0);
+        return K / n * ((pow(rho, n - 1) - pow(r0, n - 1)) / ((n - 1) * pow(r0, n)) + 1.0 / rho - 1.0 / r0);
+    }
+    return c2 * (log(rho / r0) + r0 / rho - 1.0);
+}
+
+struct InvArgs {
+    uint32_t flags;
+    double rho0, K, n, c2, shear, half_kick;
+};
+
+__global__ __launch_bounds__(kBlock) void invariants_kernel(const float4* __restrict__ posm,
+                                                            const double* __restrict__ pos64,
+                                                            const float4* __restrict__ vel,
+                                                            const float4* __restrict__ acc,
+                                                            const float* __restrict__ density,
+                                                            const float* __restrict__ dstress,
+                                                            const float* __restrict__ pot, const int n,
+                                                            double* __restrict__ partials, const InvArgs A)
+{
+    double v[GRASPH_NINV];
+#pragma unroll
+    for (int k = 0; k < GRASPH_NINV; ++k) v[k] = 0.0;
+    const size_t ns = (size_t)n;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4 p = posm[i];
+        const double m = p.w;
+        const float4 u = vel[i];
+        double vx = u.x, vy = u.y, vz = u.z;
+        if (acc && A.half_kick != 0.0) {
+            const float4 a = acc[i];
+            vx += (double)a.x * A.half_kick; vy += (double)a.y * A.half_kick; vz += (double)a.z * A.half_kick;
+        }
+        double x = p.x, y = p.y, z = p.z;
+        if (pos64) { x = pos64[i]; y = pos64[ns + i]; z = pos64[2 * ns + i]; }
+        v[GRASPH_INV_EKIN] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        if (pot) v[GRASPH_INV_EPOT] += 0.5 * m * (double)pot[i];
+        if ((A.flags & GRASPH_SPH) && density) {
+            const double rho = density[i];
+            double e = m * u_eos(A.flags, rho, A.rho0, A.K, A.n, A.c2);
+            if ((A.flags & GRASPH_SOLIDS) && dstress && rho > 0) {
+                double ss = 0;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const double s = dstress[c * ns + i];
+                    ss += (c < 3 ? 1.0 : 2.0) * s * s;
+                }
+                e += m * ss / (4.0 * A.shear * rho);
+            }
+            v[GRASPH_INV_EINT] += e;
+        }
+        v[GRASPH_INV_PX] += m * vx; v[GRASPH_INV_PY] += m * vy; v[GRASPH_INV_PZ] += m * vz;
+        v[GRASPH_INV_LX] += m * (y * vz - z * vy);
+        v[GRASPH_INV_LY] += m * (z * vx - x * vz);
+        v[GRASPH_INV_LZ] += m * (x * vy - y * vx);
+        v[GRASPH_INV_MASS] += m;
+    }
+    // fixed-order block reduction: wave shuffle, then the block's waves in order
+    __shared__ double sh[kBlock / 64][GRASPH_NINV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < GRASPH_NINV; ++k) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[wave][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < GRASPH_NINV) {
+        double x = 0;
+        for (int w = 0; w < kBlock / 64; ++w) x += sh[w][threadIdx.x];
+        partials[(size_t)blockIdx.x * GRASPH_NINV + threadIdx.x] = x;
+    }
+}
+
+__global__ void invariants_final(double* __restrict__ partials, const int nblocks)
+{
+    const int k = threadIdx.x;
+    if (k >= GRASPH_NINV) return;
+    double x = 0;
+    for (int b = 0; b < nblocks; ++b) x += partials[(size_t)b * GRASPH_NINV + k];
+    partials[(size_t)nblocks * GRASPH_NINV + k] = x;
+}
+
+}  // namespace
+
+hipError_t launch_integrate(const IntegLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    const int blocks = (L.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(leapfrog_kernel, dim3(blocks), dim3(kBlock), 0, s, L.posm, L.pos64, L.vel, L.acc, L.xvel,
+                       L.density, L.density_dt, L.dstress, L.dstress_dt, L.n, L.not_first_step, P);
+    return hipGetLastError();
+}
+
+int inv_blocks(int n)
+{
+    int b = (n + kBlock - 1) / kBlock;
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    return b;
+}
+
+hipError_t launch_invariants(const InvLaunch& L, const grasph_params& prm, hipStream_t s)
+{
+    InvArgs A;
+    A.flags = prm.flags;
+    A.rho0 = prm.rho0; A.K = prm.BULK; A.n = prm.dBULKdP;
+    A.c2 = (double)prm.SOUNDSPEED * (double)prm.SOUNDSPEED;
+    A.shear = prm.shear;
+    A.half_kick = L.half_kick;
+    hipLaunchKernelGGL(invariants_kernel, dim3(L.nblocks), dim3(kBlock), 0, s, L.posm, L.pos64, L.vel, L.acc,
+                       L.density, L.dstress, L.pot, L.n, L.partials, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(invariants_final, dim3(1), dim3(64), 0, s, L.partials, L.nblocks);
+    return hipGetLastError();
+}
+
+}  // namespace grasph

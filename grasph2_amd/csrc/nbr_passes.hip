@@ -1,0 +1,361 @@
+// nbr_passes.hip -- the two O(N k) neighbour-list passes (gfx950).
+//
+// strain:  calcBalsaraDensityDTDStressDT (src/computeDerivatives.h:96-162)
+//          over the neighbour list built by the sweep; also writes the
+//          per-particle record {sigma/rho^2, balsara, density} that the
+//          acceleration pass reads for its j side, hoisting the per-pair
+//          Murnaghan pow and nine divides of computeDerivatives.h:246-257.
+// prep:    the same record alone, for configurations that skip the strain
+//          pass (computeDerivatives.h:322-324).
+// accel:   the SPH part of calcAcceleration (computeDerivatives.h:237-295)
+//          plus environmentAcceleration (:300-312), added to the gravity the
+//          sweep left in acc with compensated (TwoSum) accumulation.
+//
+// One particle per lane in Morton order (slot t -> i = perm[t]); the list is
+// stored slot-major, nbr[k * n + t], so a wavefront's loads of its k-th
+// neighbours are contiguous.  Lists are in increasing j, the reference's
+// summation order.  A particle whose count exceeds the list capacity falls
+// back to a scan over all j with the exact neighbour test, so a full list
+// never loses partners.
+#include "internal.hpp"
+
+namespace grasph {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Tens {  // full 3x3, row-major, as mpu::Mat<float,3,3>
+    float m[3][3];
+};
+
+__device__ inline Tens sym_to_full(float xx, float yy, float zz, float xy, float xz, float yz)
+{
+    Tens t;
+    t.m[0][0] = xx; t.m[1][1] = yy; t.m[2][2] = zz;
+    t.m[0][1] = xy; t.m[1][0] = xy;
+    t.m[0][2] = xz; t.m[2][0] = xz;
+    t.m[1][2] = yz; t.m[2][1] = yz;
+    return t;
+}
+
+// src/sph/models.h:250-268, for the unique (d,e) of a symmetric result
+__device__ inline float dsdt_entry(const Tens& ed, const Tens& rd, const Tens& S, float shear, int d, int e)
+{
+    float o = 2.0f * shear * ed.m[d][e];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        if (d == e) o -= 2.0f * shear * ed.m[f][f] / 3.0f;
+        o += S.m[d][f] * rd.m[e][f];
+        o += S.m[e][f] * rd.m[d][f];
+    }
+    return o;
+}
+
+__device__ inline float ld(const float* p, size_t k) { return p[k]; }
+
+template <bool SOLIDS, bool BALS, bool INTD>
+__global__ __launch_bounds__(kBlock) void strain_kernel(
+    const float4* __restrict__ posm, const float4* __restrict__ vel, const float* __restrict__ density,
+    const float* __restrict__ dstress, const int* __restrict__ perm, const int* __restrict__ nbr,
+    const int* __restrict__ nbr_cnt, const int n, const int kmax, float* __restrict__ balsara,
+    float* __restrict__ density_dt, float* __restrict__ dstress_dt, float4* __restrict__ jrec, const DevParams P)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int i = perm ? perm[t] : t;
+    const size_t ns = (size_t)n;
+    const float4 pi = posm[i];
+    const float4 vi = vel[i];
+    const float rhoi = density[i];
+
+    float exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;  // strain rate (symmetric)
+    float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
+    float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+
+    auto body = [&](int j) {
+        const float4 pj = posm[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float r2 = dist2_exact(rx, ry, rz);
+        const float r = sqrtf(r2);
+        const float dw = dw_spline(r, P.H, P.dWpref);
+        const float s = dw / r;
+        const float gx = s * rx, gy = s * ry, gz = s * rz;
+        const float4 vj = vel[j];
+        const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+        const float mj = pj.w, rhoj = density[j];
+        if (SOLIDS) {  // models.h:225-248
+            const float tmp = -0.5f * mj / rhoj;
+            exx += tmp * (vx * gx + vx * gx);
+            exy += tmp * (vx * gy + vy * gx);
+            exz += tmp * (vx * gz + vz * gx);
+            eyy += tmp * (vy * gy + vy * gy);
+            eyz += tmp * (vy * gz + vz * gy);
+            ezz += tmp * (vz * gz + vz * gz);
+            rxy += tmp * (vx * gy - vy * gx);
+            rxz += tmp * (vx * gz - vz * gx);
+            ryz += tmp * (vy * gz - vz * gy);
+        } else if (BALS) {  // curl v, computeDerivatives.h:134
+            const float f = mj / rhoj;
+            cx += f * (vy * gz - vz * gy);
+            cy += f * (vz * gx - vx * gz);
+            cz += f * (vx * gy - vy * gx);
+        }
+        if (BALS || INTD) divv -= (mj / rhoj) * (vx * gx + vy * gy + vz * gz);  // :137
+    };
+
+    const int cnt = nbr_cnt[t];
+    if (cnt <= kmax) {
+        for (int k = 0; k < cnt; ++k) body(nbr[(size_t)k * ns + t]);
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const float4 pj = posm[j];
+            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), P.H2)) body(j);
+        }
+    }
+
+    // do_after, computeDerivatives.h:144-161
+    Tens S = sym_to_full(0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
+    if (SOLIDS)
+        S = sym_to_full(ld(dstress, 0 * ns + i), ld(dstress, 1 * ns + i), ld(dstress, 2 * ns + i),
+                        ld(dstress, 3 * ns + i), ld(dstress, 4 * ns + i), ld(dstress, 5 * ns + i));
+    float sdt[6] = {0, 0, 0, 0, 0, 0};
+    if (SOLIDS) {
+        const Tens ed = sym_to_full(exx, eyy, ezz, exy, exz, eyz);
+        Tens rd;
+        rd.m[0][0] = 0.0f; rd.m[1][1] = 0.0f; rd.m[2][2] = 0.0f;
+        rd.m[0][1] = rxy; rd.m[1][0] = -rxy;
+        rd.m[0][2] = rxz; rd.m[2][0] = -rxz;
+        rd.m[1][2] = ryz; rd.m[2][1] = -ryz;
+        sdt[0] = dsdt_entry(ed, rd, S, P.shear, 0, 0);
+        sdt[1] = dsdt_entry(ed, rd, S, P.shear, 1, 1);
+        sdt[2] = dsdt_entry(ed, rd, S, P.shear, 2, 2);
+        sdt[3] = dsdt_entry(ed, rd, S, P.shear, 0, 1);
+        sdt[4] = dsdt_entry(ed, rd, S, P.shear, 0, 2);
+        sdt[5] = dsdt_entry(ed, rd, S, P.shear, 1, 2);
+        // curl from the rotation rate, :150
+        cx = -2.0f * rd.m[1][2];
+        cy = -2.0f * rd.m[2][0];
+        cz = -2.0f * rd.m[0][1];
+    }
+    float bal = 0.0f;
+    if (BALS) bal = balsara_switch(divv, cx, cy, cz, P.c, P.H);
+    if (balsara) balsara[i] = bal;
+    if (density_dt) density_dt[i] = INTD ? -rhoi * divv : 0.0f;
+    if (dstress_dt) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dstress_dt[c * ns + i] = sdt[c];
+    }
+
+    // j-side record of the acceleration pass (its do_before, :193-219)
+    if (SOLIDS) {
+        const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
+        const float d2 = rhoi * rhoi;
+        jrec[2 * (size_t)i] = make_float4((S.m[0][0] - pres) / d2, (S.m[1][1] - pres) / d2, (S.m[2][2] - pres) / d2,
+                                          S.m[0][1] / d2);
+        jrec[2 * (size_t)i + 1] = make_float4(S.m[0][2] / d2, S.m[1][2] / d2, bal, rhoi);
+    } else {
+        const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
+        jrec[2 * (size_t)i] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
+        jrec[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
+    }
+}
+
+// record only (no strain pass in this configuration)
+__global__ __launch_bounds__(kBlock) void prep_kernel(const float* __restrict__ density,
+                                                      const float* __restrict__ dstress,
+                                                      const float* __restrict__ balsara, const int n,
+                                                      float4* __restrict__ jrec, const int solids, const DevParams P)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const size_t ns = (size_t)n;
+    const float rhoi = density[i];
+    const float bal = balsara ? balsara[i] : 0.0f;
+    if (solids) {
+        const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
+        const float d2 = rhoi * rhoi;
+        jrec[2 * (size_t)i] = make_float4((dstress[0 * ns + i] - pres) / d2, (dstress[1 * ns + i] - pres) / d2,
+                                          (dstress[2 * ns + i] - pres) / d2, dstress[3 * ns + i] / d2);
+        jrec[2 * (size_t)i + 1] = make_float4(dstress[4 * ns + i] / d2, dstress[5 * ns + i] / d2, bal, rhoi);
+    } else {
+        const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
+        jrec[2 * (size_t)i] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
+        jrec[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
+    }
+}
+
+template <bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
+__global__ __launch_bounds__(kBlock) void accel_kernel(
+    const float4* __restrict__ posm, const float4* __restrict__ vel, const float4* __restrict__ jrec,
+    const int* __restrict__ perm, const int* __restrict__ nbr, const int* __restrict__ nbr_cnt, const int n,
+    const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
+    const int acc_has_gravity, const DevParams P)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int i = perm ? perm[t] : t;
+    const size_t ns = (size_t)n;
+    const float4 pi = posm[i];
+    const float4 vi = vel[i];
+    const float4 ra = jrec[2 * (size_t)i], rb = jrec[2 * (size_t)i + 1];
+    const float rhoi = rb.w, bi = BALS ? rb.z : 1.0f;
+    // sigma_i / rho_i^2 and, for liquids with artificial pressure, arts_i
+    const float sxx = ra.x, syy = ra.y, szz = ra.z, sxy = ra.w, sxz = rb.x, syz = rb.y;
+    const float arts_i = (ARTS && !SOLIDS) ? ((sxx > 0) ? (-P.mateps * sxx) : 0.0f) : 0.0f;
+
+    CSum ax{0.0f, 0.0f}, ay{0.0f, 0.0f}, az{0.0f, 0.0f};
+    if (acc_has_gravity) {
+        const float4 g = acc[i];
+        ax.s = g.x; ay.s = g.y; az.s = g.z;
+    }
+    float mv = 0.0f;
+    float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
+
+    auto body = [&](int j) {
+        const float4 pj = posm[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float r2 = dist2_exact(rx, ry, rz);
+        const float r = sqrtf(r2);
+        const float dw = dw_spline(r, P.H, P.dWpref);
+        const float s = dw / r;
+        const float gx = s * rx, gy = s * ry, gz = s * rz;
+        const float4 qa = jrec[2 * (size_t)j], qb = jrec[2 * (size_t)j + 1];
+        const float rhoj = qb.w;
+        const float mj = pj.w;
+        float fx, fy, fz;
+        if (SOLIDS) {  // acc += m_j (stress . gradw), stress = sOR_i + sOR_j  (:260, :275)
+            const float txx = sxx + qa.x, tyy = syy + qa.y, tzz = szz + qa.z;
+            const float txy = sxy + qa.w, txz = sxz + qb.x, tyz = syz + qb.y;
+            fx = mj * (txx * gx + txy * gy + txz * gz);
+            fy = mj * (txy * gx + tyy * gy + tyz * gz);
+            fz = mj * (txz * gx + tyz * gy + tzz * gz);
+        } else {
+            float st = sxx + qa.x;
+            if (ARTS) {  // :263-272 (liquid artificial pressure)
+                const float f = powf(w_spline(r, P.H, P.Wpref) / P.Wnorm, P.matexp);
+                const float arts_j = (qa.x > 0) ? (-P.mateps * qa.x) : 0.0f;
+                st += f * (arts_i + arts_j);
+            }
+            fx = mj * (st * gx);
+            fy = mj * (st * gy);
+            fz = mj * (st * gz);
+        }
+        const float4 vj = vel[j];
+        const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+        if (AV) {  // :277-289
+            const float II = artificial_viscosity(BALS ? &mv : nullptr, P.alpha, rhoi, rhoj, vx, vy, vz, rx, ry, rz, r,
+                                                  P.c, P.c, bi, BALS ? qb.z : 1.0f);
+            const float mII = mj * II;
+            fx -= mII * gx;
+            fy -= mII * gy;
+            fz -= mII * gz;
+        }
+        ax.add(fx);
+        ay.add(fy);
+        az.add(fz);
+        if (XSPH) {  // :291-294
+            const float w = w_spline(r, P.H, P.Wpref);
+            const float f = 2.0f * mj / (rhoi + rhoj);
+            xvx += f * (vj.x - vi.x) * w;
+            xvy += f * (vj.y - vi.y) * w;
+            xvz += f * (vj.z - vi.z) * w;
+        }
+    };
+
+    const int cnt = nbr_cnt[t];
+    if (cnt <= kmax) {
+        for (int k = 0; k < cnt; ++k) body(nbr[(size_t)k * ns + t]);
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const float4 pj = posm[j];
+            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), P.H2)) body(j);
+        }
+    }
+
+    float fx = ax.value(), fy = ay.value(), fz = az.value();
+    environment_acceleration(P, pi.x, pi.y, pi.z, vi.x, vi.y, vi.z, fx, fy, fz);
+    acc[i] = make_float4(fx, fy, fz, 0.0f);
+    if (maxvsig) maxvsig[i] = mv;
+    if (xvel) xvel[i] = make_float4(xvx, xvy, xvz, 0.0f);
+}
+
+}  // namespace
+
+hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
+               in = P.flags & GRASPH_INTEGRATE_DENSITY;
+    const int blocks = (L.n + kBlock - 1) / kBlock;
+#define GRASPH_STRAIN(A, B, C)                                                                                       \
+    hipLaunchKernelGGL((strain_kernel<A, B, C>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.vel, L.density,          \
+                       L.dstress, L.perm, L.nbr, L.nbr_cnt, L.n, L.kmax, L.balsara, L.density_dt, L.dstress_dt, L.jrec, \
+                       P)
+    const int key = (so ? 4 : 0) | (ba ? 2 : 0) | (in ? 1 : 0);
+    switch (key) {
+        case 0: GRASPH_STRAIN(false, false, false); break;
+        case 1: GRASPH_STRAIN(false, false, true); break;
+        case 2: GRASPH_STRAIN(false, true, false); break;
+        case 3: GRASPH_STRAIN(false, true, true); break;
+        case 4: GRASPH_STRAIN(true, false, false); break;
+        case 5: GRASPH_STRAIN(true, false, true); break;
+        case 6: GRASPH_STRAIN(true, true, false); break;
+        default: GRASPH_STRAIN(true, true, true); break;
+    }
+#undef GRASPH_STRAIN
+    return hipGetLastError();
+}
+
+hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    const int blocks = (L.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(prep_kernel, dim3(blocks), dim3(kBlock), 0, s, L.density, L.dstress, L.balsara_in, L.n, L.jrec,
+                       (P.flags & GRASPH_SOLIDS) ? 1 : 0, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_accel(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
+               av = P.flags & GRASPH_ARTIFICIAL_VISCOSITY, xs = P.flags & GRASPH_XSPH,
+               ar = P.flags & GRASPH_ARTIFICIAL_STRESS;
+    const int blocks = (L.n + kBlock - 1) / kBlock;
+#define GRASPH_ACCEL(A, B, C, D, E)                                                                                   \
+    hipLaunchKernelGGL((accel_kernel<A, B, C, D, E>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.vel, L.jrec, L.perm, \
+                       L.nbr, L.nbr_cnt, L.n, L.kmax, L.acc, L.xvel, L.maxvsig, L.acc_has_gravity ? 1 : 0, P)
+    // SOLIDS + ARTIFICIAL_STRESS is rejected before launch (the reference does not compile it).
+    const int key = (so ? 16 : 0) | (ba ? 8 : 0) | (av ? 4 : 0) | (xs ? 2 : 0) | (ar ? 1 : 0);
+    switch (key) {
+        case 0: GRASPH_ACCEL(false, false, false, false, false); break;
+        case 1: GRASPH_ACCEL(false, false, false, false, true); break;
+        case 2: GRASPH_ACCEL(false, false, false, true, false); break;
+        case 3: GRASPH_ACCEL(false, false, false, true, true); break;
+        case 4: GRASPH_ACCEL(false, false, true, false, false); break;
+        case 5: GRASPH_ACCEL(false, false, true, false, true); break;
+        case 6: GRASPH_ACCEL(false, false, true, true, false); break;
+        case 7: GRASPH_ACCEL(false, false, true, true, true); break;
+        case 8: GRASPH_ACCEL(false, true, false, false, false); break;
+        case 9: GRASPH_ACCEL(false, true, false, false, true); break;
+        case 10: GRASPH_ACCEL(false, true, false, true, false); break;
+        case 11: GRASPH_ACCEL(false, true, false, true, true); break;
+        case 12: GRASPH_ACCEL(false, true, true, false, false); break;
+        case 13: GRASPH_ACCEL(false, true, true, false, true); break;
+        case 14: GRASPH_ACCEL(false, true, true, true, false); break;
+        case 15: GRASPH_ACCEL(false, true, true, true, true); break;
+        case 16: GRASPH_ACCEL(true, false, false, false, false); break;
+        case 18: GRASPH_ACCEL(true, false, false, true, false); break;
+        case 20: GRASPH_ACCEL(true, false, true, false, false); break;
+        case 22: GRASPH_ACCEL(true, false, true, true, false); break;
+        case 24: GRASPH_ACCEL(true, true, false, false, false); break;
+        case 26: GRASPH_ACCEL(true, true, false, true, false); break;
+        case 28: GRASPH_ACCEL(true, true, true, false, false); break;
+        case 30: GRASPH_ACCEL(true, true, true, true, false); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef GRASPH_ACCEL
+    return hipGetLastError();
+}
+
+}  // namespace grasph

@@ -1,0 +1,186 @@
+// physics.hpp -- device-side physics of the GraSPH2 hot path (gfx950).
+//
+// Each function states the reference lines whose arithmetic it follows.  The
+// operation order of the reference is kept (so FP32 results track the
+// reference's to a few ulp); hipcc may still contract a*b+c into one FMA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/grasph2.h"
+
+namespace grasph {
+
+// Runtime form of the compile-time constants of settings/settings.h and
+// src/computeDerivatives.h:26-28, all in FP32 as the reference computes them.
+struct DevParams {
+    uint32_t flags;
+    int32_t dim;
+    float H, H2, Wpref, dWpref;
+    float rho0, BULK, dBULKdP, c, c2, shear, Y, tanfr, cohesion, alpha;
+    float mateps, matexp, normalsep, Wnorm, xsph, cw_n, dt;
+    float cax, cay, caz;
+    float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, refl;
+    float cand;  // d2 = r2 + H2 threshold that admits every pair with r2 <= H2
+};
+
+// src/sph/kernel.h:41-60
+__host__ __device__ inline float spline_prefactor(int dim, float h)
+{
+    if (dim == 1) return 4.0f / (3.0f * h);
+    if (dim == 2) return 40.0f / (7.0f * float(M_PI) * h * h);
+    return 8.0f / (float(M_PI) * h * h * h);
+}
+// src/sph/kernel.h:68-87
+__host__ __device__ inline float dspline_prefactor(int dim, float h)
+{
+    if (dim == 1) return 24.0f / (3.0f * h * h);
+    if (dim == 2) return 240.0f / (7.0f * float(M_PI) * h * h * h);
+    return 48.0f / (float(M_PI) * h * h * h * h);
+}
+
+// src/sph/kernel.h:130-136 (cubic B-spline, support radius h)
+__host__ __device__ inline float w_spline(float r, float h, float factor)
+{
+    const float q = r / h;
+    return factor * ((q <= 1.0f) ? ((q < 0.5f) ? (6.0f * q * q * q - 6.0f * q * q + 1.0f)
+                                               : 2.0f * (1.0f - q) * (1.0f - q) * (1.0f - q))
+                                 : 0.0f);
+}
+// src/sph/kernel.h:169-175
+__host__ __device__ inline float dw_spline(float r, float h, float factor)
+{
+    const float q = r / h;
+    return factor * ((q <= 1.0f) ? ((q < 0.5f) ? (3.0f * q * q - 2.0f * q) : -1.0f * (1.0f - q) * (1.0f - q))
+                                 : 0.0f);
+}
+
+// src/sph/eos.h:88-96
+__device__ inline float eos_liquid(float rho, float rho0, float c2) { return c2 * (rho - rho0); }
+__device__ inline float eos_murnaghan(float rho, float rho0, float K0, float dK0)
+{
+    return K0 / dK0 * (powf(rho / rho0, dK0) - 1.0f);
+}
+
+// Exact FP32 squared distance in the reference's dot() order
+// (x*x + y*y) + z*z with every product and sum rounded (no contraction),
+// so the neighbour test r2 <= H2 && r2 > 0 decides exactly as the oracle.
+__device__ inline float dist2_exact(float rx, float ry, float rz)
+{
+    return __fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz));
+}
+
+// Neighbour predicate of src/computeDerivatives.h:72,122,238:  0 < r2 <= H2.
+// As unsigned bit patterns of non-negative floats: r2 in (0, H2]  <=>
+// bits(r2) - 1 < bits(H2); +0 wraps to 0xffffffff and NaN lies above +inf.
+__device__ inline bool is_neighbour(float r2, float H2)
+{
+    return (__float_as_uint(r2) - 1u) < __float_as_uint(H2);
+}
+
+// TwoSum (Knuth): s + e == a + b exactly.  Used for the compensated
+// (Kahan-Babuska) accumulation of acceleration, density and potential.
+struct CSum {
+    float s, c;
+    __device__ inline void add(float x)
+    {
+        const float t = s + x;
+        const float bp = t - s;
+        const float e = (s - (t - bp)) + (x - bp);
+        s = t;
+        c += e;
+    }
+    __device__ inline float value() const { return s + c; }
+};
+
+// src/sph/models.h:172-185 (with maxVsig) / :158-170 (without)
+__device__ inline float artificial_viscosity(float* maxvsig, float alpha, float di, float dj, float vx, float vy,
+                                             float vz, float rx, float ry, float rz, float r, float ci, float cj,
+                                             float bi, float bj)
+{
+    const float wij = (rx * vx + ry * vy + rz * vz) / r;
+    float II = 0.0f;
+    if (wij < 0.0f) {
+        const float vsig = ci + cj - 3.0f * wij;
+        const float rhoij = (di + dj) * 0.5f;
+        II = -0.25f * (bi + bj) * alpha * wij * vsig / rhoij;
+        if (maxvsig) *maxvsig = fmaxf(*maxvsig, vsig);
+    }
+    return II;
+}
+
+// src/sph/models.h:187-192
+__device__ inline float balsara_switch(float divv, float cx, float cy, float cz, float c, float h)
+{
+    const float absdiv = fabsf(divv);
+    const float abscurl = sqrtf(cx * cx + cy * cy + cz * cz);
+    return absdiv / (absdiv + abscurl + 0.0001f * c / h);
+}
+
+// Symmetric tensor, unique components xx yy zz xy xz yz.
+struct Sym6 {
+    float xx, yy, zz, xy, xz, yz;
+};
+
+// src/sph/models.h:194-205 von Mises (the factor is Y^2/(3 J2), applied as is).
+// J2 sums the nine entries in the reference's row-major order.
+__device__ inline void plasticity(Sym6& S, float Y)
+{
+    float J2 = 0.0f;
+    J2 += S.xx * S.xx; J2 += S.xy * S.xy; J2 += S.xz * S.xz;
+    J2 += S.xy * S.xy; J2 += S.yy * S.yy; J2 += S.yz * S.yz;
+    J2 += S.xz * S.xz; J2 += S.yz * S.yz; J2 += S.zz * S.zz;
+    J2 *= 0.5f;
+    const float f = Y * Y / (3.0f * J2);
+    if (f < 1.0f) {
+        S.xx *= f; S.yy *= f; S.zz *= f; S.xy *= f; S.xz *= f; S.yz *= f;
+    }
+}
+
+__host__ inline DevParams make_dev_params(const grasph_params* p)
+{
+    DevParams d;
+    d.flags = p->flags;
+    d.dim = p->dimension;
+    d.H = p->H;
+    d.H2 = p->H * p->H;
+    d.Wpref = spline_prefactor(d.dim, d.H);
+    d.dWpref = dspline_prefactor(d.dim, d.H);
+    d.rho0 = p->rho0; d.BULK = p->BULK; d.dBULKdP = p->dBULKdP;
+    d.c = p->SOUNDSPEED; d.c2 = p->SOUNDSPEED * p->SOUNDSPEED;
+    d.shear = p->shear; d.Y = p->Y; d.tanfr = p->tanfr; d.cohesion = p->cohesion; d.alpha = p->alpha;
+    d.mateps = p->mateps; d.matexp = p->matexp; d.normalsep = p->normalsep;
+    d.Wnorm = w_spline(p->normalsep, d.H, d.Wpref);
+    d.xsph = p->xsph_factor; d.cw_n = p->cw_n; d.dt = p->fixed_timestep;
+    d.cax = p->constant_acceleration[0]; d.cay = p->constant_acceleration[1]; d.caz = p->constant_acceleration[2];
+    d.bminx = p->box_min[0]; d.bminy = p->box_min[1]; d.bminz = p->box_min[2];
+    d.bmaxx = p->box_max[0]; d.bmaxy = p->box_max[1]; d.bmaxz = p->box_max[2];
+    d.refl = p->box_reflectiveness;
+    // d2 = fma-chain(r2) + H2 carries at most a few ulp of relative error, so
+    // pairs with r2 <= H2 all satisfy d2 <= 2 H2 (1 + 2^-18).
+    d.cand = 2.0f * d.H2 * (1.0f + 1.0f / 262144.0f);
+    return d;
+}
+
+// src/computeDerivatives.h:33-47 environmentAcceleration.  The
+// Clohessy-Wiltshire terms add a scalar to all three components, which is
+// the literal meaning of `f3 += f1` in the reference expression.
+__device__ inline void environment_acceleration(const DevParams& P, float px, float py, float pz, float vx, float vy,
+                                                float vz, float& ax, float& ay, float& az)
+{
+    if (P.flags & GRASPH_CLOHESSY_WILTSHIRE) {
+        const float a1 = 3.0f * P.cw_n * P.cw_n * px + 2.0f * P.cw_n * vy;
+        ax += a1; ay += a1; az += a1;
+        const float a2 = -2.0f * P.cw_n * vx;
+        ax += a2; ay += a2; az += a2;
+        const float a3 = -P.cw_n * P.cw_n * pz;
+        ax += a3; ay += a3; az += a3;
+    }
+    if (P.flags & GRASPH_CONSTANT_ACCELERATION) {
+        ax += P.cax; ay += P.cay; az += P.caz;
+    }
+    (void)py;
+}
+
+}  // namespace grasph

@@ -1,0 +1,109 @@
+// reorder.hip -- Morton (Z-order) ordering of the i side (gfx950).
+//
+// The sweep and the neighbour passes map slot t to particle perm[t].  Along
+// a Z-order curve consecutive slots are spatial neighbours, so the 64 lanes
+// of a wavefront share most of their SPH partners: the neighbour branch of
+// the sweep is taken by few j per wavefront and the list gathers hit the
+// same cache lines.  The order only affects speed, never results: every
+// sum still runs over j in index order.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.hpp"
+
+namespace grasph {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kBoxBlocks = 256;
+
+__global__ __launch_bounds__(kBlock) void bbox_partial(const float4* __restrict__ posm, const int n,
+                                                       float* __restrict__ part)
+{
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4 p = posm[i];
+        mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
+    }
+    __shared__ float sh[6][kBlock];
+    for (int d = 0; d < 3; ++d) {
+        sh[d][threadIdx.x] = mn[d];
+        sh[3 + d][threadIdx.x] = mx[d];
+    }
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; ++d) {
+                sh[d][threadIdx.x] = fminf(sh[d][threadIdx.x], sh[d][threadIdx.x + s]);
+                sh[3 + d][threadIdx.x] = fmaxf(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void bbox_final(float* __restrict__ part, const int nparts)
+{
+    const int d = threadIdx.x;
+    if (d >= 6) return;
+    float v = (d < 3) ? INFINITY : -INFINITY;
+    for (int b = 0; b < nparts; ++b) v = (d < 3) ? fminf(v, part[b * 6 + d]) : fmaxf(v, part[b * 6 + d]);
+    part[nparts * 6 + d] = v;
+}
+
+__device__ inline uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void morton_keys(const float4* __restrict__ posm, const int n,
+                                                      const float* __restrict__ box, uint32_t* __restrict__ keys,
+                                                      int* __restrict__ vals)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = posm[i];
+    uint32_t q[3];
+    const float c[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d) {
+        const float lo = box[d], hi = box[3 + d];
+        const float ext = hi - lo;
+        float u = (ext > 0.0f) ? (c[d] - lo) / ext : 0.0f;
+        if (!(u >= 0.0f)) u = 0.0f;  // NaN and negatives
+        if (u > 1.0f) u = 1.0f;
+        q[d] = (uint32_t)(u * 1023.0f);
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = i;
+}
+
+}  // namespace
+
+size_t reorder_tmp_bytes(int n)
+{
+    size_t bytes = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
+                                       (int*)nullptr, n, 0, 30, (hipStream_t)0);
+    return bytes;
+}
+
+hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bbox_partial, dim3(kBoxBlocks), dim3(kBlock), 0, s, posm, n, c.bbox);
+    hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, c.bbox, kBoxBlocks);
+    const int blocks = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(morton_keys, dim3(blocks), dim3(kBlock), 0, s, posm, n, c.bbox + kBoxBlocks * 6, c.keys,
+                       c.vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = c.sort_tmp_bytes;
+    return hipcub::DeviceRadixSort::SortPairs(c.sort_tmp, bytes, c.keys, c.keys_alt, c.vals, c.perm, n, 0, 30, s);
+}
+
+}  // namespace grasph

@@ -1,0 +1,210 @@
+/*
+ * grasph2.h -- C ABI of the MI355X-native GraSPH2 derivative + leapfrog path.
+ *
+ * Plain C: pointers, sizes and integer status codes only; no HIP or C++ types.
+ * Every entry point returns 0 on success and a negative GRASPH_E_* code on
+ * failure; grasph_last_error() then describes the failure.  All device work is
+ * enqueued asynchronously on the stream passed in (a hipStream_t given as
+ * void*, NULL = the null stream); the library never synchronises the device
+ * unless an entry point says so.
+ *
+ * Reference interface replaced (paths relative to the GraSPH2 source tree):
+ *   computeDerivatives(pb)            src/computeDerivatives.h:315-328
+ *   do_for_each_pair_fast<calcDensity>                       :52-91, 320
+ *   do_for_each_pair_fast<calcBalsaraDensityDTDStressDT>     :96-162, 323
+ *   do_for_each_pair_fast<calcAcceleration>                  :167-313, 327
+ *   integrate(pb, notFirstStep)       src/integration.h:219-244
+ *   do_for_each<fixedTsLeapfrog>      src/integration.h:82-125, 224
+ *   getCurrentTimestep()              src/integration.h:246-255
+ *   compile-time physics switches     settings/settings.h:31-171
+ *   DeviceParticleBuffer<DEV_...>     settings/settings.h:184,
+ *                                     src/particles/particle_buffer_impl.h:45-243
+ */
+#ifndef GRASPH2_H
+#define GRASPH2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRASPH_VERSION 10000 /* major*10000 + minor*100 + patch */
+
+/* ---- status codes ------------------------------------------------------ */
+#define GRASPH_OK 0
+#define GRASPH_E_INVALID (-1)     /* bad argument: null pointer, bad size, bad flag combination */
+#define GRASPH_E_HIP (-2)         /* a HIP runtime call failed */
+#define GRASPH_E_UNSUPPORTED (-3) /* combination the reference does not build either */
+#define GRASPH_E_NOMEM (-4)       /* device allocation of library scratch failed */
+
+/* ---- physics switches (settings/settings.h) ---------------------------- */
+#define GRASPH_SELF_GRAVITY          (1u << 0)  /* ENABLE_SELF_GRAVITY       settings.h:34  */
+#define GRASPH_SPH                   (1u << 1)  /* ENABLE_SPH                settings.h:37  */
+#define GRASPH_INTEGRATE_DENSITY     (1u << 2)  /* INTEGRATE_DENSITY         settings.h:40  */
+#define GRASPH_DEAL_WITH_NO_PARTNERS (1u << 3)  /* DEAL_WITH_NO_PARTNERS     settings.h:41  */
+#define GRASPH_SOLIDS                (1u << 4)  /* SOLIDS                    settings.h:42  */
+#define GRASPH_PLASTICITY_MIESE      (1u << 5)  /* PLASTICITY_MIESE          settings.h:116 */
+#define GRASPH_PLASTICITY_MC         (1u << 6)  /* PLASTICITY_MC             settings.h:120 */
+#define GRASPH_CLOHESSY_WILTSHIRE    (1u << 7)  /* CLOHESSY_WILTSHIRE        settings.h:136 */
+#define GRASPH_CONSTANT_ACCELERATION (1u << 8)  /* CONSTANT_ACCELERATION     settings.h:141 */
+#define GRASPH_SIMPLE_BOX_BOUNDARY   (1u << 9)  /* SIMPLE_BOX_BOUNDARY       settings.h:146 */
+#define GRASPH_ARTIFICIAL_VISCOSITY  (1u << 10) /* ARTIFICIAL_VISCOSITY      settings.h:155 */
+#define GRASPH_BALSARA_SWITCH        (1u << 11) /* BALSARA_SWITCH            settings.h:159 */
+#define GRASPH_ARTIFICIAL_STRESS     (1u << 12) /* ARTIFICIAL_STRESS         settings.h:163 */
+#define GRASPH_XSPH                  (1u << 13) /* XSPH                      settings.h:170 */
+#define GRASPH_ALL_FLAGS             ((1u << 14) - 1u)
+
+/*
+ * Runtime replacement of the compile-time constants of settings/settings.h.
+ * Gravity uses G = 1 and the SPH smoothing length H as softening
+ * (src/computeDerivatives.h:231).
+ */
+typedef struct grasph_params {
+    uint32_t flags;          /* GRASPH_* switches above */
+    int32_t dimension;       /* 1, 2 or 3                       settings.h:31  */
+    float H;                 /* smoothing length                settings.h:99  */
+    float rho0;              /* rest density                    settings.h:105 */
+    float BULK;              /* bulk modulus K0                 settings.h:106 */
+    float dBULKdP;           /* dK0/dp (Murnaghan exponent)     settings.h:107 */
+    float SOUNDSPEED;        /* speed of sound c                settings.h:108 */
+    float shear;             /* shear modulus mu                settings.h:111 */
+    float Y;                 /* von Mises yield stress          settings.h:117 */
+    float tanfr;             /* tan(friction angle)             settings.h:122 */
+    float cohesion;          /* Mohr-Coulomb cohesion           settings.h:123 */
+    float alpha;             /* artificial viscosity strength   settings.h:156 */
+    float mateps;            /* artificial stress epsilon       settings.h:164 */
+    float matexp;            /* artificial stress exponent      settings.h:165 */
+    float normalsep;         /* artificial stress separation    settings.h:166 */
+    float xsph_factor;       /* XSPH factor                     settings.h:171 */
+    float cw_n;              /* Clohessy-Wiltshire n            settings.h:137 */
+    float constant_acceleration[3];   /*                        settings.h:142 */
+    float box_min[3];        /*                                 settings.h:147 */
+    float box_max[3];        /*                                 settings.h:148 */
+    float box_reflectiveness;/*                                 settings.h:149 */
+    float fixed_timestep;    /* dt of the fixed-step leapfrog   settings.h:50  */
+} grasph_params;
+
+/* Fill *p with the defaults of settings/settings.h: self-gravity, SPH,
+ * integrated density, solids, Mohr-Coulomb plasticity, artificial viscosity
+ * and the Balsara switch, with the fixed-step leapfrog (dt 0.002) standing in
+ * for the variable-step integrator the reference selects there. */
+void grasph_params_default(grasph_params* p);
+
+/*
+ * Device particle arrays ("DeviceParticleBuffer<DEV_POSM, DEV_VEL, ...>").
+ * Caller-owned device memory; the library never frees it.  Layout:
+ *   posm       float[n][4]   x, y, z, mass            (DEV_POSM)
+ *   pos64      double[3][n]  FP64 master position planes x|y|z, or NULL.
+ *              When set, the integrator drifts in FP64 and writes the FP32
+ *              mirror posm.xyz from it; when NULL it drifts posm in FP32.
+ *   vel, acc, xvel  float[n][4]  xyz + unused pad     (DEV_VEL/ACC/XVEL)
+ *   density, density_dt, balsara, maxvsig  float[n]   (DEV_DENSITY ...)
+ *   dstress, dstress_dt  float[6][n]  symmetric tensor planes
+ *              xx|yy|zz|xy|xz|yz                      (DEV_DSTRESS[_DT])
+ * Arrays a configuration does not read or write may be NULL; an entry point
+ * returns GRASPH_E_INVALID when one it needs is NULL.
+ */
+typedef struct grasph_particles {
+    int64_t n;
+    float* posm;
+    double* pos64;
+    float* vel;
+    float* acc;
+    float* xvel;
+    float* density;
+    float* density_dt;
+    float* balsara;
+    float* maxvsig;
+    float* dstress;
+    float* dstress_dt;
+} grasph_particles;
+
+typedef struct grasph_ctx grasph_ctx;
+
+/* library identity */
+int grasph_version(void);
+/* Message of the last failure on ctx (or of the last failed grasph_create
+ * when ctx is NULL).  Never NULL. */
+const char* grasph_last_error(const grasph_ctx* ctx);
+
+/* Per-device context: owns scratch (neighbour lists, Morton order, partial
+ * sums, timing events).  Not thread-safe; use one context per host thread. */
+int grasph_create(int device, grasph_ctx** out);
+int grasph_destroy(grasph_ctx* ctx);
+
+/* Tuning knobs.  Keys: "max_neighbours" (default 160), "reorder_interval"
+ * (Morton re-sort every k derivative evaluations, default 1; 0 = never),
+ * "profile" (1 = record HIP events around each kernel). */
+int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
+
+/* ---- the hot path ------------------------------------------------------- */
+
+/* computeDerivatives(pb)                       src/computeDerivatives.h:315-328
+ * Runs the passes the flags select, in the reference's order:
+ * density summation (:320), strain rate / Balsara / density rate (:323),
+ * acceleration (:327).  Fused here into one all-pairs sweep (gravity,
+ * density, neighbour list) plus two neighbour-list passes. */
+int grasph_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p,
+                               const grasph_params* prm, void* stream);
+
+/* The three passes individually, each standalone (each runs its own sweep):
+ *   do_for_each_pair_fast<calcDensity>                   computeDerivatives.h:52-91
+ *   do_for_each_pair_fast<calcBalsaraDensityDTDStressDT> computeDerivatives.h:96-162
+ *   do_for_each_pair_fast<calcAcceleration>              computeDerivatives.h:167-313 */
+int grasph_calc_density(grasph_ctx* ctx, const grasph_particles* p,
+                        const grasph_params* prm, void* stream);
+int grasph_calc_balsara_density_dt_dstress_dt(grasph_ctx* ctx, const grasph_particles* p,
+                                              const grasph_params* prm, void* stream);
+int grasph_calc_acceleration(grasph_ctx* ctx, const grasph_particles* p,
+                             const grasph_params* prm, void* stream);
+
+/* integrate(pb, notFirstStep) with FIXED_TIMESTEP_LEAPFROG
+ * src/integration.h:219-244 -> fixedTsLeapfrog src/integration.h:82-125 */
+int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p,
+                     const grasph_params* prm, int not_first_step, void* stream);
+
+/* getCurrentTimestep()                         src/integration.h:246-255 */
+double grasph_get_current_timestep(const grasph_ctx* ctx, const grasph_params* prm);
+
+/* ---- diagnostics (no counterpart in the reference) --------------------- */
+
+#define GRASPH_INV_EKIN 0   /* 1/2 sum m |v + a*half_kick_dt|^2             */
+#define GRASPH_INV_EPOT 1   /* -1/2 sum_i sum_{j!=i} m_i m_j / sqrt(r^2+H^2) */
+#define GRASPH_INV_EINT 2   /* elastic + compressional energy (SPH only)     */
+#define GRASPH_INV_PX 3     /* linear momentum sum m v                      */
+#define GRASPH_INV_PY 4
+#define GRASPH_INV_PZ 5
+#define GRASPH_INV_LX 6     /* angular momentum sum m x cross v             */
+#define GRASPH_INV_LY 7
+#define GRASPH_INV_LZ 8
+#define GRASPH_INV_MASS 9   /* total mass                                   */
+#define GRASPH_NINV 10
+
+/* FP64 global invariants.  half_kick_dt synchronises the half-step leapfrog
+ * velocities to the current time (pass dt/2 after computeDerivatives, 0 to
+ * use the stored velocities).  Runs its own gravity-potential sweep when
+ * with_potential is nonzero.  Synchronises `stream` before returning. */
+int grasph_reduce_invariants(grasph_ctx* ctx, const grasph_particles* p,
+                             const grasph_params* prm, double half_kick_dt,
+                             int with_potential, double out[GRASPH_NINV], void* stream);
+
+/* Per-kernel timing recorded while option "profile" is 1: accumulated
+ * milliseconds and launch counts per kernel class, in the order
+ * sweep, strain (pass B), accel (pass C), integrate, reorder, other.
+ * Synchronises the recorded events.  Resets the counters when reset != 0. */
+#define GRASPH_NKERNEL_CLASSES 6
+int grasph_get_kernel_times(grasph_ctx* ctx, double ms[GRASPH_NKERNEL_CLASSES],
+                            int64_t launches[GRASPH_NKERNEL_CLASSES], int reset);
+
+/* Neighbour statistics of the last neighbour list this context built:
+ * ordered neighbour pairs (0 < r^2 <= H^2), the largest per-particle count,
+ * and how many particles exceeded "max_neighbours" (those take the all-j
+ * fallback).  Synchronises the device. */
+int grasph_get_neighbour_stats(grasph_ctx* ctx, int64_t* pairs, int32_t* max_count, int64_t* overflowed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRASPH2_H */

@@ -1,0 +1,246 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerance: relative 1e-5 against the FP64 oracle for acceleration and
+density (north_star's bound), computed with tests/_util.py's metrics: the
+largest per-particle relative error over particles whose reference magnitude
+is at least 1e-3 of the median, and the norm-wise error.  Tensor rates are
+compared relative to their largest entry.  Exactly-defined quantities
+(neighbour-order independence, overflow fallback) are compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+import grasph2_amd as g
+from grasph2_amd import ics
+from oracle import pyoracle as O
+from tests._util import scalar_errors, tensor_errors, vec_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def gpu_run(ctx, h, prm, op="derivs", **kw):
+    d = g.DeviceParticles.from_host(h)
+    if op == "derivs":
+        ctx.compute_derivatives(d, prm)
+    elif op == "density":
+        ctx.calc_density(d, prm)
+    elif op == "balsara":
+        ctx.calc_balsara_density_dt_dstress_dt(d, prm)
+    elif op == "accel":
+        ctx.calc_acceleration(d, prm)
+    elif op == "integrate":
+        ctx.integrate(d, prm, kw.get("not_first_step", True))
+    g.hip.synchronize()
+    out = d.download()
+    d.free()
+    return out
+
+
+def randomise(h, prm, seed, vscale=0.05, sscale=0.02):
+    rng = np.random.default_rng(seed)
+    n = h["posm"].shape[0]
+    if "vel" in h:
+        h["vel"][:, :3] += rng.normal(scale=vscale, size=(n, 3))
+    if "dstress" in h:
+        h["dstress"][:] = rng.normal(scale=sscale, size=(6, n))
+    if "density" in h:
+        h["density"][:] = prm.rho0 * (1 + rng.uniform(-0.05, 0.05, size=n))
+    if "balsara" in h:
+        h["balsara"][:] = rng.uniform(0, 1, size=n)
+    return h
+
+
+def check_vec(got, ref, tol=TOL, what="acc"):
+    mx, nr = vec_errors(got, ref)
+    assert nr <= tol and mx <= 10 * tol, "%s: max per-particle rel %.3g, norm-wise %.3g" % (what, mx, nr)
+    return mx, nr
+
+
+# ------------------------------------------------------------------ gravity
+@pytest.mark.parametrize("n", [1, 2, 255, 1000, 4096])
+def test_gravity_matches_fp64(gpu_ctx, n):
+    h, prm = ics.plummer(n, seed=n)
+    out = gpu_run(gpu_ctx, h, prm)
+    ref = O.calc_acceleration(prm, h, "f64")
+    if n == 1:
+        assert np.all(out["acc"][:, :3] == 0)
+        return
+    check_vec(out["acc"], ref["acc"])
+
+
+def test_gravity_unsoftened_and_coincident(gpu_ctx):
+    h, prm = ics.plummer(600, seed=3)
+    prm.H = 0.0
+    h["posm"][5] = h["posm"][4]  # two coincident particles: r2 = 0 pairs are skipped
+    h["pos64"][:, 5] = h["pos64"][:, 4]
+    out = gpu_run(gpu_ctx, h, prm)
+    assert np.all(np.isfinite(out["acc"]))
+    ref = O.calc_acceleration(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
+
+
+# ------------------------------------------------------------------ SPH
+def liquid_case(n=1500, seed=7):
+    h, prm = ics.liquid_blob(n, seed=seed, fields=("posm", "pos64", "vel", "acc", "density", "maxvsig", "xvel",
+                                                   "balsara"))
+    h = randomise(h, prm, seed, vscale=0.2)
+    return h, prm
+
+
+def solid_case(n=3000, seed=8, gravity=True):
+    h, prm = ics.collision(n, seed=seed, gravity=gravity)
+    h = randomise(h, prm, seed)
+    return h, prm
+
+
+def test_liquid_derivatives(gpu_ctx):
+    h, prm = liquid_case()
+    out = gpu_run(gpu_ctx, h, prm)
+    ref = O.compute_derivatives(prm, h, "f64")
+    mx, nr = scalar_errors(out["density"], ref["density"])
+    assert nr <= TOL and mx <= 10 * TOL, (mx, nr)
+    check_vec(out["acc"], ref["acc"])
+    # AV without the Balsara switch does not track the signal velocity
+    assert np.all(out["maxvsig"] == 0) and np.all(ref["maxvsig"] == 0)
+
+
+@pytest.mark.parametrize("gravity", [True, False])
+def test_solid_derivatives(gpu_ctx, gravity):
+    h, prm = solid_case(gravity=gravity)
+    out = gpu_run(gpu_ctx, h, prm)
+    ref = O.compute_derivatives(prm, h, "f64")
+    mx, nr = scalar_errors(out["density"], ref["density"])
+    assert nr <= TOL and mx <= 10 * TOL, ("density", mx, nr)
+    check_vec(out["acc"], ref["acc"])
+    mx, nr = tensor_errors(out["dstress_dt"], ref["dstress_dt"])
+    assert mx <= 10 * TOL and nr <= TOL, ("dstress_dt", mx, nr)
+    mx, nr = scalar_errors(out["balsara"], ref["balsara"], floor=1e-2)
+    assert nr <= 10 * TOL, ("balsara", mx, nr)
+    mx, nr = scalar_errors(out["maxvsig"], ref["maxvsig"], floor=1e-2)
+    assert nr <= TOL, ("maxvsig", mx, nr)
+    assert np.all(out["density_dt"] == 0)  # not integrated: stays zero as in the reference
+
+
+def test_integrate_density_and_mohr_coulomb(gpu_ctx):
+    h, prm = solid_case(2000, seed=12)
+    prm.flags = (prm.flags | g.INTEGRATE_DENSITY | g.PLASTICITY_MC) & ~g.PLASTICITY_MIESE
+    prm.tanfr, prm.cohesion = 0.8, 0.01
+    out = gpu_run(gpu_ctx, h, prm)
+    ref = O.compute_derivatives(prm, h, "f64")
+    assert np.array_equal(out["density"], h["density"])  # no summation pass with INTEGRATE_DENSITY
+    mx, nr = scalar_errors(out["density_dt"], ref["density_dt"], floor=1e-2)
+    assert nr <= 10 * TOL, ("density_dt", mx, nr)
+    check_vec(out["acc"], ref["acc"])
+
+
+def test_liquid_extras(gpu_ctx):
+    """Balsara, XSPH, artificial pressure, constant acceleration, no-partner density."""
+    h, prm = liquid_case(1200, seed=21)
+    prm.flags |= g.BALSARA_SWITCH | g.XSPH | g.ARTIFICIAL_STRESS | g.CONSTANT_ACCELERATION | g.DEAL_WITH_NO_PARTNERS
+    prm.normalsep = 0.3 * prm.H
+    h["posm"][-1, :3] = [50.0, 0, 0]  # an isolated particle
+    h["pos64"][:, -1] = [50.0, 0, 0]
+    out = gpu_run(gpu_ctx, h, prm)
+    ref = O.compute_derivatives(prm, h, "f64")
+    assert out["density"][-1] == pytest.approx(0.01 * prm.rho0)
+    check_vec(out["acc"], ref["acc"])
+    check_vec(out["xvel"], ref["xvel"], what="xvel")
+
+
+def test_standalone_passes(gpu_ctx):
+    h, prm = solid_case(2500, seed=31)
+    out = gpu_run(gpu_ctx, h, prm, "density")
+    ref = O.calc_density(prm, h, "f64")
+    assert scalar_errors(out["density"], ref["density"])[1] <= TOL
+    out = gpu_run(gpu_ctx, h, prm, "balsara")
+    ref = O.calc_balsara(prm, h, "f64")
+    assert tensor_errors(out["dstress_dt"], ref["dstress_dt"])[1] <= TOL
+    out = gpu_run(gpu_ctx, h, prm, "accel")
+    ref = O.calc_acceleration(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
+
+
+def test_order_and_list_capacity_do_not_change_results(gpu_ctx):
+    h, prm = solid_case(3000, seed=41)
+    base = gpu_run(gpu_ctx, h, prm)
+    try:
+        gpu_ctx.set_option("reorder_interval", 0)  # identity order
+        ident = gpu_run(gpu_ctx, h, prm)
+        gpu_ctx.set_option("reorder_interval", 1)
+        gpu_ctx.set_option("max_neighbours", 8)  # every list overflows: all-j fallback
+        small = gpu_run(gpu_ctx, h, prm)
+    finally:
+        gpu_ctx.set_option("reorder_interval", 1)
+        gpu_ctx.set_option("max_neighbours", 160)
+    for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
+        assert np.array_equal(base[f], ident[f]), f
+        assert np.array_equal(base[f], small[f]), f
+
+
+# ------------------------------------------------------------------ integrator
+def test_leapfrog_fp64_drift(gpu_ctx):
+    h, prm = solid_case(2000, seed=51)
+    prm.flags |= g.INTEGRATE_DENSITY
+    rng = np.random.default_rng(1)
+    n = h["posm"].shape[0]
+    h["acc"][:, :3] = rng.normal(size=(n, 3))
+    h["dstress_dt"][:] = rng.normal(scale=30.0, size=(6, n))  # drives some particles past yield
+    h["density_dt"][:] = rng.normal(scale=5.0, size=n)
+    for nfs in (False, True):
+        out = gpu_run(gpu_ctx, h, prm, "integrate", not_first_step=nfs)
+        ref = O.leapfrog(prm, h, not_first_step=nfs, prec="f64")
+        assert np.allclose(out["pos64"].T, ref["posm"][:, :3], rtol=0, atol=1e-9)
+        assert np.allclose(out["posm"][:, :3], ref["posm"][:, :3], rtol=0, atol=3e-7)
+        assert np.allclose(out["vel"][:, :3], ref["vel"][:, :3], rtol=1e-6, atol=1e-7)
+        assert np.allclose(out["dstress"], ref["dstress"], rtol=1e-5, atol=1e-7)
+        assert np.allclose(out["density"], ref["density"], rtol=1e-6, atol=1e-7)
+
+
+def test_leapfrog_fp32_drift_matches_fp32_oracle(gpu_ctx):
+    h, prm = solid_case(2000, seed=52)
+    rng = np.random.default_rng(2)
+    n = h["posm"].shape[0]
+    h["acc"][:, :3] = rng.normal(size=(n, 3))
+    del h["pos64"]
+    out = gpu_run(gpu_ctx, h, prm, "integrate")
+    ref = O.leapfrog(prm, h, prec="f32")
+    assert np.allclose(out["posm"], ref["posm"], rtol=1e-6, atol=1e-7)
+    assert np.allclose(out["vel"], ref["vel"], rtol=1e-6, atol=1e-7)
+
+
+# ------------------------------------------------------------------ invariants
+def test_invariants(gpu_ctx):
+    h, prm = solid_case(2000, seed=61)
+    d = g.DeviceParticles.from_host(h)
+    gpu_ctx.compute_derivatives(d, prm)
+    inv = gpu_ctx.invariants(d, prm, half_kick_dt=0.5 * prm.fixed_timestep, with_potential=True)
+    hh = d.download()
+    d.free()
+    ref = O.invariants(prm, hh, half_kick=0.5 * prm.fixed_timestep, phi=O.potential(prm, hh))
+    for k in (g.INV_EKIN, g.INV_EINT, g.INV_MASS):
+        assert inv[k] == pytest.approx(ref[k], rel=1e-9)
+    assert inv[g.INV_EPOT] == pytest.approx(ref[g.INV_EPOT], rel=1e-6)
+    scale = np.abs(ref[g.INV_PX:g.INV_LZ + 1]).max()
+    assert np.allclose(inv[g.INV_PX:g.INV_LZ + 1], ref[g.INV_PX:g.INV_LZ + 1], atol=1e-9 * scale)
+
+
+# ------------------------------------------------------------------ errors
+def test_errors_are_reported(gpu_ctx):
+    h, prm = solid_case(300, seed=71)
+    d = g.DeviceParticles.from_host(h)
+    bad = prm.copy()
+    bad.flags |= g.ARTIFICIAL_STRESS
+    with pytest.raises(g.GrasphError) as e:
+        gpu_ctx.compute_derivatives(d, bad)
+    assert e.value.code == g.E_UNSUPPORTED
+    v = d.view()
+    v.dstress = None
+    with pytest.raises(g.GrasphError) as e:
+        gpu_ctx.compute_derivatives(v, prm)
+    assert e.value.code == g.E_INVALID and "dstress" in str(e.value)
+    empty = g.Particles()
+    empty.n = 0
+    gpu_ctx.compute_derivatives(empty, prm)  # zero particles: nothing to do
+    d.free()

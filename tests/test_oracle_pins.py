@@ -1,0 +1,400 @@
+"""Pins of the CPU oracle against closed forms, invariants and physics.
+
+The reference ships no tests, fixtures or golden vectors for this path and
+its physics headers cannot be built here (they need mpUtils, absent from the
+image), so each oracle function is checked against what the mathematics
+fixes, with checks a plausible mistake (dropped term, wrong sign, transposed
+index, wrong factor) would fail.  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import grasph2_amd as g
+from grasph2_amd import ics
+from oracle import pyoracle as O
+
+
+def cubic_lattice(m, d, center=True):
+    ax = np.arange(m) * d
+    p = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), axis=-1).reshape(-1, 3)
+    if center:
+        p -= p.mean(0)
+    return p
+
+
+def make(pos, mass, vel=None, fields=ics.SOLID_FIELDS + ("xvel",)):
+    n = len(pos)
+    h = g.host_particles(n, fields)
+    h["posm"][:, :3] = pos
+    h["posm"][:, 3] = mass
+    if "pos64" in h:
+        h["pos64"][:] = np.asarray(pos, dtype=np.float64).T
+    if vel is not None:
+        h["vel"][:, :3] = vel
+    return h
+
+
+def interior(pos, margin):
+    lo, hi = pos.min(0), pos.max(0)
+    return np.where(np.all((pos > lo + margin) & (pos < hi - margin), axis=1))[0]
+
+
+# ---------------------------------------------------------------- kernel
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_kernel_normalised(dim):
+    h = 0.37
+    r = np.linspace(0.0, h, 20001)
+    w = np.array([O.W(x, h, dim) for x in r])
+    if dim == 3:
+        f = 4 * np.pi * r * r * w
+    elif dim == 2:
+        f = 2 * np.pi * r * w
+    else:
+        f = 2 * w
+    integral = np.trapezoid(f, r)
+    assert abs(integral - 1.0) < 1e-6
+
+
+def test_kernel_values_and_support():
+    h = 0.1
+    assert O.W(0.0, h) == pytest.approx(8 / (math.pi * h ** 3), rel=1e-14)
+    # continuity where the pieces meet (both give 1/4) and at the support radius h
+    for q in (0.5, 1.0):
+        lo, hi = O.W(q * h * (1 - 1e-10), h), O.W(q * h * (1 + 1e-10), h)
+        assert abs(lo - hi) < 1e-6 * O.W(0, h)
+    assert O.W(0.5 * h, h) == pytest.approx(0.25 * 8 / (math.pi * h ** 3), rel=1e-12)
+    assert O.W(1.0001 * h, h) == 0.0 and O.W(2 * h, h) == 0.0
+    assert O.W(0.999 * h, h) > 0.0
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_kernel_derivative_is_dW_dr(dim):
+    h = 0.2
+    for q in (0.05, 0.2, 0.45, 0.55, 0.7, 0.95):
+        r = q * h
+        e = 1e-7 * h
+        fd = (O.W(r + e, h, dim) - O.W(r - e, h, dim)) / (2 * e)
+        assert O.dW(r, h, dim) == pytest.approx(fd, rel=1e-6)
+    assert O.dW(1.5 * h, h, dim) == 0.0
+
+
+# ---------------------------------------------------------------- EOS
+def test_eos():
+    assert O.eos_liquid(1.2, 1.0, 4.0) == pytest.approx(0.8)
+    assert O.eos_liquid(1.0, 1.0, 4.0) == 0.0
+    assert O.eos_murnaghan(1.0, 2.0 / 2.0, 5.0, 3.0) == 0.0
+    # bulk modulus K = rho dp/drho at the rest density, for any exponent
+    for n in (1.0, 4.0, 7.5):
+        K, r0 = 2.0, 1.3
+        e = 1e-6
+        dp = (O.eos_murnaghan(r0 + e, r0, K, n) - O.eos_murnaghan(r0 - e, r0, K, n)) / (2 * e)
+        assert r0 * dp == pytest.approx(K, rel=1e-7)
+    assert O.eos_murnaghan(2.0, 1.0, 3.0, 1.0) == pytest.approx(3.0)
+
+
+# ---------------------------------------------------------------- AV / Balsara / plasticity
+def test_artificial_viscosity():
+    # receding pair: no viscosity, signal velocity untouched
+    II, mv = O.artificial_viscosity(1.0, 1.0, 1.0, [0.5, 0, 0], [1.0, 0, 0], 1.0, 1.0, 1.0, maxvsig=0.25)
+    assert II == 0.0 and mv == 0.25
+    # head-on approach at 0.5: w=-0.5, vsig = 1+1+1.5 = 3.5, Pi = 0.25*2*1*0.5*3.5/1 = 0.875
+    II, mv = O.artificial_viscosity(1.0, 1.0, 1.0, [-0.5, 0, 0], [1.0, 0, 0], 1.0, 1.0, 1.0, maxvsig=0.0)
+    assert II == pytest.approx(0.875) and mv == pytest.approx(3.5)
+    # Balsara factors and mean density enter linearly / inversely
+    II2, _ = O.artificial_viscosity(1.0, 2.0, 2.0, [-0.5, 0, 0], [1.0, 0, 0], 1.0, 1.0, 1.0, 0.5, 0.5)
+    assert II2 == pytest.approx(0.875 / 4)
+
+
+def test_balsara_switch():
+    assert O.balsara_switch(3.0, [0, 0, 0], 1.0, 0.1) == pytest.approx(3.0 / (3.0 + 1e-3))
+    assert O.balsara_switch(0.0, [1.0, 2.0, 0], 1.0, 0.1) == 0.0
+    assert O.balsara_switch(-1.0, [0, 3.0, 4.0], 1.0, 0.1) == pytest.approx(1.0 / (6.0 + 1e-3))
+
+
+def test_plasticity_von_mises():
+    S = np.zeros((3, 3))
+    S[0, 1] = S[1, 0] = 0.1
+    out = O.plasticity(S, 0.05)  # J2 = 0.01, factor Y^2/(3 J2) = 1/12
+    assert out[0, 1] == pytest.approx(0.1 / 12) and out[1, 0] == pytest.approx(0.1 / 12)
+    assert np.all(out[np.eye(3, dtype=bool)] == 0)
+    assert np.array_equal(O.plasticity(S, 1.0), S)          # inside the yield surface
+    assert np.array_equal(O.plasticity(np.zeros((3, 3)), 0.05), np.zeros((3, 3)))  # J2 = 0
+
+
+def test_dstress_dt_closed_forms():
+    mu = 1.7
+    # no rotation: Hooke's deviatoric law 2 mu (edot - tr/3 I)
+    E = np.array([[0.3, 0.1, 0.0], [0.1, -0.2, 0.05], [0.0, 0.05, 0.4]])
+    S = np.array([[0.2, 0.0, 0.1], [0.0, -0.1, 0.0], [0.1, 0.0, -0.1]])
+    out = O.dstress_dt(E, np.zeros((3, 3)), S, mu)
+    assert np.allclose(out, 2 * mu * (E - np.trace(E) / 3 * np.eye(3)), atol=1e-14)
+    # pure rotation: the stress co-rotates, so J2 is conserved and the rate is R S - S R
+    R = np.array([[0.0, 0.3, -0.2], [-0.3, 0.0, 0.5], [0.2, -0.5, 0.0]])
+    out = O.dstress_dt(np.zeros((3, 3)), R, S, mu)
+    assert abs(np.sum(S * out)) < 1e-14
+    Q = np.eye(3)
+    Sq = S.copy()
+    dt = 1e-4
+    for _ in range(1000):  # d/dt (Q S Q^T) with Qdot = R Q integrates to exp(Rt) S exp(-Rt)
+        Sq = Sq + dt * O.dstress_dt(np.zeros((3, 3)), R, Sq, mu)
+    w, V = np.linalg.eig(R * 0.1)
+    Qt = (V @ np.diag(np.exp(w)) @ np.linalg.inv(V)).real
+    assert np.allclose(Sq, Qt @ S @ Qt.T, atol=2e-5)
+
+
+# ---------------------------------------------------------------- density pass
+def test_density_pair_and_isolated():
+    H = 0.1
+    prm = ics.liquid_params(H, gravity=False)
+    h = make(np.array([[0.0, 0, 0], [0.06, 0, 0], [5.0, 0, 0]]), np.array([1.0, 2.0, 3.0]))
+    out = O.calc_density(prm, h)
+    Hf = prm.H  # H as the params struct stores it (float32)
+    assert out["density"][0] == pytest.approx(2.0 * O.W(0.06, Hf), rel=1e-12)
+    assert out["density"][1] == pytest.approx(1.0 * O.W(0.06, Hf), rel=1e-12)
+    assert out["density"][2] == 0.0  # self term excluded
+    prm.flags |= g.DEAL_WITH_NO_PARTNERS
+    out = O.calc_density(prm, h)
+    assert out["density"][2] == pytest.approx(0.01 * prm.rho0)
+
+
+def test_density_lattice_normalisation():
+    d = 0.05
+    H = 2.5 * d
+    pos = cubic_lattice(16, d)
+    rho0 = 1.0
+    m = rho0 * d ** 3
+    prm = ics.liquid_params(H, gravity=False, rho0=rho0)
+    h = make(pos, np.full(len(pos), m))
+    idx = interior(pos, H * 1.05)
+    out = O.calc_density(prm, h, sample=idx)
+    rho = out["density"][idx]
+    selfterm = m * O.W(0.0, H)
+    assert np.all(np.abs(rho + selfterm - rho0) < 0.01 * rho0)
+    assert np.all(rho < 0.9 * rho0)  # the missing self term is a ~16% deficit at H = 2.5 spacing
+
+
+# ---------------------------------------------------------------- strain / rotation pass
+def _strain_case(L, S0=None):
+    d = 0.05
+    H = 2.5 * d
+    pos = cubic_lattice(14, d)
+    prm = ics.solid_params(H, gravity=False, mu=1.3)
+    n = len(pos)
+    h = make(pos, np.full(n, d ** 3), vel=pos @ np.asarray(L).T)
+    h["density"][:] = 1.0  # m/rho equals the particle volume
+    if S0 is not None:
+        for c, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+            h["dstress"][c] = S0[a][b]
+    idx = interior(pos, H * 1.05)
+    out = O.calc_balsara(prm, h, sample=idx)
+    return prm, out, idx
+
+
+def _full(six, i):
+    xx, yy, zz, xy, xz, yz = (six[c][i] for c in range(6))
+    return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]])
+
+
+def test_strain_rate_of_linear_field():
+    L = np.array([[0.3, 1.0, 0.0], [0.0, -0.1, 0.2], [0.4, 0.0, 0.2]])
+    prm, out, idx = _strain_case(L)
+    # with S = 0, dS/dt = 2 mu dev(edot) with edot ~ sym(L): asymmetric L pins the transpose
+    sym = 0.5 * (L + L.T)
+    expect = 2 * prm.shear * (sym - np.trace(sym) / 3 * np.eye(3))
+    for i in idx[:: max(1, len(idx) // 20)]:
+        got = _full(out["dstress_dt"], i)
+        assert np.allclose(got, expect, atol=0.02 * np.abs(expect).max())
+    # Balsara: |div v| / (|div v| + |curl v| + eps); curl of L x is (L32-L23, L13-L31, L21-L12)
+    div = np.trace(L)
+    curl = np.array([L[2, 1] - L[1, 2], L[0, 2] - L[2, 0], L[1, 0] - L[0, 1]])
+    b = abs(div) / (abs(div) + np.linalg.norm(curl) + 1e-4 * prm.SOUNDSPEED / prm.H)
+    assert np.allclose(out["balsara"][idx], b, rtol=0.03)
+
+
+def test_stress_corotates_with_rigid_rotation():
+    w = np.array([0.3, -0.2, 0.5])
+    Om = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])  # Om x = w cross x
+    S0 = np.array([[0.2, 0.05, 0.0], [0.05, -0.1, 0.03], [0.0, 0.03, -0.1]])
+    prm, out, idx = _strain_case(Om, S0)
+    expect = Om @ S0 - S0 @ Om  # d/dt (Q S Q^T) at t = 0
+    for i in idx[:: max(1, len(idx) // 20)]:
+        got = _full(out["dstress_dt"], i)
+        assert np.allclose(got, expect, atol=0.02 * np.abs(expect).max())
+    assert np.all(out["balsara"][idx] < 1e-3)  # pure rotation: no compression
+
+
+# ---------------------------------------------------------------- acceleration pass
+def test_two_body_gravity():
+    prm = ics.plummer_params(H=0.0)
+    h = make(np.array([[0.0, 0, 0], [1.0, 0, 0]]), np.array([1.0, 2.0]))
+    out = O.calc_acceleration(prm, h)
+    assert np.allclose(out["acc"][0, :3], [2.0, 0, 0], rtol=1e-14)
+    assert np.allclose(out["acc"][1, :3], [-1.0, 0, 0], rtol=1e-14)
+    prm.H = 0.5  # softening is H: 2 / 1.25^1.5
+    out = O.calc_acceleration(prm, h)
+    assert out["acc"][0, 0] == pytest.approx(2.0 / 1.25 ** 1.5, rel=1e-7)
+
+
+def test_gravity_is_minus_grad_potential():
+    rng = np.random.default_rng(11)
+    pos = rng.normal(size=(40, 3))
+    m = rng.uniform(0.5, 1.5, size=40)
+    prm = ics.plummer_params(H=0.3)
+    h = make(pos, m)
+    acc = O.calc_acceleration(prm, h)["acc"][:, :3]
+    i, e = 7, 1e-5
+    grad = np.zeros(3)
+    for d in range(3):
+        hp, hm = make(pos.copy(), m), make(pos.copy(), m)
+        hp["posm"][i, d] += e
+        hm["posm"][i, d] -= e
+        hp["pos64"][d, i] += e
+        hm["pos64"][d, i] -= e
+        grad[d] = (O.potential(prm, hp)[i] - O.potential(prm, hm)[i]) / (2 * e)
+    assert np.allclose(acc[i], -grad, rtol=1e-6)
+    mf = h["posm"][:, 3].astype(np.float64)  # the masses the oracle saw (float32 storage)
+    assert np.allclose((acc * mf[:, None]).sum(0), 0, atol=1e-12)
+
+
+def _sph_lattice(m=14, d=0.05):
+    pos = cubic_lattice(m, d)
+    return pos, d, 2.5 * d
+
+
+def test_pressure_gradient_drives_acceleration():
+    pos, d, H = _sph_lattice()
+    eps, c = 0.2, 1.5
+    rho = 1.0 * (1 + eps * pos[:, 0])
+    prm = ics.liquid_params(H, gravity=False, c=c)
+    prm.flags &= ~g.ARTIFICIAL_VISCOSITY
+    h = make(pos, rho * d ** 3, fields=("posm", "vel", "acc", "density", "maxvsig", "xvel"))
+    h["density"][:] = rho
+    idx = interior(pos, H * 1.05)
+    acc = O.calc_acceleration(prm, h, sample=idx)["acc"][idx, :3]
+    expect = -c * c * 1.0 * eps / rho[idx]  # -grad p / rho
+    assert np.allclose(acc[:, 0], expect, rtol=0.05)
+    assert np.all(np.abs(acc[:, 1:]) < 0.02 * np.abs(expect).max())
+
+
+def test_shear_stress_divergence():
+    pos, d, H = _sph_lattice()
+    s = 0.3
+    prm = ics.solid_params(H, gravity=False)
+    prm.flags &= ~(g.ARTIFICIAL_VISCOSITY | g.BALSARA_SWITCH)
+    h = make(pos, np.full(len(pos), d ** 3))
+    h["density"][:] = 1.0  # rho = rho0: no pressure
+    h["dstress"][3] = s * pos[:, 0]  # sigma_xy = s x  ->  a_y = d sigma_yx / dx = s
+    idx = interior(pos, H * 1.05)
+    acc = O.calc_acceleration(prm, h, sample=idx)["acc"][idx, :3]
+    assert np.allclose(acc[:, 1], s, rtol=0.05)
+    assert np.all(np.abs(acc[:, [0, 2]]) < 0.02 * s)
+
+
+def test_sph_momentum_conservation_and_viscous_dissipation():
+    rng = np.random.default_rng(5)
+    pos, d, H = _sph_lattice(10)
+    pos = pos + rng.uniform(-0.2 * d, 0.2 * d, size=pos.shape)
+    n = len(pos)
+    m = d ** 3 * rng.uniform(0.9, 1.1, size=n)
+    vel = rng.normal(scale=0.3, size=(n, 3))
+    prm = ics.solid_params(H, gravity=True)
+    h = make(pos, m, vel=vel)
+    h["dstress"][:] = rng.normal(scale=0.05, size=(6, n))
+    out = O.compute_derivatives(prm, h)
+    a = out["acc"][:, :3]
+    mf = h["posm"][:, 3].astype(np.float64)
+    P = (a * mf[:, None]).sum(0)
+    assert np.all(np.abs(P) < 1e-13 * np.abs(a * mf[:, None]).sum())
+    # viscosity alone (no pressure, no stress, no gravity) removes kinetic energy
+    prm2 = ics.solid_params(H, gravity=False)
+    h2 = make(pos, m, vel=vel)
+    h2["density"][:] = prm2.rho0
+    prm2.flags &= ~(g.BALSARA_SWITCH)
+    h2_out = O.calc_acceleration(prm2, h2)
+    a2 = h2_out["acc"][:, :3]
+    v2 = h2["vel"][:, :3].astype(np.float64)
+    assert np.sum(mf[:, None] * v2 * a2) < 0
+    assert np.all(h2_out["maxvsig"] == 0)  # the signal velocity is only tracked with the Balsara switch
+
+
+# ---------------------------------------------------------------- integrator
+def test_leapfrog_exact_for_constant_acceleration():
+    prm = ics.plummer_params(H=0.0, dt=0.01)
+    prm.flags = g.CONSTANT_ACCELERATION
+    prm.constant_acceleration[:] = (0.0, -2.0, 0.5)
+    x0 = np.array([[0.3, 0.2, -0.1]])
+    v0 = np.array([[1.0, 0.5, 0.0]])
+    h = make(x0, np.array([1.0]), vel=v0, fields=("posm", "pos64", "vel", "acc"))
+    gacc = np.array(prm.constant_acceleration[:])
+    for k in range(1, 51):
+        h.update({f: v for f, v in O.calc_acceleration(prm, h).items() if f == "acc"})
+        o = O.leapfrog(prm, h, not_first_step=(k > 1))
+        h["posm"], h["vel"] = o["posm"], o["vel"]
+        h["pos64"] = o["posm"][:, :3].T.copy()
+        t = k * prm.fixed_timestep
+        assert np.allclose(h["posm"][0, :3], x0[0] + v0[0] * t + 0.5 * gacc * t * t, atol=1e-12)
+        assert np.allclose(h["vel"][0, :3], v0[0] + gacc * (t - 0.5 * prm.fixed_timestep), atol=1e-12)
+
+
+def test_leapfrog_circular_orbit():
+    prm = ics.plummer_params(H=0.0, dt=2 * math.pi / 2000)
+    h = make(np.array([[-0.5, 0, 0], [0.5, 0, 0]]), np.array([0.5, 0.5]), vel=np.array([[0, -0.5, 0], [0, 0.5, 0]]),
+             fields=("posm", "pos64", "vel", "acc"))
+    e0 = None
+    for k in range(2000):
+        h["acc"] = O.calc_acceleration(prm, h)["acc"]
+        if k == 0:
+            inv = O.invariants(prm, h, half_kick=0.0, phi=O.potential(prm, h))
+            e0 = inv[g.INV_EKIN] + inv[g.INV_EPOT]
+        o = O.leapfrog(prm, h, not_first_step=(k > 0))
+        h["posm"], h["vel"] = o["posm"], o["vel"]
+        h["pos64"] = o["posm"][:, :3].T.copy()
+    # one period: back at the start
+    assert np.allclose(h["posm"][:, :3], [[-0.5, 0, 0], [0.5, 0, 0]], atol=2e-4)
+    h["acc"] = O.calc_acceleration(prm, h)["acc"]
+    inv = O.invariants(prm, h, half_kick=0.5 * prm.fixed_timestep, phi=O.potential(prm, h))
+    assert abs(inv[g.INV_EKIN] + inv[g.INV_EPOT] - e0) < 1e-6 * abs(e0)
+    assert inv[g.INV_LZ] == pytest.approx(0.25, rel=1e-9)
+    assert abs(e0 - (-0.25 + 0.125)) < 1e-12  # E = -m1 m2 / d + 2 * 1/2 m v^2
+
+
+def test_leapfrog_stress_density_boundary():
+    prm = ics.solid_params(0.1, dt=0.01)
+    prm.flags |= g.INTEGRATE_DENSITY | g.SIMPLE_BOX_BOUNDARY
+    prm.box_min[:] = (-1, -1, -1)
+    prm.box_max[:] = (1, 1, 1)
+    prm.box_reflectiveness = 0.5
+    h = make(np.array([[0.0, 0, 0], [0.999, 0, 0]]), np.array([1.0, 1.0]), vel=np.array([[0, 0, 0], [1.0, 0, 0]]))
+    h["density"][:] = [1.0, 0.001]
+    h["density_dt"][:] = [2.0, -1.0]
+    h["dstress_dt"][3, 0] = 0.1 / prm.fixed_timestep  # S_xy = 0.1 after the drift -> beyond yield
+    o = O.leapfrog(prm, h, not_first_step=True)
+    assert o["density"][0] == pytest.approx(1.02)
+    assert o["density"][1] == 0.0  # clamped at zero
+    assert o["dstress"][3, 0] == pytest.approx(0.1 / 12)  # von Mises factor Y^2/(3 J2), Y = 0.05
+    assert o["posm"][1, 0] == pytest.approx(1.0)  # clamped to the wall
+    assert o["vel"][1, 0] == pytest.approx(-0.5)  # reflected with reflectiveness 0.5
+
+
+# ---------------------------------------------------------------- invariants
+def test_invariants_two_body():
+    prm = ics.plummer_params(H=0.5)
+    h = make(np.array([[0.0, 0, 0], [1.0, 0, 0]]), np.array([1.0, 3.0]), vel=np.array([[0, 1.0, 0], [0, -0.5, 2.0]]),
+             fields=("posm", "pos64", "vel", "acc"))
+    inv = O.invariants(prm, h, phi=O.potential(prm, h))
+    assert inv[g.INV_EPOT] == pytest.approx(-3.0 / math.sqrt(1.25), rel=1e-12)
+    assert inv[g.INV_EKIN] == pytest.approx(0.5 * 1 + 0.5 * 3 * (0.25 + 4), rel=1e-12)
+    assert np.allclose(inv[[g.INV_PX, g.INV_PY, g.INV_PZ]], [0, -0.5, 6.0])
+    assert np.allclose(inv[[g.INV_LX, g.INV_LY, g.INV_LZ]], [0, -6.0, -1.5])
+    assert inv[g.INV_MASS] == 4.0
+
+
+def test_fp32_oracle_tracks_fp64():
+    h, prm = ics.collision(2000, seed=9)
+    o64 = O.compute_derivatives(prm, h, "f64")
+    o32 = O.compute_derivatives(prm, h, "f32")
+    a64, a32 = o64["acc"][:, :3], o32["acc"][:, :3]
+    assert np.linalg.norm(a32 - a64) / np.linalg.norm(a64) < 1e-5
+    assert np.allclose(o32["density"], o64["density"], rtol=1e-5)
