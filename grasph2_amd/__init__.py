@@ -14,7 +14,7 @@ from ._capi import (  # noqa: F401
     ARTIFICIAL_STRESS, ARTIFICIAL_VISCOSITY, BALSARA_SWITCH, CLOHESSY_WILTSHIRE, CONSTANT_ACCELERATION,
     DEAL_WITH_NO_PARTNERS, INTEGRATE_DENSITY, PLASTICITY_MC, PLASTICITY_MIESE, SELF_GRAVITY, SIMPLE_BOX_BOUNDARY,
     SOLIDS, SPH, XSPH, E_HIP, E_INVALID, E_NOMEM, E_UNSUPPORTED, NINV, INV_EKIN, INV_EPOT, INV_EINT, INV_PX, INV_PY,
-    INV_PZ, INV_LX, INV_LY, INV_LZ, INV_MASS, KERNEL_CLASSES, NKERNEL_CLASSES, GrasphError, Params, Particles,
+    INV_PZ, INV_LX, INV_LY, INV_LZ, INV_MASS, KERNEL_CLASSES, NKERNEL_CLASSES, Batch, GrasphError, Params, Particles,
     default_params, lib,
 )
 
@@ -37,6 +37,37 @@ FIELDS = {
 def host_particles(n, fields=tuple(FIELDS)):
     """Zero-filled host arrays (numpy) in the C-ABI layout."""
     return {f: np.zeros(FIELDS[f][0](n), dtype=FIELDS[f][1]) for f in fields}
+
+
+def pack_batch(instances, fields=None):
+    """Concatenate per-instance host arrays into the batch layout of
+    grasph_batch: every instance keeps its own single-instance layout, so
+    planar arrays hold instance k's planes back to back at planes * offset_k.
+    Returns (host dict shaped like one n_total instance, offsets)."""
+    fields = fields or [f for f in FIELDS if f in instances[0]]
+    sizes = [h["posm"].shape[0] for h in instances]
+    offsets = np.zeros(len(sizes) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(sizes)
+    n = int(offsets[-1])
+    out = {}
+    for f in fields:
+        flat = np.concatenate([np.ascontiguousarray(h[f], dtype=FIELDS[f][1]).ravel() for h in instances])
+        out[f] = flat.reshape(FIELDS[f][0](n))
+    return out, offsets
+
+
+def unpack_batch(h, offsets):
+    """Inverse of pack_batch: a list of per-instance host dicts."""
+    out = []
+    for k in range(len(offsets) - 1):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        d = {}
+        for f, arr in h.items():
+            flat = np.ascontiguousarray(arr).ravel()
+            per = int(np.prod(FIELDS[f][0](1)))  # values per particle (planes for planar arrays)
+            d[f] = flat[per * a:per * b].reshape(FIELDS[f][0](b - a))
+        out.append(d)
+    return out
 
 
 class DeviceParticles:
@@ -156,6 +187,49 @@ class Context:
         self._check(self._lib.grasph_get_neighbour_stats(self.handle, ctypes.byref(pairs), ctypes.byref(mx),
                                                          ctypes.byref(over)))
         return pairs.value, mx.value, over.value
+
+    # ---- batched ensembles
+    def batch(self, offsets, params):
+        """A grasph_batch; keeps its host arrays alive on the returned object."""
+        offs = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        arr = (Params * len(params))(*params)
+        b = Batch()
+        b.n_instances = len(params)
+        b.offsets = offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        b.params = arr
+        b._keep = (offs, arr)
+        return b
+
+    def batch_compute_derivatives(self, parts, b, stream=None):
+        v = parts.view() if isinstance(parts, DeviceParticles) else parts
+        return self._check(self._lib.grasph_batch_compute_derivatives(self.handle, ctypes.byref(v), ctypes.byref(b),
+                                                                      self._s(stream)))
+
+    def batch_integrate(self, parts, b, not_first_step=True, stream=None):
+        v = parts.view() if isinstance(parts, DeviceParticles) else parts
+        return self._check(self._lib.grasph_batch_integrate(self.handle, ctypes.byref(v), ctypes.byref(b),
+                                                            1 if not_first_step else 0, self._s(stream)))
+
+    def batch_step(self, parts, b, n_steps=1, stream=None):
+        v = parts.view() if isinstance(parts, DeviceParticles) else parts
+        return self._check(self._lib.grasph_batch_step(self.handle, ctypes.byref(v), ctypes.byref(b), int(n_steps),
+                                                       self._s(stream)))
+
+    def batch_step_loop(self, parts, b, n_steps=1, stream=None):
+        v = parts.view() if isinstance(parts, DeviceParticles) else parts
+        return self._check(self._lib.grasph_batch_step_loop(self.handle, ctypes.byref(v), ctypes.byref(b),
+                                                            int(n_steps), self._s(stream)))
+
+    def autotune(self, parts, b, cache_path=None, stream=None):
+        v = parts.view() if isinstance(parts, DeviceParticles) else parts
+        return self._check(self._lib.grasph_autotune(self.handle, ctypes.byref(v), ctypes.byref(b),
+                                                     cache_path.encode() if cache_path else None, self._s(stream)))
+
+    def batch_tuning(self):
+        ni = (ctypes.c_int32 * 4)()
+        w, bp = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._lib.grasph_get_batch_tuning(self.handle, ni, ctypes.byref(w), ctypes.byref(bp)))
+        return {"ni": list(ni), "waves_per_block": w.value, "blocks_per_cu": bp.value}
 
     def close(self):
         if self.handle is not None and self.handle.value:

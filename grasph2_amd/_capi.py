@@ -30,8 +30,8 @@ E_INVALID, E_HIP, E_UNSUPPORTED, E_NOMEM = -1, -2, -3, -4
 
 INV_EKIN, INV_EPOT, INV_EINT, INV_PX, INV_PY, INV_PZ, INV_LX, INV_LY, INV_LZ, INV_MASS = range(10)
 NINV = 10
-NKERNEL_CLASSES = 6
-KERNEL_CLASSES = ("sweep", "strain", "accel", "integrate", "reorder", "other")
+NKERNEL_CLASSES = 7
+KERNEL_CLASSES = ("sweep", "density", "strain", "accel", "integrate", "reorder", "other")
 
 
 class Params(ctypes.Structure):
@@ -90,6 +90,14 @@ class Particles(ctypes.Structure):
     ]
 
 
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("n_instances", ctypes.c_int32),
+        ("offsets", ctypes.POINTER(ctypes.c_int64)),
+        ("params", ctypes.POINTER(Params)),
+    ]
+
+
 # every symbol include/grasph2.h declares, with (restype, argtypes)
 _P = ctypes.POINTER
 _SIGNATURES = {
@@ -110,6 +118,15 @@ _SIGNATURES = {
     "grasph_get_kernel_times": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), _P(ctypes.c_int64), ctypes.c_int]),
     "grasph_get_neighbour_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int64), _P(ctypes.c_int32),
                                                   _P(ctypes.c_int64)]),
+    "grasph_batch_compute_derivatives": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_void_p]),
+    "grasph_batch_integrate": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_int,
+                                              ctypes.c_void_p]),
+    "grasph_batch_step": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_int, ctypes.c_void_p]),
+    "grasph_batch_step_loop": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_int,
+                                              ctypes.c_void_p]),
+    "grasph_autotune": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_char_p, ctypes.c_void_p]),
+    "grasph_get_batch_tuning": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32),
+                                               _P(ctypes.c_int32)]),
 }
 
 _LIB = None

@@ -53,6 +53,21 @@ def build_library(verbose=True):
     return LIB
 
 
+def build_cpp_example(verbose=True):
+    """Compile tests/cpp/api_roundtrip.cpp, the C++ template API driven like
+    the reference's main loop (host code only: g++ and the HIP runtime)."""
+    src = os.path.join(ROOT, "tests", "cpp", "api_roundtrip.cpp")
+    exe = os.path.join(ROOT, "tests", "cpp", "api_roundtrip")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lgrasph2_amd", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../grasph2_amd", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C++ example build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    if verbose:
+        print("built", exe)
+
+
 def build_oracle(verbose=True):
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)

@@ -8,17 +8,13 @@
 
 #include "internal.hpp"
 
-struct grasph_ctx {
-    grasph::Ctx c;
-};
-
 namespace grasph {
 
 static thread_local std::string g_create_err = "no error";
 
 ProfScope::ProfScope(Ctx* ctx, hipStream_t st, int k) : c(ctx), s(st), cls(k)
 {
-    if (!c || !c->profile) return;
+    if (!c || !c->profile || c->capturing) return;
     for (int q = 0; q < 2; ++q) {
         hipEvent_t e = nullptr;
         if (!c->ev_free.empty()) {
@@ -44,67 +40,6 @@ ProfScope::~ProfScope()
 }
 
 namespace {
-
-int fail(Ctx& c, int code, const std::string& msg)
-{
-    c.err = msg;
-    return code;
-}
-
-int hip_fail(Ctx& c, hipError_t e, const char* where)
-{
-    c.err = std::string(where) + ": " + hipGetErrorString(e);
-    return GRASPH_E_HIP;
-}
-
-template <typename T>
-int grow(Ctx& c, T*& ptr, int64_t& cap, int64_t need, const char* what)
-{
-    if (need <= cap && ptr) return 0;
-    if (ptr) {
-        hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-    hipError_t e = hipMalloc((void**)&ptr, (size_t)need * sizeof(T));
-    if (e != hipSuccess) {
-        ptr = nullptr;
-        (void)hipGetLastError();
-        return fail(c, GRASPH_E_NOMEM, std::string("device allocation of ") + what + " failed");
-    }
-    cap = need;
-    return 0;
-}
-
-int check_params(Ctx& c, const grasph_params* prm)
-{
-    if (!prm) return fail(c, GRASPH_E_INVALID, "params is NULL");
-    if (prm->flags & ~GRASPH_ALL_FLAGS) return fail(c, GRASPH_E_INVALID, "unknown flag bits");
-    if (prm->dimension < 1 || prm->dimension > 3) return fail(c, GRASPH_E_INVALID, "dimension must be 1, 2 or 3");
-    if (!(prm->H >= 0.0f) || !std::isfinite(prm->H)) return fail(c, GRASPH_E_INVALID, "H must be finite and >= 0");
-    if ((prm->flags & GRASPH_SPH) && !(prm->H > 0.0f)) return fail(c, GRASPH_E_INVALID, "SPH needs H > 0");
-    if ((prm->flags & GRASPH_SOLIDS) && (prm->flags & GRASPH_ARTIFICIAL_STRESS))
-        return fail(c, GRASPH_E_UNSUPPORTED,
-                    "SOLIDS with ARTIFICIAL_STRESS is not supported (the reference does not compile it, "
-                    "src/computeDerivatives.h:267)");
-    if ((prm->flags & GRASPH_PLASTICITY_MIESE) && (prm->flags & GRASPH_PLASTICITY_MC))
-        return fail(c, GRASPH_E_INVALID, "choose up to one plasticity model (settings.h:203-205)");
-    return 0;
-}
-
-int check_particles(Ctx& c, const grasph_particles* p)
-{
-    if (!p) return fail(c, GRASPH_E_INVALID, "particles is NULL");
-    if (p->n < 0) return fail(c, GRASPH_E_INVALID, "negative particle count");
-    if (p->n > 0x7ffffff0LL) return fail(c, GRASPH_E_INVALID, "particle count exceeds 2^31 - 16");
-    if (p->n > 0 && !p->posm) return fail(c, GRASPH_E_INVALID, "posm is NULL");
-    return 0;
-}
-
-#define NEED(ptr, name)                                                                       \
-    do {                                                                                      \
-        if (!(ptr)) return grasph::fail(c, GRASPH_E_INVALID, std::string(name) + " is NULL but the flags need it"); \
-    } while (0)
 
 int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_eval)
 {
@@ -182,7 +117,6 @@ SweepLaunch sweep_base(Ctx& c, const grasph_particles* p)
     L.perm = order_of(c);
     L.n = (int)p->n;
     L.acc = (float4*)p->acc;
-    L.density = p->density;
     L.nbr = c.nbr;
     L.nbr_cnt = c.nbr_cnt;
     L.pot = c.pot;
@@ -190,6 +124,7 @@ SweepLaunch sweep_base(Ctx& c, const grasph_particles* p)
     L.vel = (const float4*)p->vel;
     L.xvel = (float4*)p->xvel;
     L.maxvsig = p->maxvsig;
+    L.ni = c.sweep_ni;
     return L;
 }
 
@@ -225,7 +160,57 @@ int run_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s)
     return 0;
 }
 
+}  // namespace
+
 // which == 0: full computeDerivatives; 1: calcDensity; 2: calcBalsara...; 3: calcAcceleration
+int check_derivative_arrays(Ctx& c, const grasph_particles* p, uint32_t f, int which)
+{
+    const bool sph = f & GRASPH_SPH;
+    const bool dens_pass = (which == 0) ? (sph && !(f & GRASPH_INTEGRATE_DENSITY)) : (which == 1);
+    const bool strain_pass =
+        (which == 0) ? (sph && ((f & GRASPH_SOLIDS) || (f & GRASPH_INTEGRATE_DENSITY) || (f & GRASPH_BALSARA_SWITCH)))
+                     : (which == 2);
+    const bool accel_pass = (which == 0) || (which == 3);
+    if (accel_pass) GRASPH_NEED(p->acc, "acc");
+    if (dens_pass) GRASPH_NEED(p->density, "density");
+    if (strain_pass) {
+        GRASPH_NEED(p->vel, "vel");
+        GRASPH_NEED(p->density, "density");
+        if (f & GRASPH_SOLIDS) { GRASPH_NEED(p->dstress, "dstress"); }
+        if (f & GRASPH_SOLIDS) { GRASPH_NEED(p->dstress_dt, "dstress_dt"); }
+        if (f & GRASPH_BALSARA_SWITCH) { GRASPH_NEED(p->balsara, "balsara"); }
+        if (f & GRASPH_INTEGRATE_DENSITY) { GRASPH_NEED(p->density_dt, "density_dt"); }
+    }
+    if (accel_pass && sph) {
+        GRASPH_NEED(p->vel, "vel");
+        GRASPH_NEED(p->density, "density");
+        if (f & GRASPH_SOLIDS) { GRASPH_NEED(p->dstress, "dstress"); }
+        if ((f & GRASPH_BALSARA_SWITCH) && which == 3) { GRASPH_NEED(p->balsara, "balsara"); }
+        if (f & GRASPH_XSPH) { GRASPH_NEED(p->xvel, "xvel"); }
+    }
+    if (accel_pass && (f & GRASPH_CLOHESSY_WILTSHIRE)) GRASPH_NEED(p->vel, "vel");
+    return 0;
+}
+
+int check_integrate_arrays(Ctx& c, const grasph_particles* p, uint32_t f)
+{
+    GRASPH_NEED(p->vel, "vel");
+    GRASPH_NEED(p->acc, "acc");
+    if ((f & GRASPH_SPH) && (f & GRASPH_INTEGRATE_DENSITY)) {
+        GRASPH_NEED(p->density, "density");
+        GRASPH_NEED(p->density_dt, "density_dt");
+    }
+    if ((f & GRASPH_SPH) && (f & GRASPH_SOLIDS)) {
+        GRASPH_NEED(p->dstress, "dstress");
+        GRASPH_NEED(p->dstress_dt, "dstress_dt");
+        if (f & GRASPH_PLASTICITY_MC) { GRASPH_NEED(p->density, "density"); }
+    }
+    if ((f & GRASPH_SPH) && (f & GRASPH_XSPH)) GRASPH_NEED(p->xvel, "xvel");
+    return 0;
+}
+
+namespace {
+
 int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream, int which)
 {
     if (!ctx) return GRASPH_E_INVALID;
@@ -241,28 +226,10 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         (which == 0) ? (sph && ((f & GRASPH_SOLIDS) || (f & GRASPH_INTEGRATE_DENSITY) || (f & GRASPH_BALSARA_SWITCH)))
                      : (which == 2);
     const bool accel_pass = (which == 0) || (which == 3);
-    const bool need_list = (which == 0 || which == 3) ? sph : (which == 2);
+    const bool need_list = (which == 0 || which == 3) ? sph : true;
     if ((which == 1 || which == 2) && !(prm->H > 0.0f)) return fail(c, GRASPH_E_INVALID, "SPH passes need H > 0");
 
-    // required arrays
-    if (accel_pass) NEED(p->acc, "acc");
-    if (dens_pass) NEED(p->density, "density");
-    if (strain_pass) {
-        NEED(p->vel, "vel");
-        NEED(p->density, "density");
-        if (f & GRASPH_SOLIDS) { NEED(p->dstress, "dstress"); }
-        if (f & GRASPH_SOLIDS) { NEED(p->dstress_dt, "dstress_dt"); }
-        if (f & GRASPH_BALSARA_SWITCH) { NEED(p->balsara, "balsara"); }
-        if (f & GRASPH_INTEGRATE_DENSITY) { NEED(p->density_dt, "density_dt"); }
-    }
-    if (accel_pass && sph) {
-        NEED(p->vel, "vel");
-        NEED(p->density, "density");
-        if (f & GRASPH_SOLIDS) { NEED(p->dstress, "dstress"); }
-        if ((f & GRASPH_BALSARA_SWITCH) && which == 3) { NEED(p->balsara, "balsara"); }
-        if (f & GRASPH_XSPH) { NEED(p->xvel, "xvel"); }
-    }
-    if (accel_pass && (f & GRASPH_CLOHESSY_WILTSHIRE)) NEED(p->vel, "vel");
+    if ((r = check_derivative_arrays(c, p, f, which))) return r;
 
     const int64_t n = p->n;
     if ((r = ensure_order(c, p, s, true))) return r;
@@ -278,13 +245,17 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     // one all-pairs sweep
     SweepLaunch L = sweep_base(c, p);
     L.grav = accel_pass && grav;
-    L.dens = dens_pass;
     L.want_nbr = need_list;
     L.want_pot = false;
     L.finish = accel_pass && !sph;
     if ((r = run_sweep(c, L, P, s))) return r;
 
     NbrLaunch N = nbr_base(c, p);
+    if (dens_pass) {
+        ProfScope ps(&c, s, KC_DENSITY);
+        hipError_t e = launch_density(N, p->density, P, s);
+        if (e != hipSuccess) return hip_fail(c, e, "density pass launch");
+    }
     if (strain_pass) {
         ProfScope ps(&c, s, KC_STRAIN);
         hipError_t e = launch_strain(N, P, s);
@@ -394,9 +365,11 @@ int grasph_destroy(grasph_ctx* ctx)
     Ctx& c = ctx->c;
     hipSetDevice(c.device);
     hipDeviceSynchronize();
+    grasph::destroy_batch_state(c.batch);
+    c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys, c.keys_alt, c.vals, c.sort_tmp, c.bbox, c.nbr, c.nbr_cnt, c.jrec, c.pot, c.red};
     for (void* b : bufs)
-        if (b) hipFree(b);
+        if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {
         hipEventDestroy(ev.a);
         hipEventDestroy(ev.b);
@@ -419,6 +392,9 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
         if (value < 0) return grasph::fail(c, GRASPH_E_INVALID, "reorder_interval must be >= 0");
         c.reorder_interval = (int)(value > 1000000000 ? 1000000000 : value);
         c.perm_n = 0;
+    } else if (k == "sweep_i_per_lane") {
+        if (value != 1 && value != 2) return grasph::fail(c, GRASPH_E_INVALID, "sweep_i_per_lane must be 1 or 2");
+        c.sweep_ni = (int)value;
     } else if (k == "profile") {
         c.profile = value != 0;
     } else {
@@ -453,20 +429,9 @@ int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_pa
     int r;
     if ((r = grasph::check_params(c, prm)) || (r = grasph::check_particles(c, p))) return r;
     if (p->n == 0) return 0;
-    const uint32_t f = prm->flags;
-    NEED(p->vel, "vel");
-    NEED(p->acc, "acc");
-    if ((f & GRASPH_SPH) && (f & GRASPH_INTEGRATE_DENSITY)) {
-        NEED(p->density, "density");
-        NEED(p->density_dt, "density_dt");
-    }
-    if ((f & GRASPH_SPH) && (f & GRASPH_SOLIDS)) {
-        NEED(p->dstress, "dstress");
-        NEED(p->dstress_dt, "dstress_dt");
-        if (f & GRASPH_PLASTICITY_MC) { NEED(p->density, "density"); }
-    }
-    if ((f & GRASPH_SPH) && (f & GRASPH_XSPH)) NEED(p->xvel, "xvel");
+    if ((r = grasph::check_integrate_arrays(c, p, prm->flags))) return r;
     grasph::IntegLaunch L;
+    std::memset(&L, 0, sizeof L);
     L.posm = (float4*)p->posm;
     L.pos64 = p->pos64;
     L.vel = (float4*)p->vel;
@@ -501,8 +466,8 @@ int grasph_reduce_invariants(grasph_ctx* ctx, const grasph_particles* p, const g
     if (!out) return grasph::fail(c, GRASPH_E_INVALID, "out is NULL");
     for (int k = 0; k < GRASPH_NINV; ++k) out[k] = 0.0;
     if (p->n == 0) return 0;
-    NEED(p->vel, "vel");
-    if (half_kick_dt != 0.0) NEED(p->acc, "acc");
+    GRASPH_NEED(p->vel, "vel");
+    if (half_kick_dt != 0.0) GRASPH_NEED(p->acc, "acc");
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = p->n;
     if (with_potential) {

@@ -6,7 +6,8 @@
 // per lane, every array read and written once, coalesced (HBM-bound).  With
 // FP64 master positions the drift runs in FP64 and the FP32 mirror is
 // rewritten from it; without them it drifts the FP32 positions as the
-// reference does.
+// reference does.  One kernel body serves one instance or a batch (block
+// descriptors and per-instance parameters from tables).
 #include "internal.hpp"
 
 namespace grasph {
@@ -14,6 +15,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ posm, double* __restrict__ pos64,
                                                           float4* __restrict__ vel, const float4* __restrict__ acc,
                                                           const float4* __restrict__ xvel,
@@ -21,11 +23,13 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
                                                           const float* __restrict__ density_dt,
                                                           float* __restrict__ dstress,
                                                           const float* __restrict__ dstress_dt, const int n,
-                                                          const int not_first_step, const DevParams P)
+                                                          const int not_first_step, const BlockTables T,
+                                                          const DevParams P0)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const size_t ns = (size_t)n;
+    const SlotMap m = slot_map<BATCH>(T, n);
+    const int i = m.slot0 + threadIdx.x;  // the integrator walks particles in index order
+    if (i >= m.slot_end) return;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const float dt = P.dt;
     const bool sph = P.flags & GRASPH_SPH;
 
@@ -48,9 +52,9 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
     float4 p = posm[i];
     double X = 0, Y = 0, Z = 0;
     if (pos64) {
-        X = pos64[i] + (double)ux * (double)dt;
-        Y = pos64[ns + i] + (double)uy * (double)dt;
-        Z = pos64[2 * ns + i] + (double)uz * (double)dt;
+        X = pos64[m.plane3(0, i)] + (double)ux * (double)dt;
+        Y = pos64[m.plane3(1, i)] + (double)uy * (double)dt;
+        Z = pos64[m.plane3(2, i)] + (double)uz * (double)dt;
         p.x = (float)X; p.y = (float)Y; p.z = (float)Z;
     } else {
         p.x = p.x + ux * dt;
@@ -69,36 +73,34 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
         }
         if (P.flags & GRASPH_SOLIDS) {  // :111-117
             Sym6 S;
-            S.xx = dstress[0 * ns + i] + dstress_dt[0 * ns + i] * dt;
-            S.yy = dstress[1 * ns + i] + dstress_dt[1 * ns + i] * dt;
-            S.zz = dstress[2 * ns + i] + dstress_dt[2 * ns + i] * dt;
-            S.xy = dstress[3 * ns + i] + dstress_dt[3 * ns + i] * dt;
-            S.xz = dstress[4 * ns + i] + dstress_dt[4 * ns + i] * dt;
-            S.yz = dstress[5 * ns + i] + dstress_dt[5 * ns + i] * dt;
+            S.xx = dstress[m.plane6(0, i)] + dstress_dt[m.plane6(0, i)] * dt;
+            S.yy = dstress[m.plane6(1, i)] + dstress_dt[m.plane6(1, i)] * dt;
+            S.zz = dstress[m.plane6(2, i)] + dstress_dt[m.plane6(2, i)] * dt;
+            S.xy = dstress[m.plane6(3, i)] + dstress_dt[m.plane6(3, i)] * dt;
+            S.xz = dstress[m.plane6(4, i)] + dstress_dt[m.plane6(4, i)] * dt;
+            S.yz = dstress[m.plane6(5, i)] + dstress_dt[m.plane6(5, i)] * dt;
             if (P.flags & GRASPH_PLASTICITY_MC)
                 plasticity(S, P.tanfr * eos_murnaghan(rho, P.rho0, P.BULK, P.dBULKdP) + P.cohesion);
             else if (P.flags & GRASPH_PLASTICITY_MIESE)
                 plasticity(S, P.Y);
-            dstress[0 * ns + i] = S.xx; dstress[1 * ns + i] = S.yy; dstress[2 * ns + i] = S.zz;
-            dstress[3 * ns + i] = S.xy; dstress[4 * ns + i] = S.xz; dstress[5 * ns + i] = S.yz;
+            dstress[m.plane6(0, i)] = S.xx; dstress[m.plane6(1, i)] = S.yy; dstress[m.plane6(2, i)] = S.zz;
+            dstress[m.plane6(3, i)] = S.xy; dstress[m.plane6(4, i)] = S.xz; dstress[m.plane6(5, i)] = S.yz;
         }
     }
 
     if (P.flags & GRASPH_SIMPLE_BOX_BOUNDARY) {  // :33-73
         const float vm = 1.0f + P.refl;
-        bool moved = false;
-        if (p.x > P.bmaxx) { p.x = P.bmaxx; v.x -= vm * v.x; X = p.x; moved = true; }
-        else if (p.x < P.bminx) { p.x = P.bminx; v.x -= vm * v.x; X = p.x; moved = true; }
-        if (p.y > P.bmaxy) { p.y = P.bmaxy; v.y -= vm * v.y; Y = p.y; moved = true; }
-        else if (p.y < P.bminy) { p.y = P.bminy; v.y -= vm * v.y; Y = p.y; moved = true; }
+        if (p.x > P.bmaxx) { p.x = P.bmaxx; v.x -= vm * v.x; X = p.x; }
+        else if (p.x < P.bminx) { p.x = P.bminx; v.x -= vm * v.x; X = p.x; }
+        if (p.y > P.bmaxy) { p.y = P.bmaxy; v.y -= vm * v.y; Y = p.y; }
+        else if (p.y < P.bminy) { p.y = P.bminy; v.y -= vm * v.y; Y = p.y; }
         if (P.dim == 3) {
-            if (p.z > P.bmaxz) { p.z = P.bmaxz; v.z -= vm * v.z; Z = p.z; moved = true; }
-            else if (p.z < P.bminz) { p.z = P.bminz; v.z -= vm * v.z; Z = p.z; moved = true; }
+            if (p.z > P.bmaxz) { p.z = P.bmaxz; v.z -= vm * v.z; Z = p.z; }
+            else if (p.z < P.bminz) { p.z = P.bminz; v.z -= vm * v.z; Z = p.z; }
         }
-        (void)moved;
     }
     if (pos64) {
-        pos64[i] = X; pos64[ns + i] = Y; pos64[2 * ns + i] = Z;
+        pos64[m.plane3(0, i)] = X; pos64[m.plane3(1, i)] = Y; pos64[m.plane3(2, i)] = Z;
     }
     posm[i] = p;
     vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -197,9 +199,13 @@ __global__ void invariants_final(double* __restrict__ partials, const int nblock
 hipError_t launch_integrate(const IntegLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
-    const int blocks = (L.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(leapfrog_kernel, dim3(blocks), dim3(kBlock), 0, s, L.posm, L.pos64, L.vel, L.acc, L.xvel,
-                       L.density, L.density_dt, L.dstress, L.dstress_dt, L.n, L.not_first_step, P);
+    if (L.T.nblocks > 0)
+        hipLaunchKernelGGL(leapfrog_kernel<true>, dim3(L.T.nblocks), dim3(kBlock), 0, s, L.posm, L.pos64, L.vel, L.acc,
+                           L.xvel, L.density, L.density_dt, L.dstress, L.dstress_dt, L.n, L.not_first_step, L.T, P);
+    else
+        hipLaunchKernelGGL(leapfrog_kernel<false>, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.posm,
+                           L.pos64, L.vel, L.acc, L.xvel, L.density, L.density_dt, L.dstress, L.dstress_dt, L.n,
+                           L.not_first_step, L.T, P);
     return hipGetLastError();
 }
 

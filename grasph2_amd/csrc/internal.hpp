@@ -11,7 +11,15 @@
 
 namespace grasph {
 
-enum KernelClass { KC_SWEEP = 0, KC_STRAIN = 1, KC_ACCEL = 2, KC_INTEGRATE = 3, KC_REORDER = 4, KC_OTHER = 5 };
+enum KernelClass {
+    KC_SWEEP = 0,
+    KC_DENSITY = 1,
+    KC_STRAIN = 2,
+    KC_ACCEL = 3,
+    KC_INTEGRATE = 4,
+    KC_REORDER = 5,
+    KC_OTHER = 6
+};
 
 // Per-device library state.  Scratch grows on demand and is reused.
 struct Ctx {
@@ -21,7 +29,12 @@ struct Ctx {
     // options
     int max_neighbours = 160;
     int reorder_interval = 1;
+    int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     bool profile = false;
+    // batched sweep tuning: i per lane per size bucket, persistent launch shape
+    int batch_ni[4] = {1, 1, 1, 1};
+    int batch_wpb = 4;
+    int batch_bpc = 8;
 
     // Morton order of the i side (slot t -> particle perm[t])
     int* perm = nullptr;
@@ -60,8 +73,11 @@ struct Ctx {
     };
     std::vector<Ev> ev_pending;
     std::vector<hipEvent_t> ev_free;
-    double ms[GRASPH_NKERNEL_CLASSES] = {0, 0, 0, 0, 0, 0};
-    int64_t launches[GRASPH_NKERNEL_CLASSES] = {0, 0, 0, 0, 0, 0};
+    double ms[GRASPH_NKERNEL_CLASSES] = {};
+    int64_t launches[GRASPH_NKERNEL_CLASSES] = {};
+    bool capturing = false;  // no timing events while a graph is being captured
+
+    struct BatchState* batch = nullptr;
 };
 
 // Bracket a kernel launch with events when profiling is on.
@@ -80,18 +96,104 @@ struct SweepLaunch {
     const int* perm;
     int n;
     float4* acc;        // gravity (written when grav)
-    float* density;     // written when dens
-    int* nbr;           // written when nbr
+    int* nbr;           // written when want_nbr
     int* nbr_cnt;
-    float* pot;         // written when pot
+    float* pot;         // written when want_pot
     int kmax;
-    bool grav, dens, want_nbr, want_pot;
+    bool grav, want_nbr, want_pot;
     bool finish;        // apply environmentAcceleration, zero xvel / maxvsig (SPH off)
     const float4* vel;
     float4* xvel;
     float* maxvsig;
+    int ni;             // i particles per lane (1 or 2)
 };
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s);
+
+// One instance of a batch, as the device kernels see it.
+struct BatchInst {
+    int off;     // first particle
+    int n;       // particle count
+    float H2;    // smoothing length squared
+    float cand;  // neighbour-candidate threshold on r^2 + H^2
+};
+
+// Block descriptors of the per-particle passes over a batch: block b covers
+// slots [blk[b].y, blk[b].z) of instance blk[b].x (never two instances).
+struct BlockTables {
+    const int4* blk;
+    const BatchInst* insts;
+    const DevParams* prm;  // per-instance parameters
+    int nblocks;           // 0: one instance, the grid covers all slots
+};
+
+// What one block of a per-particle pass works on.
+struct SlotMap {
+    int slot0, slot_end, off, n, inst;
+    // element c of particle i in an array of 6 (resp. 3) planes per instance
+    __device__ size_t plane6(int c, int i) const { return 6 * (size_t)off + (size_t)c * n + (size_t)(i - off); }
+    __device__ size_t plane3(int c, int i) const { return 3 * (size_t)off + (size_t)c * n + (size_t)(i - off); }
+};
+
+template <bool BATCH>
+__device__ inline SlotMap slot_map(const BlockTables& T, int n)
+{
+    SlotMap m;
+    if (BATCH) {
+        const int4 b = T.blk[blockIdx.x];
+        const BatchInst I = T.insts[b.x];
+        m.inst = b.x;
+        m.slot0 = b.y;
+        m.slot_end = b.z;
+        m.off = I.off;
+        m.n = I.n;
+    } else {
+        m.inst = 0;
+        m.slot0 = blockIdx.x * 256;
+        m.slot_end = n;
+        m.off = 0;
+        m.n = n;
+    }
+    return m;
+}
+
+struct BatchSweepLaunch {
+    const float4* posm;
+    const int* perm;
+    const int4* items;  // (instance, first slot, i per lane, 0) per work item, longest instance first
+    int n_items;
+    const BatchInst* insts;
+    int* queue;         // work counter, zero before the launch
+    float4* acc;
+    int* nbr;
+    int* nbr_cnt;
+    int ntotal;
+    int kmax;
+    int blocks;           // persistent grid
+    int waves_per_block;  // 1, 2 or 4
+    bool grav, want_nbr, soft;
+};
+hipError_t launch_batch_sweep(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s);
+hipError_t launch_reset_queue(int* queue, int n, hipStream_t s);
+
+// Batched Morton order: per-instance bounding boxes, keys (instance, Z-order
+// cell), one radix sort; perm maps each instance's slots into its own range.
+struct BatchOrder {
+    const float4* posm;
+    const int* inst_of;      // instance of every particle
+    const BatchInst* insts;
+    int n_inst;
+    int ntotal;
+    float* bbox;             // [n_inst][6]
+    uint64_t* keys;
+    uint64_t* keys_alt;
+    int* vals;
+    int* perm;
+    void* sort_tmp;
+    size_t sort_tmp_bytes;
+    int key_bits;
+};
+size_t batch_order_tmp_bytes(int ntotal, int key_bits);
+hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s);
 
 struct NbrLaunch {
     const float4* posm;
@@ -112,7 +214,9 @@ struct NbrLaunch {
     float4* xvel;
     float* maxvsig;
     bool acc_has_gravity;
+    BlockTables T;  // batch tables (T.nblocks == 0 for one instance)
 };
+hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P, hipStream_t s);
 hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s);
 hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s);
 hipError_t launch_accel(const NbrLaunch& L, const DevParams& P, hipStream_t s);
@@ -129,6 +233,7 @@ struct IntegLaunch {
     const float* dstress_dt;
     int n;
     int not_first_step;
+    BlockTables T;  // batch tables (T.nblocks == 0 for one instance)
 };
 hipError_t launch_integrate(const IntegLaunch& L, const DevParams& P, hipStream_t s);
 
@@ -153,4 +258,82 @@ hipError_t launch_invariants(const InvLaunch& L, const grasph_params& prm, hipSt
 size_t reorder_tmp_bytes(int n);
 hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s);
 
+// environmentAcceleration and the zero XVEL / MAXVSIG stores of
+// calcAcceleration for configurations without SPH (batched form).
+hipError_t launch_finish(const NbrLaunch& L, const DevParams& P, hipStream_t s);
+
+// ---- host helpers shared by the C-ABI units -------------------------------
+inline int fail(Ctx& c, int code, const std::string& msg)
+{
+    c.err = msg;
+    return code;
+}
+
+inline int hip_fail(Ctx& c, hipError_t e, const char* where)
+{
+    c.err = std::string(where) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return GRASPH_E_HIP;
+}
+
+template <typename T>
+int grow(Ctx& c, T*& ptr, int64_t& cap, int64_t need, const char* what)
+{
+    if (need <= cap && ptr) return 0;
+    if (ptr) {
+        (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    hipError_t e = hipMalloc((void**)&ptr, (size_t)(need > 0 ? need : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        (void)hipGetLastError();
+        return fail(c, GRASPH_E_NOMEM, std::string("device allocation of ") + what + " failed");
+    }
+    cap = need;
+    return 0;
+}
+
+inline int check_params(Ctx& c, const grasph_params* prm)
+{
+    if (!prm) return fail(c, GRASPH_E_INVALID, "params is NULL");
+    if (prm->flags & ~GRASPH_ALL_FLAGS) return fail(c, GRASPH_E_INVALID, "unknown flag bits");
+    if (prm->dimension < 1 || prm->dimension > 3) return fail(c, GRASPH_E_INVALID, "dimension must be 1, 2 or 3");
+    if (!(prm->H >= 0.0f) || !(prm->H < 3.0e38f)) return fail(c, GRASPH_E_INVALID, "H must be finite and >= 0");
+    if ((prm->flags & GRASPH_SPH) && !(prm->H > 0.0f)) return fail(c, GRASPH_E_INVALID, "SPH needs H > 0");
+    if ((prm->flags & GRASPH_SOLIDS) && (prm->flags & GRASPH_ARTIFICIAL_STRESS))
+        return fail(c, GRASPH_E_UNSUPPORTED,
+                    "SOLIDS with ARTIFICIAL_STRESS is not supported (the reference does not compile it, "
+                    "src/computeDerivatives.h:267)");
+    if ((prm->flags & GRASPH_PLASTICITY_MIESE) && (prm->flags & GRASPH_PLASTICITY_MC))
+        return fail(c, GRASPH_E_INVALID, "choose up to one plasticity model (settings.h:203-205)");
+    return 0;
+}
+
+inline int check_particles(Ctx& c, const grasph_particles* p)
+{
+    if (!p) return fail(c, GRASPH_E_INVALID, "particles is NULL");
+    if (p->n < 0) return fail(c, GRASPH_E_INVALID, "negative particle count");
+    if (p->n > 0x7ffffff0LL) return fail(c, GRASPH_E_INVALID, "particle count exceeds 2^31 - 16");
+    if (p->n > 0 && !p->posm) return fail(c, GRASPH_E_INVALID, "posm is NULL");
+    return 0;
+}
+
+// arrays each pass needs, by configuration; 0 or a GRASPH_E_INVALID code
+int check_derivative_arrays(Ctx& c, const grasph_particles* p, uint32_t f, int which);
+int check_integrate_arrays(Ctx& c, const grasph_particles* p, uint32_t f);
+
+struct BatchState;
+void destroy_batch_state(BatchState* b);
+
 }  // namespace grasph
+
+struct grasph_ctx {
+    grasph::Ctx c;
+};
+
+#define GRASPH_NEED(ptr, name)                                                                                \
+    do {                                                                                                      \
+        if (!(ptr)) return grasph::fail(c, GRASPH_E_INVALID, std::string(name) + " is NULL but the flags need it"); \
+    } while (0)

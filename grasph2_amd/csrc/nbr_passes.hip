@@ -1,22 +1,25 @@
-// nbr_passes.hip -- the two O(N k) neighbour-list passes (gfx950).
+// nbr_passes.hip -- the O(N k) neighbour-list passes (gfx950).
 //
-// strain:  calcBalsaraDensityDTDStressDT (src/computeDerivatives.h:96-162)
-//          over the neighbour list built by the sweep; also writes the
+// density: calcDensity (src/computeDerivatives.h:52-91) over the list.
+// strain:  calcBalsaraDensityDTDStressDT (:96-162); also writes the
 //          per-particle record {sigma/rho^2, balsara, density} that the
 //          acceleration pass reads for its j side, hoisting the per-pair
-//          Murnaghan pow and nine divides of computeDerivatives.h:246-257.
+//          Murnaghan pow and nine divides of :246-257.
 // prep:    the same record alone, for configurations that skip the strain
-//          pass (computeDerivatives.h:322-324).
-// accel:   the SPH part of calcAcceleration (computeDerivatives.h:237-295)
-//          plus environmentAcceleration (:300-312), added to the gravity the
+//          pass (:322-324).
+// accel:   the SPH part of calcAcceleration (:237-295) plus
+//          environmentAcceleration (:300-312), added to the gravity the
 //          sweep left in acc with compensated (TwoSum) accumulation.
 //
 // One particle per lane in Morton order (slot t -> i = perm[t]); the list is
-// stored slot-major, nbr[k * n + t], so a wavefront's loads of its k-th
-// neighbours are contiguous.  Lists are in increasing j, the reference's
-// summation order.  A particle whose count exceeds the list capacity falls
-// back to a scan over all j with the exact neighbour test, so a full list
-// never loses partners.
+// stored slot-major, nbr[k * nstride + t], so a wavefront's loads of its
+// k-th neighbours are contiguous.  Lists are in increasing j, the
+// reference's summation order.  A particle whose count exceeds the list
+// capacity falls back to a scan over its instance with the exact neighbour
+// test, so a full list never loses partners.
+//
+// Every kernel serves one instance (SlotMap from the kernel arguments) or a
+// batch (SlotMap and parameters of the block's instance from tables).
 #include "internal.hpp"
 
 namespace grasph {
@@ -51,19 +54,59 @@ __device__ inline float dsdt_entry(const Tens& ed, const Tens& rd, const Tens& S
     return o;
 }
 
-__device__ inline float ld(const float* p, size_t k) { return p[k]; }
+// Walk particle i's neighbours: the list, or on overflow the instance scan.
+template <typename F>
+__device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ posm, const int* __restrict__ nbr,
+                                                   const int cnt, const size_t nstride, const int t, const int kmax,
+                                                   const float4 pi, const SlotMap& m, const float H2, F&& body)
+{
+    if (cnt <= kmax) {
+        for (int k = 0; k < cnt; ++k) body(nbr[(size_t)k * nstride + t]);
+    } else {
+        for (int j = m.off; j < m.off + m.n; ++j) {
+            const float4 pj = posm[j];
+            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
+        }
+    }
+}
 
-template <bool SOLIDS, bool BALS, bool INTD>
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
+                                                         const int* __restrict__ nbr, const int* __restrict__ nbr_cnt,
+                                                         const size_t nstride, const int kmax,
+                                                         float* __restrict__ density, const BlockTables T,
+                                                         const DevParams P0)
+{
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
+    const int t = m.slot0 + threadIdx.x;
+    if (t >= m.slot_end) return;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
+    const int i = perm ? perm[t] : t;
+    const float4 pi = posm[i];
+    CSum rho{0.0f, 0.0f};
+    for_each_neighbour(posm, nbr, nbr_cnt[t], nstride, t, kmax, pi, m, P.H2, [&](int j) {
+        const float4 pj = posm[j];
+        const float r = sqrtf(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
+        rho.add(pj.w * w_spline(r, P.H, P.Wpref));
+    });
+    float r = rho.value();
+    if ((P.flags & GRASPH_DEAL_WITH_NO_PARTNERS) && r < 0.001f) r = P.rho0 * 0.01f;  // :85-88
+    density[i] = r;
+}
+
+template <bool BATCH, bool SOLIDS, bool BALS, bool INTD>
 __global__ __launch_bounds__(kBlock) void strain_kernel(
     const float4* __restrict__ posm, const float4* __restrict__ vel, const float* __restrict__ density,
     const float* __restrict__ dstress, const int* __restrict__ perm, const int* __restrict__ nbr,
-    const int* __restrict__ nbr_cnt, const int n, const int kmax, float* __restrict__ balsara,
-    float* __restrict__ density_dt, float* __restrict__ dstress_dt, float4* __restrict__ jrec, const DevParams P)
+    const int* __restrict__ nbr_cnt, const size_t nstride, const int kmax, float* __restrict__ balsara,
+    float* __restrict__ density_dt, float* __restrict__ dstress_dt, float4* __restrict__ jrec, const BlockTables T,
+    const DevParams P0)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
+    const int t = m.slot0 + threadIdx.x;
+    if (t >= m.slot_end) return;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const size_t ns = (size_t)n;
     const float4 pi = posm[i];
     const float4 vi = vel[i];
     const float rhoi = density[i];
@@ -72,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 
-    auto body = [&](int j) {
+    for_each_neighbour(posm, nbr, nbr_cnt[t], nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = posm[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
@@ -101,23 +144,13 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
             cz += f * (vx * gy - vy * gx);
         }
         if (BALS || INTD) divv -= (mj / rhoj) * (vx * gx + vy * gy + vz * gz);  // :137
-    };
-
-    const int cnt = nbr_cnt[t];
-    if (cnt <= kmax) {
-        for (int k = 0; k < cnt; ++k) body(nbr[(size_t)k * ns + t]);
-    } else {
-        for (int j = 0; j < n; ++j) {
-            const float4 pj = posm[j];
-            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), P.H2)) body(j);
-        }
-    }
+    });
 
     // do_after, computeDerivatives.h:144-161
     Tens S = sym_to_full(0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
     if (SOLIDS)
-        S = sym_to_full(ld(dstress, 0 * ns + i), ld(dstress, 1 * ns + i), ld(dstress, 2 * ns + i),
-                        ld(dstress, 3 * ns + i), ld(dstress, 4 * ns + i), ld(dstress, 5 * ns + i));
+        S = sym_to_full(dstress[m.plane6(0, i)], dstress[m.plane6(1, i)], dstress[m.plane6(2, i)],
+                        dstress[m.plane6(3, i)], dstress[m.plane6(4, i)], dstress[m.plane6(5, i)]);
     float sdt[6] = {0, 0, 0, 0, 0, 0};
     if (SOLIDS) {
         const Tens ed = sym_to_full(exx, eyy, ezz, exy, exz, eyz);
@@ -143,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     if (density_dt) density_dt[i] = INTD ? -rhoi * divv : 0.0f;
     if (dstress_dt) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) dstress_dt[c * ns + i] = sdt[c];
+        for (int c = 0; c < 6; ++c) dstress_dt[m.plane6(c, i)] = sdt[c];
     }
 
     // j-side record of the acceleration pass (its do_before, :193-219)
@@ -161,22 +194,27 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
 }
 
 // record only (no strain pass in this configuration)
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void prep_kernel(const float* __restrict__ density,
                                                       const float* __restrict__ dstress,
-                                                      const float* __restrict__ balsara, const int n,
-                                                      float4* __restrict__ jrec, const int solids, const DevParams P)
+                                                      const float* __restrict__ balsara, const int* __restrict__ perm,
+                                                      const size_t nstride, float4* __restrict__ jrec,
+                                                      const BlockTables T, const DevParams P0)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const size_t ns = (size_t)n;
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
+    const int t = m.slot0 + threadIdx.x;
+    if (t >= m.slot_end) return;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
+    const int i = perm ? perm[t] : t;
     const float rhoi = density[i];
     const float bal = balsara ? balsara[i] : 0.0f;
-    if (solids) {
+    if (P.flags & GRASPH_SOLIDS) {
         const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
         const float d2 = rhoi * rhoi;
-        jrec[2 * (size_t)i] = make_float4((dstress[0 * ns + i] - pres) / d2, (dstress[1 * ns + i] - pres) / d2,
-                                          (dstress[2 * ns + i] - pres) / d2, dstress[3 * ns + i] / d2);
-        jrec[2 * (size_t)i + 1] = make_float4(dstress[4 * ns + i] / d2, dstress[5 * ns + i] / d2, bal, rhoi);
+        jrec[2 * (size_t)i] =
+            make_float4((dstress[m.plane6(0, i)] - pres) / d2, (dstress[m.plane6(1, i)] - pres) / d2,
+                        (dstress[m.plane6(2, i)] - pres) / d2, dstress[m.plane6(3, i)] / d2);
+        jrec[2 * (size_t)i + 1] = make_float4(dstress[m.plane6(4, i)] / d2, dstress[m.plane6(5, i)] / d2, bal, rhoi);
     } else {
         const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
         jrec[2 * (size_t)i] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
@@ -184,17 +222,18 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(const float* __restrict__ 
     }
 }
 
-template <bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
+template <bool BATCH, bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
 __global__ __launch_bounds__(kBlock) void accel_kernel(
     const float4* __restrict__ posm, const float4* __restrict__ vel, const float4* __restrict__ jrec,
-    const int* __restrict__ perm, const int* __restrict__ nbr, const int* __restrict__ nbr_cnt, const int n,
+    const int* __restrict__ perm, const int* __restrict__ nbr, const int* __restrict__ nbr_cnt, const size_t nstride,
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
-    const int acc_has_gravity, const DevParams P)
+    const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
+    const int t = m.slot0 + threadIdx.x;
+    if (t >= m.slot_end) return;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const size_t ns = (size_t)n;
     const float4 pi = posm[i];
     const float4 vi = vel[i];
     const float4 ra = jrec[2 * (size_t)i], rb = jrec[2 * (size_t)i + 1];
@@ -211,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    auto body = [&](int j) {
+    for_each_neighbour(posm, nbr, nbr_cnt[t], nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = posm[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
@@ -260,17 +299,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
             xvy += f * (vj.y - vi.y) * w;
             xvz += f * (vj.z - vi.z) * w;
         }
-    };
-
-    const int cnt = nbr_cnt[t];
-    if (cnt <= kmax) {
-        for (int k = 0; k < cnt; ++k) body(nbr[(size_t)k * ns + t]);
-    } else {
-        for (int j = 0; j < n; ++j) {
-            const float4 pj = posm[j];
-            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), P.H2)) body(j);
-        }
-    }
+    });
 
     float fx = ax.value(), fy = ay.value(), fz = az.value();
     environment_acceleration(P, pi.x, pi.y, pi.z, vi.x, vi.y, vi.z, fx, fy, fz);
@@ -279,18 +308,44 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     if (xvel) xvel[i] = make_float4(xvx, xvy, xvz, 0.0f);
 }
 
-}  // namespace
-
-hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+// Without SPH: environmentAcceleration on the gravity and the zero XVEL /
+// MAXVSIG of calcAcceleration's store (computeDerivatives.h:171, 300-312).
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void finish_kernel(const float4* __restrict__ posm,
+                                                        const float4* __restrict__ vel, float4* __restrict__ acc,
+                                                        float4* __restrict__ xvel, float* __restrict__ maxvsig,
+                                                        const int n, const BlockTables T, const DevParams P0)
 {
-    if (L.n <= 0) return hipSuccess;
+    const SlotMap m = slot_map<BATCH>(T, n);
+    const int i = m.slot0 + threadIdx.x;
+    if (i >= m.slot_end) return;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
+    const bool grav = P.flags & GRASPH_SELF_GRAVITY;
+    if (!grav || (P.flags & (GRASPH_CLOHESSY_WILTSHIRE | GRASPH_CONSTANT_ACCELERATION))) {
+        const float4 p = posm[i];
+        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 a = grav ? acc[i] : make_float4(0.f, 0.f, 0.f, 0.f);  // no sweep ran without gravity
+        environment_acceleration(P, p.x, p.y, p.z, v.x, v.y, v.z, a.x, a.y, a.z);
+        acc[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    }
+    if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (maxvsig) maxvsig[i] = 0.0f;
+}
+
+inline dim3 grid_of(const NbrLaunch& L)
+{
+    return dim3(L.T.nblocks > 0 ? L.T.nblocks : (L.n + kBlock - 1) / kBlock);
+}
+
+template <bool BATCH>
+hipError_t strain_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
     const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
                in = P.flags & GRASPH_INTEGRATE_DENSITY;
-    const int blocks = (L.n + kBlock - 1) / kBlock;
-#define GRASPH_STRAIN(A, B, C)                                                                                       \
-    hipLaunchKernelGGL((strain_kernel<A, B, C>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.vel, L.density,          \
-                       L.dstress, L.perm, L.nbr, L.nbr_cnt, L.n, L.kmax, L.balsara, L.density_dt, L.dstress_dt, L.jrec, \
-                       P)
+#define GRASPH_STRAIN(A, B, C)                                                                                     \
+    hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.density,   \
+                       L.dstress, L.perm, L.nbr, L.nbr_cnt, (size_t)L.n, L.kmax, L.balsara, L.density_dt,           \
+                       L.dstress_dt, L.jrec, L.T, P)
     const int key = (so ? 4 : 0) | (ba ? 2 : 0) | (in ? 1 : 0);
     switch (key) {
         case 0: GRASPH_STRAIN(false, false, false); break;
@@ -306,25 +361,16 @@ hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+template <bool BATCH>
+hipError_t accel_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
-    if (L.n <= 0) return hipSuccess;
-    const int blocks = (L.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(prep_kernel, dim3(blocks), dim3(kBlock), 0, s, L.density, L.dstress, L.balsara_in, L.n, L.jrec,
-                       (P.flags & GRASPH_SOLIDS) ? 1 : 0, P);
-    return hipGetLastError();
-}
-
-hipError_t launch_accel(const NbrLaunch& L, const DevParams& P, hipStream_t s)
-{
-    if (L.n <= 0) return hipSuccess;
     const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
                av = P.flags & GRASPH_ARTIFICIAL_VISCOSITY, xs = P.flags & GRASPH_XSPH,
                ar = P.flags & GRASPH_ARTIFICIAL_STRESS;
-    const int blocks = (L.n + kBlock - 1) / kBlock;
-#define GRASPH_ACCEL(A, B, C, D, E)                                                                                   \
-    hipLaunchKernelGGL((accel_kernel<A, B, C, D, E>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.vel, L.jrec, L.perm, \
-                       L.nbr, L.nbr_cnt, L.n, L.kmax, L.acc, L.xvel, L.maxvsig, L.acc_has_gravity ? 1 : 0, P)
+#define GRASPH_ACCEL(A, B, C, D, E)                                                                                  \
+    hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.jrec,  \
+                       L.perm, L.nbr, L.nbr_cnt, (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig,                      \
+                       L.acc_has_gravity ? 1 : 0, L.T, P)
     // SOLIDS + ARTIFICIAL_STRESS is rejected before launch (the reference does not compile it).
     const int key = (so ? 16 : 0) | (ba ? 8 : 0) | (av ? 4 : 0) | (xs ? 2 : 0) | (ar ? 1 : 0);
     switch (key) {
@@ -355,6 +401,56 @@ hipError_t launch_accel(const NbrLaunch& L, const DevParams& P, hipStream_t s)
         default: return hipErrorInvalidValue;
     }
 #undef GRASPH_ACCEL
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    if (L.T.nblocks > 0)
+        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, L.nbr, L.nbr_cnt,
+                           (size_t)L.n, L.kmax, density, L.T, P);
+    else
+        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, L.nbr, L.nbr_cnt,
+                           (size_t)L.n, L.kmax, density, L.T, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    return L.T.nblocks > 0 ? strain_b<true>(L, P, s) : strain_b<false>(L, P, s);
+}
+
+hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    if (L.T.nblocks > 0)
+        hipLaunchKernelGGL(prep_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.density, L.dstress, L.balsara_in,
+                           L.perm, (size_t)L.n, L.jrec, L.T, P);
+    else
+        hipLaunchKernelGGL(prep_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.density, L.dstress, L.balsara_in,
+                           L.perm, (size_t)L.n, L.jrec, L.T, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_accel(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    return L.T.nblocks > 0 ? accel_b<true>(L, P, s) : accel_b<false>(L, P, s);
+}
+
+hipError_t launch_finish(const NbrLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    if (L.T.nblocks > 0)
+        hipLaunchKernelGGL(finish_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.acc, L.xvel,
+                           L.maxvsig, L.n, L.T, P);
+    else
+        hipLaunchKernelGGL(finish_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.acc, L.xvel,
+                           L.maxvsig, L.n, L.T, P);
     return hipGetLastError();
 }
 

@@ -82,7 +82,78 @@ __global__ __launch_bounds__(kBlock) void morton_keys(const float4* __restrict__
     vals[i] = i;
 }
 
+// one block per instance: bounding box of its particles
+__global__ __launch_bounds__(kBlock) void batch_bbox(const float4* __restrict__ posm,
+                                                     const BatchInst* __restrict__ insts, float* __restrict__ box)
+{
+    const BatchInst I = insts[blockIdx.x];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = I.off + threadIdx.x; i < I.off + I.n; i += kBlock) {
+        const float4 p = posm[i];
+        mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
+    }
+    __shared__ float sh[6][kBlock];
+    for (int d = 0; d < 3; ++d) {
+        sh[d][threadIdx.x] = mn[d];
+        sh[3 + d][threadIdx.x] = mx[d];
+    }
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; ++d) {
+                sh[d][threadIdx.x] = fminf(sh[d][threadIdx.x], sh[d][threadIdx.x + s]);
+                sh[3 + d][threadIdx.x] = fmaxf(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) box[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBlock) void batch_keys(const float4* __restrict__ posm, const int* __restrict__ inst_of,
+                                                     const int n, const float* __restrict__ box,
+                                                     uint64_t* __restrict__ keys, int* __restrict__ vals)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int k = inst_of[i];
+    const float4 p = posm[i];
+    const float c[3] = {p.x, p.y, p.z};
+    uint32_t q[3];
+    for (int d = 0; d < 3; ++d) {
+        const float lo = box[6 * k + d], hi = box[6 * k + 3 + d];
+        const float ext = hi - lo;
+        float u = (ext > 0.0f) ? (c[d] - lo) / ext : 0.0f;
+        if (!(u >= 0.0f)) u = 0.0f;
+        if (u > 1.0f) u = 1.0f;
+        q[d] = (uint32_t)(u * 1023.0f);
+    }
+    keys[i] = ((uint64_t)k << 30) | (uint64_t)((spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]));
+    vals[i] = i;
+}
+
 }  // namespace
+
+size_t batch_order_tmp_bytes(int ntotal, int key_bits)
+{
+    size_t bytes = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr,
+                                       (int*)nullptr, ntotal, 0, key_bits, (hipStream_t)0);
+    return bytes;
+}
+
+hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s)
+{
+    if (B.ntotal <= 0 || B.n_inst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_bbox, dim3(B.n_inst), dim3(kBlock), 0, s, B.posm, B.insts, B.bbox);
+    hipLaunchKernelGGL(batch_keys, dim3((B.ntotal + kBlock - 1) / kBlock), dim3(kBlock), 0, s, B.posm, B.inst_of,
+                       B.ntotal, B.bbox, B.keys, B.vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = B.sort_tmp_bytes;
+    return hipcub::DeviceRadixSort::SortPairs(B.sort_tmp, bytes, B.keys, B.keys_alt, B.vals, B.perm, B.ntotal, 0,
+                                              B.key_bits, s);
+}
 
 size_t reorder_tmp_bytes(int n)
 {

@@ -1,174 +1,349 @@
 // sweep.hip -- the O(N^2) all-pairs sweep (gfx950).
 //
 // One sweep serves the three pair jobs of src/computeDerivatives.h: it sums
-// the softened gravity of calcAcceleration (:229-235) over every pair, the
-// SPH density of calcDensity (:68-80) over neighbour pairs, and records the
-// neighbour list (0 < r^2 <= H^2, :72,122,238) that the two O(N k) passes of
-// nbr_passes.hip walk.  It replaces the tiled device loop
+// the softened gravity of calcAcceleration (:229-235) over every pair and
+// records the neighbour list (0 < r^2 <= H^2, :72,122,238) that the O(N k)
+// passes of nbr_passes.hip walk for the SPH sums (density, strain rate,
+// SPH acceleration).  It replaces the tiled device loop
 // do_for_each_pair_fast_impl (src/particles/algorithms.h:270-309).
 //
-// Mapping: one i particle per lane, i taken in Morton order (perm) so that a
-// wavefront's 64 particles are spatial neighbours and the neighbour branch is
-// coherent.  All lanes visit the same j at the same time, so the j stream
-// (float4 x,y,z,m) is read with scalar loads into SGPRs -- no LDS, no
-// barriers -- prefetched one chunk of 8 ahead.  Per pair: 3 sub, 3 FMA for
-// r^2 + H^2, one v_rsq_f32, 3 mul, 3 FMA; neighbour candidates take a
-// wave-uniform branch.  Sums are plain FP32 over blocks of 64 j and then
-// compensated (TwoSum) into the running total.
+// Mapping: NI i particles per lane (slots t and t + 64 for NI = 2), i taken
+// in Morton order (perm) so that a wavefront's particles are spatial
+// neighbours and the neighbour branch is coherent.  All lanes visit the same
+// j at the same time, so the j stream (float4 x,y,z,m) is read with scalar
+// loads into SGPRs -- no LDS, no barriers -- in chunks of 8, two SGPR
+// buffers in ping-pong so the next chunk loads while this one computes.
+// Per pair: 3 sub, 3 FMA for r^2 + H^2, one v_rsq_f32, 3 mul, 3 FMA, and a
+// v_min towards one wave-uniform neighbour-candidate test per chunk; only
+// chunks where some lane has a candidate enter the (rare) neighbour path.
+// Sums are plain FP32 over blocks of 128 j, then compensated (TwoSum) into
+// the running total.
+//
+// The batched kernel runs the same code persistently over a queue of work
+// items (instance, 64 NI slots) ordered longest instance first; each
+// wavefront takes the next item with one atomic add.
 #include "internal.hpp"
 
 namespace grasph {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kChunk = 8;   // j per prefetch chunk
-constexpr int kFold = 64;   // j per plain partial sum before the compensated add
+constexpr int kChunk = 8;    // j per prefetch chunk and per neighbour-candidate test
+constexpr int kFold = 128;   // j per plain partial sum before the compensated add
 
+template <int NI>
 struct Acc {
-    float ax, ay, az, rho, phi;
-    CSum sx, sy, sz, srho, sphi;
-    int cnt;
+    float ax[NI], ay[NI], az[NI], phi[NI];
+    CSum sx[NI], sy[NI], sz[NI], sphi[NI];
+    int cnt[NI];
 };
 
-template <bool GRAV, bool BRANCH, bool POT, bool SOFT>
-__device__ __forceinline__ void pair(const float4 pj, const int j, const float xi, const float yi, const float zi,
-                                     Acc& a, int* __restrict__ nbr, const size_t nstride, const int slot,
-                                     const int kmax, const bool dens, const bool want_nbr, const DevParams& P)
+// Constants of the rare path, held in VGPRs: values the compiler keeps in
+// SGPRs would, under the SGPR pressure of the j buffers, be re-read from
+// the kernel arguments, and that wait would also stall on the j prefetch.
+struct Rare {
+    int* nbr;
+    size_t nstride;
+    int kmax;
+    float h2;
+};
+
+__device__ __forceinline__ int vgpr_i32(int x)
 {
-    const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-    float d2;
-    if (SOFT)
-        d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, P.H2)));
-    else
-        d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    float inv = 0.0f;
-    if (GRAV || POT) inv = __builtin_amdgcn_rsqf(d2);
-    if (GRAV) {
-        float s = pj.w * inv;
-        s = s * (inv * inv);
-        if (!SOFT) s = (d2 > 0.0f) ? s : 0.0f;  // r^2 > 0 (computeDerivatives.h:226)
-        a.ax = fmaf(dx, s, a.ax);
-        a.ay = fmaf(dy, s, a.ay);
-        a.az = fmaf(dz, s, a.az);
+    int r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+__device__ __forceinline__ float vgpr_f32(float x) { return __int_as_float(vgpr_i32(__float_as_int(x))); }
+__device__ __forceinline__ uint64_t vgpr_u64(uint64_t x)
+{
+    const uint32_t lo = (uint32_t)vgpr_i32((int)(uint32_t)x), hi = (uint32_t)vgpr_i32((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ Rare make_rare(int* nbr, size_t nstride, int kmax, float h2)
+{
+    Rare r;
+    r.nbr = (int*)vgpr_u64((uint64_t)nbr);
+    r.nstride = (size_t)vgpr_u64((uint64_t)nstride);
+    r.kmax = vgpr_i32(kmax);
+    r.h2 = vgpr_f32(h2);
+    return r;
+}
+
+// Rare path: the pair may be an SPH neighbour (d2 <= cand for this lane).
+// Only the exact test and the list append live here.
+template <bool NBR, bool POT>
+__device__ __forceinline__ void neighbour_pair(const float4 pj, const int j, const float xi, const float yi,
+                                               const float zi, const float d2, float& phi, int& cnt,
+                                               const int slot, const Rare& R)
+{
+    // rij = pi - pj, r2 = dot(rij, rij) exactly as the oracle rounds it
+    const float r2 = dist2_exact(xi - pj.x, yi - pj.y, zi - pj.z);
+    if (POT && r2 == 0.0f) phi -= pj.w * __builtin_amdgcn_rsqf(d2);  // the self pair was added on the fast path
+    if (NBR && is_neighbour(r2, R.h2)) {
+        if (cnt < R.kmax) R.nbr[(size_t)cnt * R.nstride + slot] = j;
+        ++cnt;
     }
-    float pc = 0.0f;
-    if (POT) pc = (d2 > 0.0f) ? pj.w * inv : 0.0f;
-    if (BRANCH) {
-        if (d2 <= P.cand) {
-            // rij = pi - pj, r2 = dot(rij, rij) exactly as the oracle rounds it
-            const float r2 = dist2_exact(-dx, -dy, -dz);
-            if (POT && r2 == 0.0f) pc = 0.0f;
-            if (is_neighbour(r2, P.H2)) {
-                if (dens) {
-                    const float r = sqrtf(r2);
-                    a.rho += pj.w * w_spline(r, P.H, P.Wpref);
-                }
-                if (want_nbr) {
-                    if (a.cnt < kmax) nbr[(size_t)a.cnt * nstride + slot] = j;
-                    ++a.cnt;
-                }
+}
+
+// One chunk of kChunk j against the lane's NI particles: gravity (and
+// potential) on every pair, then one wave-uniform candidate test.
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+__device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, const float (&xi)[NI],
+                                      const float (&yi)[NI], const float (&zi)[NI], Acc<NI>& a, const int slot,
+                                      const float H2, const float cand, const Rare& R)
+{
+    constexpr bool BRANCH = NBR || (POT && SOFT);
+    float d2[NI][kChunk];
+#pragma unroll
+    for (int c = 0; c < kChunk; ++c) {
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const float dx = pj[c].x - xi[q], dy = pj[c].y - yi[q], dz = pj[c].z - zi[q];
+            if (SOFT)
+                d2[q][c] = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, H2)));
+            else
+                d2[q][c] = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float inv = 0.0f;
+            if (GRAV || POT) inv = __builtin_amdgcn_rsqf(d2[q][c]);
+            if (GRAV) {
+                float s = pj[c].w * inv;
+                s = s * (inv * inv);
+                if (!SOFT) s = (d2[q][c] > 0.0f) ? s : 0.0f;  // r^2 > 0 (computeDerivatives.h:226)
+                a.ax[q] = fmaf(dx, s, a.ax[q]);
+                a.ay[q] = fmaf(dy, s, a.ay[q]);
+                a.az[q] = fmaf(dz, s, a.az[q]);
             }
+            if (POT) a.phi[q] += SOFT ? pj[c].w * inv : ((d2[q][c] > 0.0f) ? pj[c].w * inv : 0.0f);
         }
     }
-    if (POT) a.phi += pc;
-}
-
-__device__ __forceinline__ void fold(Acc& a)
-{
-    a.sx.add(a.ax); a.sy.add(a.ay); a.sz.add(a.az); a.srho.add(a.rho); a.sphi.add(a.phi);
-    a.ax = a.ay = a.az = a.rho = a.phi = 0.0f;
-}
-
-// Sweep of i slots [t, t+1) (one per lane) against j in [j0, j1).
-template <bool GRAV, bool BRANCH, bool POT, bool SOFT>
-__device__ __forceinline__ void sweep_range(const float4* __restrict__ posm, const int j0, const int j1,
-                                            const float xi, const float yi, const float zi, Acc& a,
-                                            int* __restrict__ nbr, const size_t nstride, const int slot,
-                                            const int kmax, const bool dens, const bool want_nbr,
-                                            const DevParams& P)
-{
-    const int nfull = j0 + ((j1 - j0) / kChunk) * kChunk;
-    float4 cur[kChunk];
-    if (nfull > j0) {
+    if (BRANCH) {
+        float mn = d2[0][0];
 #pragma unroll
-        for (int c = 0; c < kChunk; ++c) cur[c] = posm[j0 + c];
+        for (int q = 0; q < NI; ++q)
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c) mn = fminf(mn, d2[q][c]);
+        if (mn <= cand) {
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c)
+#pragma unroll
+                for (int q = 0; q < NI; ++q)
+                    if (d2[q][c] <= cand)
+                        neighbour_pair<NBR, POT && SOFT>(pj[c], jb + c, xi[q], yi[q], zi[q], d2[q][c], a.phi[q],
+                                                         a.cnt[q], slot + 64 * q, R);
+        }
     }
-    for (int jb = j0; jb < nfull; jb += kChunk) {
-        float4 nxt[kChunk];
-        const int jn = (jb + kChunk < nfull) ? (jb + kChunk) : jb;
+}
+
+template <int NI, bool GRAV, bool POT>
+__device__ __forceinline__ void fold(Acc<NI>& a)
+{
 #pragma unroll
-        for (int c = 0; c < kChunk; ++c) nxt[c] = posm[jn + c];
+    for (int q = 0; q < NI; ++q) {
+        if (GRAV) {
+            a.sx[q].add(a.ax[q]); a.sy[q].add(a.ay[q]); a.sz[q].add(a.az[q]);
+            a.ax[q] = a.ay[q] = a.az[q] = 0.0f;
+        }
+        if (POT) {
+            a.sphi[q].add(a.phi[q]);
+            a.phi[q] = 0.0f;
+        }
+    }
+}
+
+// Sweep of the lane's NI slots against j in [j0, j1).
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+__device__ __forceinline__ void sweep_range(const float4* __restrict__ posm, const int j0, const int j1,
+                                            const float (&xi)[NI], const float (&yi)[NI], const float (&zi)[NI],
+                                            Acc<NI>& a, const int slot, const float H2, const float cand,
+                                            const Rare& R)
+{
+    // Two SGPR buffers in ping-pong: while one chunk is computed the next one
+    // loads into the other buffer, so no register copies are needed.
+    constexpr int kFoldChunks = kFold / kChunk;
+    const int nchunks = (j1 - j0) / kChunk;
+    const int nfull = j0 + nchunks * kChunk;
+    float4 A[kChunk], B[kChunk];
+    if (nchunks > 0) {
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) A[c] = posm[j0 + c];
+    }
+    int k = 0;
+    for (; k + 1 < nchunks; k += 2) {
+        const int jb = j0 + k * kChunk;
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) B[c] = posm[jb + kChunk + c];
+        chunk<NI, GRAV, NBR, POT, SOFT>(A, jb, xi, yi, zi, a, slot, H2, cand, R);
+        if ((k % kFoldChunks) == kFoldChunks - 1) fold<NI, GRAV, POT>(a);
+        // next A: chunk k + 2, or chunk k + 1 again when k + 2 is past the end
+        const int ja = j0 + ((k + 2 < nchunks) ? (k + 2) : (k + 1)) * kChunk;
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) A[c] = posm[ja + c];
+        chunk<NI, GRAV, NBR, POT, SOFT>(B, jb + kChunk, xi, yi, zi, a, slot, H2, cand, R);
+        if (((k + 1) % kFoldChunks) == kFoldChunks - 1) fold<NI, GRAV, POT>(a);
+    }
+    if (k < nchunks) chunk<NI, GRAV, NBR, POT, SOFT>(A, j0 + k * kChunk, xi, yi, zi, a, slot, H2, cand, R);
+    // tail: pad the last partial chunk with far-away massless partners
+    if (nfull < j1) {
+        float4 t[kChunk];
 #pragma unroll
         for (int c = 0; c < kChunk; ++c)
-            pair<GRAV, BRANCH, POT, SOFT>(cur[c], jb + c, xi, yi, zi, a, nbr, nstride, slot, kmax, dens, want_nbr, P);
-        if (((jb - j0) & (kFold - 1)) == kFold - kChunk) fold(a);
-#pragma unroll
-        for (int c = 0; c < kChunk; ++c) cur[c] = nxt[c];
+            t[c] = (nfull + c < j1) ? posm[nfull + c] : make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
+        chunk<NI, GRAV, NBR, POT, SOFT>(t, nfull, xi, yi, zi, a, slot, H2, cand, R);
     }
-    for (int j = nfull; j < j1; ++j)
-        pair<GRAV, BRANCH, POT, SOFT>(posm[j], j, xi, yi, zi, a, nbr, nstride, slot, kmax, dens, want_nbr, P);
-    fold(a);
+    fold<NI, GRAV, POT>(a);
 }
 
-template <bool GRAV, bool BRANCH, bool POT, bool SOFT>
-__global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
-                                                       const int n, float4* __restrict__ acc,
-                                                       float* __restrict__ density, int* __restrict__ nbr,
-                                                       int* __restrict__ nbr_cnt, float* __restrict__ pot,
-                                                       const int kmax, const int dens, const int want_nbr,
-                                                       const int finish, const float4* __restrict__ vel,
-                                                       float4* __restrict__ xvel, float* __restrict__ maxvsig,
-                                                       const DevParams P)
+// Load the lane's particles (slots s0 + 64 q below s_end), run the sweep
+// over [j0, j1) and store the results.
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+__device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, const int* __restrict__ perm,
+                                            const int s0, const int s_end, const int j0, const int j1,
+                                            float4* __restrict__ acc, int* __restrict__ nbr_cnt,
+                                            float* __restrict__ pot, const int finish,
+                                            const float4* __restrict__ vel, float4* __restrict__ xvel,
+                                            float* __restrict__ maxvsig, const DevParams& P, const float H2,
+                                            const float cand, const Rare& R)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = t < n;
-    const int i = active ? (perm ? perm[t] : t) : 0;
-    float xi = __int_as_float(0x7fc00000), yi = xi, zi = xi;  // NaN: inactive lanes never branch
-    if (active) {
-        const float4 p = posm[i];
-        xi = p.x; yi = p.y; zi = p.z;
-    }
-    Acc a;
-    a.ax = a.ay = a.az = a.rho = a.phi = 0.0f;
-    a.sx = a.sy = a.sz = a.srho = a.sphi = CSum{0.0f, 0.0f};
-    a.cnt = 0;
-    sweep_range<GRAV, BRANCH, POT, SOFT>(posm, 0, n, xi, yi, zi, a, nbr, (size_t)n, t, kmax, dens != 0,
-                                         want_nbr != 0, P);
-    if (!active) return;
-    if (GRAV) {
-        float ax = a.sx.value(), ay = a.sy.value(), az = a.sz.value();
-        if (finish) {
-            const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            environment_acceleration(P, xi, yi, zi, v.x, v.y, v.z, ax, ay, az);
+    float xi[NI], yi[NI], zi[NI];
+    int ii[NI];
+    bool act[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int t = s0 + 64 * q;
+        act[q] = t < s_end;
+        ii[q] = act[q] ? (perm ? perm[t] : t) : 0;
+        xi[q] = yi[q] = zi[q] = __int_as_float(0x7fc00000);  // NaN: inactive slots never branch
+        if (act[q]) {
+            const float4 p = posm[ii[q]];
+            xi[q] = p.x; yi[q] = p.y; zi[q] = p.z;
         }
-        acc[i] = make_float4(ax, ay, az, 0.0f);
-    } else if (finish && acc) {
-        float ax = 0.f, ay = 0.f, az = 0.f;
-        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        environment_acceleration(P, xi, yi, zi, v.x, v.y, v.z, ax, ay, az);
-        acc[i] = make_float4(ax, ay, az, 0.0f);
     }
-    if (finish) {
-        if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (maxvsig) maxvsig[i] = 0.0f;
+    Acc<NI> a;
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        a.ax[q] = a.ay[q] = a.az[q] = a.phi[q] = 0.0f;
+        a.sx[q] = a.sy[q] = a.sz[q] = a.sphi[q] = CSum{0.0f, 0.0f};
+        a.cnt[q] = 0;
     }
-    if (BRANCH && dens) {
-        float rho = a.srho.value();
-        // computeDerivatives.h:85-88
-        if ((P.flags & GRASPH_DEAL_WITH_NO_PARTNERS) && rho < 0.001f) rho = P.rho0 * 0.01f;
-        density[i] = rho;
+    sweep_range<NI, GRAV, NBR, POT, SOFT>(posm, j0, j1, xi, yi, zi, a, s0, H2, cand, R);
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        if (!act[q]) continue;
+        const int i = ii[q];
+        if (GRAV || (finish && acc)) {
+            float ax = GRAV ? a.sx[q].value() : 0.0f, ay = GRAV ? a.sy[q].value() : 0.0f;
+            float az = GRAV ? a.sz[q].value() : 0.0f;
+            if (finish) {
+                const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                environment_acceleration(P, xi[q], yi[q], zi[q], v.x, v.y, v.z, ax, ay, az);
+            }
+            acc[i] = make_float4(ax, ay, az, 0.0f);
+        }
+        if (finish) {
+            if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (maxvsig) maxvsig[i] = 0.0f;
+        }
+        if (NBR) nbr_cnt[s0 + 64 * q] = a.cnt[q];
+        if (POT) pot[i] = -a.sphi[q].value();
     }
-    if (BRANCH && want_nbr) nbr_cnt[t] = a.cnt;
-    if (POT) pot[i] = -a.sphi.value();
 }
 
-template <bool GRAV, bool BRANCH, bool POT, bool SOFT>
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+__global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
+                                                       const int n, float4* __restrict__ acc, int* __restrict__ nbr,
+                                                       int* __restrict__ nbr_cnt, float* __restrict__ pot,
+                                                       const int kmax, const int finish,
+                                                       const float4* __restrict__ vel, float4* __restrict__ xvel,
+                                                       float* __restrict__ maxvsig, const DevParams P)
+{
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int s0 = wave * 64 * NI + lane;
+    const Rare R = make_rare(nbr, (size_t)n, kmax, P.H2);
+    const int j1 = (GRAV || NBR || POT) ? n : 0;  // nothing pairwise to do: skip the j loop
+    sweep_slots<NI, GRAV, NBR, POT, SOFT>(posm, perm, s0, n, 0, j1, acc, nbr_cnt, pot, finish, vel, xvel, maxvsig, P,
+                                          P.H2, P.cand, R);
+}
+
+// Persistent sweep over a batch of independent instances.  Item w =
+// (inst, slot0, ni) covers slots [slot0, slot0 + 64 ni) of instance inst
+// with ni particles per lane (chosen per size bucket); the instance's
+// particles are [off, off + n) and its slots are the same range of perm.
+// Every wavefront leaves the loop once the counter passes n_items.
+template <bool GRAV, bool NBR, bool SOFT>
+__global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __restrict__ posm,
+                                                             const int* __restrict__ perm,
+                                                             const int4* __restrict__ items, const int n_items,
+                                                             const BatchInst* __restrict__ insts,
+                                                             int* __restrict__ queue, float4* __restrict__ acc,
+                                                             int* __restrict__ nbr, int* __restrict__ nbr_cnt,
+                                                             const int ntotal, const int kmax, const DevParams P)
+{
+    const int lane = threadIdx.x & 63;
+    const Rare R0 = make_rare(nbr, (size_t)ntotal, kmax, 0.0f);
+    for (;;) {
+        int w = 0;
+        if (lane == 0) w = atomicAdd(queue, 1);
+        w = __builtin_amdgcn_readfirstlane(w);
+        if (w >= n_items) break;
+        const int4 it = items[w];
+        const BatchInst I = insts[it.x];
+        Rare R = R0;
+        R.h2 = vgpr_f32(I.H2);
+        if (it.z == 2)
+            sweep_slots<2, GRAV, NBR, false, SOFT>(posm, perm, it.y + lane, I.off + I.n, I.off, I.off + I.n, acc,
+                                                   nbr_cnt, nullptr, 0, nullptr, nullptr, nullptr, P, I.H2, I.cand, R);
+        else
+            sweep_slots<1, GRAV, NBR, false, SOFT>(posm, perm, it.y + lane, I.off + I.n, I.off, I.off + I.n, acc,
+                                                   nbr_cnt, nullptr, 0, nullptr, nullptr, nullptr, P, I.H2, I.cand, R);
+    }
+}
+
+// Zero the work counters of the batched sweep (a kernel, so that a captured
+// graph holds no memset node).
+__global__ void reset_queue_kernel(int* __restrict__ queue, const int n)
+{
+    const int i = threadIdx.x;
+    if (i < n) queue[i] = 0;
+}
+
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
 hipError_t launch_t(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 {
-    const int blocks = (L.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((sweep_kernel<GRAV, BRANCH, POT, SOFT>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.perm, L.n,
-                       L.acc, L.density, L.nbr, L.nbr_cnt, L.pot, L.kmax, L.dens ? 1 : 0, L.want_nbr ? 1 : 0,
-                       L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+    const int waves = (L.n + 64 * NI - 1) / (64 * NI);
+    const int blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    hipLaunchKernelGGL((sweep_kernel<NI, GRAV, NBR, POT, SOFT>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.perm,
+                       L.n, L.acc, L.nbr, L.nbr_cnt, L.pot, L.kmax, L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+    return hipGetLastError();
+}
+
+template <int NI>
+hipError_t launch_ni(const SweepLaunch& L, const DevParams& P, hipStream_t s)
+{
+    const bool soft = P.H2 > 0.0f;
+    const int key = (L.grav ? 8 : 0) | (L.want_nbr ? 4 : 0) | (L.want_pot ? 2 : 0) | (soft ? 1 : 0);
+    switch (key) {
+        case 0: return launch_t<NI, false, false, false, false>(L, P, s);
+        case 1: return launch_t<NI, false, false, false, true>(L, P, s);
+        case 2: return launch_t<NI, false, false, true, false>(L, P, s);
+        case 3: return launch_t<NI, false, false, true, true>(L, P, s);
+        case 4: return launch_t<NI, false, true, false, false>(L, P, s);
+        case 5: return launch_t<NI, false, true, false, true>(L, P, s);
+        case 8: return launch_t<NI, true, false, false, false>(L, P, s);
+        case 9: return launch_t<NI, true, false, false, true>(L, P, s);
+        case 12: return launch_t<NI, true, true, false, false>(L, P, s);
+        case 13: return launch_t<NI, true, true, false, true>(L, P, s);
+        default: return hipErrorInvalidValue;  // the potential sweep runs alone
+    }
+}
+
+template <bool GRAV, bool NBR, bool SOFT>
+hipError_t launch_b(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch_sweep_kernel<GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0, s,
+                       L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal, L.kmax,
+                       P);
     return hipGetLastError();
 }
 
@@ -177,23 +352,27 @@ hipError_t launch_t(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
-    const bool branch = L.dens || L.want_nbr || L.want_pot;
-    const bool soft = P.H2 > 0.0f;
-    const int key = (L.grav ? 8 : 0) | (branch ? 4 : 0) | (L.want_pot ? 2 : 0) | (soft ? 1 : 0);
+    return L.ni == 2 ? launch_ni<2>(L, P, s) : launch_ni<1>(L, P, s);
+}
+
+hipError_t launch_reset_queue(int* queue, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, queue, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sweep(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n_items <= 0) return hipSuccess;
+    if (L.waves_per_block < 1 || L.waves_per_block > kBlock / 64) return hipErrorInvalidValue;
+    const int key = (L.grav ? 4 : 0) | (L.want_nbr ? 2 : 0) | (L.soft ? 1 : 0);
     switch (key) {
-        case 0: return launch_t<false, false, false, false>(L, P, s);
-        case 1: return launch_t<false, false, false, true>(L, P, s);
-        case 4: return launch_t<false, true, false, false>(L, P, s);
-        case 5: return launch_t<false, true, false, true>(L, P, s);
-        case 6: return launch_t<false, true, true, false>(L, P, s);
-        case 7: return launch_t<false, true, true, true>(L, P, s);
-        case 8: return launch_t<true, false, false, false>(L, P, s);
-        case 9: return launch_t<true, false, false, true>(L, P, s);
-        case 12: return launch_t<true, true, false, false>(L, P, s);
-        case 13: return launch_t<true, true, false, true>(L, P, s);
-        case 14: return launch_t<true, true, true, false>(L, P, s);
-        case 15: return launch_t<true, true, true, true>(L, P, s);
-        default: return hipErrorInvalidValue;  // POT always takes the branch
+        case 1: return launch_b<false, false, true>(L, P, s);
+        case 3: return launch_b<false, true, true>(L, P, s);
+        case 4: return launch_b<true, false, false>(L, P, s);
+        case 5: return launch_b<true, false, true>(L, P, s);
+        case 7: return launch_b<true, true, true>(L, P, s);
+        default: return hipErrorInvalidValue;
     }
 }
 

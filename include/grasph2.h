@@ -167,6 +167,46 @@ int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p,
 /* getCurrentTimestep()                         src/integration.h:246-255 */
 double grasph_get_current_timestep(const grasph_ctx* ctx, const grasph_params* prm);
 
+/* ---- batched ensembles (no counterpart in the reference) ---------------
+ * Many independent simulations advanced together.  Instance k owns the
+ * particles [offsets[k], offsets[k+1]) of one grasph_particles whose n is
+ * offsets[n_instances]; each array is the concatenation of the instances'
+ * own single-instance layouts (a planar array of P planes holds instance k's
+ * planes at P * offsets[k], plane stride offsets[k+1] - offsets[k]).  So an
+ * instance's sub-view is a pointer offset and every single-instance entry
+ * point applies to it unchanged.  params[k] may differ per instance except
+ * for the flags.  offsets and params are host arrays. */
+typedef struct grasph_batch {
+    int32_t n_instances;
+    const int64_t* offsets;      /* [n_instances + 1], offsets[0] == 0, non-decreasing */
+    const grasph_params* params; /* [n_instances] */
+} grasph_batch;
+
+/* computeDerivatives for every instance: one persistent all-pairs launch
+ * over a queue of (instance, tile) items, longest instances first, then one
+ * launch per neighbour pass over all instances. */
+int grasph_batch_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b,
+                                     void* stream);
+/* integrate(pb, notFirstStep) for every instance, one launch. */
+int grasph_batch_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b, int not_first_step,
+                           void* stream);
+/* n_steps of (computeDerivatives, integrate(true)) for every instance; each
+ * step is one replay of a captured HIP graph. */
+int grasph_batch_step(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b, int n_steps, void* stream);
+/* The same steps as the plain per-instance launch loop (single-instance
+ * entry points on each instance's sub-view), for comparison. */
+int grasph_batch_step_loop(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b, int n_steps,
+                           void* stream);
+/* Pick, by timing the batched sweep on this batch, the particles per lane of
+ * each size bucket (N < 2048, < 4096, < 8192, >= 8192) and the wavefronts
+ * per block and blocks per CU of the persistent launch.  With cache_path,
+ * reads a previous result for this device and bucket set from that file or
+ * appends the new one.  Synchronises `stream`. */
+int grasph_autotune(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b, const char* cache_path,
+                    void* stream);
+/* The tuning in use: ni[4] per bucket, waves per block, blocks per CU. */
+int grasph_get_batch_tuning(const grasph_ctx* ctx, int32_t ni[4], int32_t* waves_per_block, int32_t* blocks_per_cu);
+
 /* ---- diagnostics (no counterpart in the reference) --------------------- */
 
 #define GRASPH_INV_EKIN 0   /* 1/2 sum m |v + a*half_kick_dt|^2             */
@@ -191,9 +231,9 @@ int grasph_reduce_invariants(grasph_ctx* ctx, const grasph_particles* p,
 
 /* Per-kernel timing recorded while option "profile" is 1: accumulated
  * milliseconds and launch counts per kernel class, in the order
- * sweep, strain (pass B), accel (pass C), integrate, reorder, other.
+ * sweep, density, strain, accel, integrate, reorder, other.
  * Synchronises the recorded events.  Resets the counters when reset != 0. */
-#define GRASPH_NKERNEL_CLASSES 6
+#define GRASPH_NKERNEL_CLASSES 7
 int grasph_get_kernel_times(grasph_ctx* ctx, double ms[GRASPH_NKERNEL_CLASSES],
                             int64_t launches[GRASPH_NKERNEL_CLASSES], int reset);
 
