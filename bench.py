@@ -52,6 +52,9 @@ def parse():
                     help="instances of the batched-ensemble measurement (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--loop-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run device-to-device copies of MIB MiB (the reference kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     return ap.parse_args()
 
 
@@ -111,6 +114,16 @@ def main():
 
     import grasph2_amd as g
     from grasph2_amd import ics
+
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        src, dst = g.hip.DeviceArray(nbytes), g.hip.DeviceArray(nbytes)
+        src.zero()
+        for _ in range(3):
+            dst.copy_from(src)
+        g.hip.synchronize()
+        src.free()
+        dst.free()
 
     kw = {"n": args.n} if args.n else {}
     h, prm = ics.workload(args.workload, **kw)

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <string>
 
@@ -92,21 +93,42 @@ int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_ev
 
 const int* order_of(Ctx& c) { return c.reorder_interval > 0 ? c.perm : nullptr; }
 
-int ensure_nbr(Ctx& c, int64_t n)
+// lists for `slices` slices: slices * kmax * n entries and slices * n counts
+int ensure_nbr(Ctx& c, int64_t n, int slices)
 {
     const int k = c.max_neighbours;
-    if (c.nbr && c.nbr_cap_n >= n && c.nbr_cap_k >= k) return 0;
-    if (c.nbr) hipFree(c.nbr);
-    if (c.nbr_cnt) hipFree(c.nbr_cnt);
+    const int64_t need_n = n * slices;
+    if (c.nbr && c.nbr_cap_n >= need_n && c.nbr_cap_k >= k) return 0;
+    if (c.nbr) (void)hipFree(c.nbr);
+    if (c.nbr_cnt) (void)hipFree(c.nbr_cnt);
     c.nbr = nullptr; c.nbr_cnt = nullptr; c.nbr_cap_n = 0; c.nbr_cap_k = 0;
-    if (hipMalloc((void**)&c.nbr, (size_t)n * (size_t)k * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&c.nbr_cnt, (size_t)n * sizeof(int)) != hipSuccess) {
+    if (hipMalloc((void**)&c.nbr, (size_t)need_n * (size_t)k * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&c.nbr_cnt, (size_t)need_n * sizeof(int)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(c, GRASPH_E_NOMEM, "device allocation of the neighbour list failed");
     }
-    c.nbr_cap_n = n;
+    c.nbr_cap_n = need_n;
     c.nbr_cap_k = k;
     return 0;
+}
+
+// j-split of the single-instance sweep: enough slices for ~4 wavefronts per
+// SIMD, each slice at least 1024 j; slice length a multiple of 8.
+void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len)
+{
+    int S = c.sweep_slices;
+    if (S <= 0) {
+        const int64_t waves = (n + 64 * ni - 1) / (64 * ni);
+        const int64_t target = 4 * 4 * (int64_t)c.cus;
+        S = (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + waves - 1) / waves));
+        S = (int)std::max<int64_t>(1, std::min<int64_t>(S, n / 1024));
+    }
+    int64_t len = (n + S - 1) / S;
+    len = (len + 7) / 8 * 8;
+    if (len < 8) len = 8;
+    *slice_len = (int)len;
+    *slices = (int)((n + len - 1) / len);
+    if (*slices < 1) *slices = 1;
 }
 
 SweepLaunch sweep_base(Ctx& c, const grasph_particles* p)
@@ -232,10 +254,18 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     if ((r = check_derivative_arrays(c, p, f, which))) return r;
 
     const int64_t n = p->n;
-    if ((r = ensure_order(c, p, s, true))) return r;
+    // the order only matters for the neighbour branch: gravity alone runs in index order
+    if (need_list && (r = ensure_order(c, p, s, true))) return r;
+    const bool sweep_grav = accel_pass && grav;
+    int slices = 1, slice_len = (int)n;
+    if (sweep_grav || need_list) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
     if (need_list) {
-        if ((r = ensure_nbr(c, n))) return r;
+        if ((r = ensure_nbr(c, n, slices))) return r;
         c.last_list_n = n;
+        c.last_list_slices = slices;
+    }
+    if (slices > 1 && sweep_grav) {
+        if ((r = grow(c, c.part, c.part_cap, 2 * n * slices, "sweep partial sums"))) return r;
     }
     if ((accel_pass && sph) || strain_pass) {
         if ((r = grow(c, c.jrec, c.jrec_cap, 2 * n, "pass record"))) return r;
@@ -244,13 +274,19 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
 
     // one all-pairs sweep
     SweepLaunch L = sweep_base(c, p);
-    L.grav = accel_pass && grav;
+    if (!need_list) L.perm = nullptr;
+    L.grav = sweep_grav;
     L.want_nbr = need_list;
     L.want_pot = false;
     L.finish = accel_pass && !sph;
+    L.nslices = slices;
+    L.slice_len = slice_len;
+    L.part = c.part;
     if ((r = run_sweep(c, L, P, s))) return r;
 
     NbrLaunch N = nbr_base(c, p);
+    N.nslices = slices;
+    N.slice_len = slice_len;
     if (dens_pass) {
         ProfScope ps(&c, s, KC_DENSITY);
         hipError_t e = launch_density(N, p->density, P, s);
@@ -354,6 +390,10 @@ int grasph_create(int device, grasph_ctx** out)
         return GRASPH_E_NOMEM;
     }
     ctx->c.device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->c.cus = cus;
+    (void)hipGetLastError();
     ctx->c.err = "no error";
     *out = ctx;
     return 0;
@@ -367,7 +407,8 @@ int grasph_destroy(grasph_ctx* ctx)
     hipDeviceSynchronize();
     grasph::destroy_batch_state(c.batch);
     c.batch = nullptr;
-    void* bufs[] = {c.perm, c.keys, c.keys_alt, c.vals, c.sort_tmp, c.bbox, c.nbr, c.nbr_cnt, c.jrec, c.pot, c.red};
+    void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
+                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {
@@ -395,6 +436,16 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sweep_i_per_lane") {
         if (value != 1 && value != 2) return grasph::fail(c, GRASPH_E_INVALID, "sweep_i_per_lane must be 1 or 2");
         c.sweep_ni = (int)value;
+    } else if (k == "sweep_slices") {
+        if (value < 0 || value > 64) return grasph::fail(c, GRASPH_E_INVALID, "sweep_slices must be in [0, 64]");
+        c.sweep_slices = (int)value;
+    } else if (k == "batch_waves_per_block") {
+        if (value != 1 && value != 2 && value != 4)
+            return grasph::fail(c, GRASPH_E_INVALID, "batch_waves_per_block must be 1, 2 or 4");
+        c.batch_wpb = (int)value;
+    } else if (k == "batch_blocks_per_cu") {
+        if (value < 1 || value > 64) return grasph::fail(c, GRASPH_E_INVALID, "batch_blocks_per_cu must be in [1, 64]");
+        c.batch_bpc = (int)value;
     } else if (k == "profile") {
         c.profile = value != 0;
     } else {
@@ -541,17 +592,25 @@ int grasph_get_neighbour_stats(grasph_ctx* ctx, int64_t* pairs, int32_t* max_cou
     if (max_count) *max_count = 0;
     if (overflowed) *overflowed = 0;
     if (!c.nbr_cnt || c.last_list_n <= 0) return 0;
-    std::vector<int> cnt((size_t)c.last_list_n);
+    const size_t n = (size_t)c.last_list_n, S = (size_t)c.last_list_slices;
+    std::vector<int> cnt(n * S);
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess)
         e = hipMemcpy(cnt.data(), c.nbr_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return grasph::hip_fail(c, e, "neighbour counts copy");
     int64_t tot = 0, over = 0;
     int32_t mx = 0;
-    for (int v : cnt) {
-        tot += v;
-        if (v > mx) mx = v;
-        if (v > c.max_neighbours) ++over;
+    for (size_t t = 0; t < n; ++t) {
+        int sum = 0;
+        bool o = false;
+        for (size_t s = 0; s < S; ++s) {
+            const int v = cnt[s * n + t];
+            sum += v;
+            o = o || v > c.max_neighbours;
+        }
+        tot += sum;
+        if (sum > mx) mx = sum;
+        if (o) ++over;
     }
     if (pairs) *pairs = tot;
     if (max_count) *max_count = mx;

@@ -30,6 +30,8 @@ struct Ctx {
     int max_neighbours = 160;
     int reorder_interval = 1;
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
+    int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
+    int cus = 256;         // compute units of the device
     bool profile = false;
     // batched sweep tuning: i per lane per size bucket, persistent launch shape
     int batch_ni[4] = {1, 1, 1, 1};
@@ -55,6 +57,9 @@ struct Ctx {
     int64_t nbr_cap_n = 0;
     int nbr_cap_k = 0;
     int64_t last_list_n = 0;  // particles of the last neighbour list built
+    int last_list_slices = 1;
+    float4* part = nullptr;   // partial sums of a j-split sweep
+    int64_t part_cap = 0;
 
     // pass-C per-particle record: sigma/rho^2 (6), balsara, density
     float4* jrec = nullptr;
@@ -106,6 +111,9 @@ struct SweepLaunch {
     float4* xvel;
     float* maxvsig;
     int ni;             // i particles per lane (1 or 2)
+    int nslices;        // j-split: slices of slice_len j (lists and counts per slice)
+    int slice_len;
+    float4* part;       // [nslices][n][2] partial sums when nslices > 1
 };
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s);
 
@@ -215,6 +223,8 @@ struct NbrLaunch {
     float* maxvsig;
     bool acc_has_gravity;
     BlockTables T;  // batch tables (T.nblocks == 0 for one instance)
+    int nslices;    // lists per slice of the j-split sweep (0 or 1: one list)
+    int slice_len;
 };
 hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P, hipStream_t s);
 hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s);

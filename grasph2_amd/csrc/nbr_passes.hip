@@ -54,25 +54,41 @@ __device__ inline float dsdt_entry(const Tens& ed, const Tens& rd, const Tens& S
     return o;
 }
 
-// Walk particle i's neighbours: the list, or on overflow the instance scan.
+// Lists of a j-split sweep: one list per slice of slice_len j, walked in
+// slice order (so j stays increasing).
+struct Lists {
+    const int* nbr;
+    const int* cnt;
+    int nslices;
+    int slice_len;
+};
+
+// Walk particle i's neighbours: the lists, or for a slice whose list
+// overflowed, a scan of that slice's j with the exact test.
 template <typename F>
-__device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ posm, const int* __restrict__ nbr,
-                                                   const int cnt, const size_t nstride, const int t, const int kmax,
+__device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ posm, const Lists& Ls,
+                                                   const size_t nstride, const int t, const int kmax,
                                                    const float4 pi, const SlotMap& m, const float H2, F&& body)
 {
-    if (cnt <= kmax) {
-        for (int k = 0; k < cnt; ++k) body(nbr[(size_t)k * nstride + t]);
-    } else {
-        for (int j = m.off; j < m.off + m.n; ++j) {
-            const float4 pj = posm[j];
-            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
+    for (int s = 0; s < Ls.nslices; ++s) {
+        const int* list = Ls.nbr + (size_t)s * kmax * nstride;
+        const int cnt = Ls.cnt[(size_t)s * nstride + t];
+        if (cnt <= kmax) {
+            for (int k = 0; k < cnt; ++k) body(list[(size_t)k * nstride + t]);
+        } else {
+            const int a = m.off + s * Ls.slice_len;
+            const int b = min(m.off + m.n, a + Ls.slice_len);
+            for (int j = a; j < b; ++j) {
+                const float4 pj = posm[j];
+                if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
+            }
         }
     }
 }
 
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
-                                                         const int* __restrict__ nbr, const int* __restrict__ nbr_cnt,
+                                                         const Lists Ls,
                                                          const size_t nstride, const int kmax,
                                                          float* __restrict__ density, const BlockTables T,
                                                          const DevParams P0)
@@ -84,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     const int i = perm ? perm[t] : t;
     const float4 pi = posm[i];
     CSum rho{0.0f, 0.0f};
-    for_each_neighbour(posm, nbr, nbr_cnt[t], nstride, t, kmax, pi, m, P.H2, [&](int j) {
+    for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = posm[j];
         const float r = sqrtf(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
         rho.add(pj.w * w_spline(r, P.H, P.Wpref));
@@ -97,8 +113,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
 template <bool BATCH, bool SOLIDS, bool BALS, bool INTD>
 __global__ __launch_bounds__(kBlock) void strain_kernel(
     const float4* __restrict__ posm, const float4* __restrict__ vel, const float* __restrict__ density,
-    const float* __restrict__ dstress, const int* __restrict__ perm, const int* __restrict__ nbr,
-    const int* __restrict__ nbr_cnt, const size_t nstride, const int kmax, float* __restrict__ balsara,
+    const float* __restrict__ dstress, const int* __restrict__ perm, const Lists Ls, const size_t nstride, const int kmax, float* __restrict__ balsara,
     float* __restrict__ density_dt, float* __restrict__ dstress_dt, float4* __restrict__ jrec, const BlockTables T,
     const DevParams P0)
 {
@@ -115,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 
-    for_each_neighbour(posm, nbr, nbr_cnt[t], nstride, t, kmax, pi, m, P.H2, [&](int j) {
+    for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = posm[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
@@ -225,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(const float* __restrict__ 
 template <bool BATCH, bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
 __global__ __launch_bounds__(kBlock) void accel_kernel(
     const float4* __restrict__ posm, const float4* __restrict__ vel, const float4* __restrict__ jrec,
-    const int* __restrict__ perm, const int* __restrict__ nbr, const int* __restrict__ nbr_cnt, const size_t nstride,
+    const int* __restrict__ perm, const Lists Ls, const size_t nstride,
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
     const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
@@ -250,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    for_each_neighbour(posm, nbr, nbr_cnt[t], nstride, t, kmax, pi, m, P.H2, [&](int j) {
+    for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = posm[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
@@ -332,6 +347,11 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const float4* __restrict
     if (maxvsig) maxvsig[i] = 0.0f;
 }
 
+inline Lists lists_of(const NbrLaunch& L)
+{
+    return Lists{L.nbr, L.nbr_cnt, L.nslices > 1 ? L.nslices : 1, L.nslices > 1 ? L.slice_len : (1 << 30)};
+}
+
 inline dim3 grid_of(const NbrLaunch& L)
 {
     return dim3(L.T.nblocks > 0 ? L.T.nblocks : (L.n + kBlock - 1) / kBlock);
@@ -344,7 +364,7 @@ hipError_t strain_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
                in = P.flags & GRASPH_INTEGRATE_DENSITY;
 #define GRASPH_STRAIN(A, B, C)                                                                                     \
     hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.density,   \
-                       L.dstress, L.perm, L.nbr, L.nbr_cnt, (size_t)L.n, L.kmax, L.balsara, L.density_dt,           \
+                       L.dstress, L.perm, lists_of(L), (size_t)L.n, L.kmax, L.balsara, L.density_dt,           \
                        L.dstress_dt, L.jrec, L.T, P)
     const int key = (so ? 4 : 0) | (ba ? 2 : 0) | (in ? 1 : 0);
     switch (key) {
@@ -369,7 +389,7 @@ hipError_t accel_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
                ar = P.flags & GRASPH_ARTIFICIAL_STRESS;
 #define GRASPH_ACCEL(A, B, C, D, E)                                                                                  \
     hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.jrec,  \
-                       L.perm, L.nbr, L.nbr_cnt, (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig,                      \
+                       L.perm, lists_of(L), (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig,                      \
                        L.acc_has_gravity ? 1 : 0, L.T, P)
     // SOLIDS + ARTIFICIAL_STRESS is rejected before launch (the reference does not compile it).
     const int key = (so ? 16 : 0) | (ba ? 8 : 0) | (av ? 4 : 0) | (xs ? 2 : 0) | (ar ? 1 : 0);
@@ -410,10 +430,10 @@ hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, L.nbr, L.nbr_cnt,
+        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, lists_of(L),
                            (size_t)L.n, L.kmax, density, L.T, P);
     else
-        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, L.nbr, L.nbr_cnt,
+        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, lists_of(L),
                            (size_t)L.n, L.kmax, density, L.T, P);
     return hipGetLastError();
 }

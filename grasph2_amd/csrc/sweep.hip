@@ -195,7 +195,9 @@ __device__ __forceinline__ void sweep_range(const float4* __restrict__ posm, con
 }
 
 // Load the lane's particles (slots s0 + 64 q below s_end), run the sweep
-// over [j0, j1) and store the results.
+// over [j0, j1) and store the results -- or, with part set (one slice of a
+// j-split sweep), the compensated partial sums (sum, compensation) of the
+// slice, combined later in slice order.
 template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
 __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, const int* __restrict__ perm,
                                             const int s0, const int s_end, const int j0, const int j1,
@@ -203,7 +205,7 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, con
                                             float* __restrict__ pot, const int finish,
                                             const float4* __restrict__ vel, float4* __restrict__ xvel,
                                             float* __restrict__ maxvsig, const DevParams& P, const float H2,
-                                            const float cand, const Rare& R)
+                                            const float cand, const Rare& R, float4* __restrict__ part = nullptr)
 {
     float xi[NI], yi[NI], zi[NI];
     int ii[NI];
@@ -231,6 +233,14 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, con
     for (int q = 0; q < NI; ++q) {
         if (!act[q]) continue;
         const int i = ii[q];
+        if (part) {
+            if (GRAV) {
+                part[2 * (size_t)i] = make_float4(a.sx[q].s, a.sy[q].s, a.sz[q].s, 0.0f);
+                part[2 * (size_t)i + 1] = make_float4(a.sx[q].c, a.sy[q].c, a.sz[q].c, 0.0f);
+            }
+            if (NBR) nbr_cnt[s0 + 64 * q] = a.cnt[q];
+            continue;
+        }
         if (GRAV || (finish && acc)) {
             float ax = GRAV ? a.sx[q].value() : 0.0f, ay = GRAV ? a.sy[q].value() : 0.0f;
             float az = GRAV ? a.sz[q].value() : 0.0f;
@@ -255,15 +265,48 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict_
                                                        int* __restrict__ nbr_cnt, float* __restrict__ pot,
                                                        const int kmax, const int finish,
                                                        const float4* __restrict__ vel, float4* __restrict__ xvel,
-                                                       float* __restrict__ maxvsig, const DevParams P)
+                                                       float* __restrict__ maxvsig, const DevParams P,
+                                                       const int slice_len, float4* __restrict__ part)
 {
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int s0 = wave * 64 * NI + lane;
-    const Rare R = make_rare(nbr, (size_t)n, kmax, P.H2);
-    const int j1 = (GRAV || NBR || POT) ? n : 0;  // nothing pairwise to do: skip the j loop
-    sweep_slots<NI, GRAV, NBR, POT, SOFT>(posm, perm, s0, n, 0, j1, acc, nbr_cnt, pot, finish, vel, xvel, maxvsig, P,
-                                          P.H2, P.cand, R);
+    // slice blockIdx.y of the j range; its list and counts have their own planes
+    const int sl = blockIdx.y;
+    const int j0 = sl * slice_len;
+    int j1 = min(n, j0 + slice_len);
+    if (!(GRAV || NBR || POT)) j1 = j0;  // nothing pairwise to do: skip the j loop
+    const Rare R = make_rare(nbr + (size_t)sl * kmax * n, (size_t)n, kmax, P.H2);
+    sweep_slots<NI, GRAV, NBR, POT, SOFT>(posm, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot, finish, vel,
+                                          xvel, maxvsig, P, P.H2, P.cand, R,
+                                          part ? part + 2 * (size_t)sl * n : nullptr);
+}
+
+// Sum the slices' partial gravity in slice order (compensated) and finish
+// the acceleration as a one-slice sweep would.
+__global__ __launch_bounds__(kBlock) void combine_kernel(const float4* __restrict__ part, const int nslices,
+                                                         const int n, const float4* __restrict__ posm,
+                                                         float4* __restrict__ acc, const int finish,
+                                                         const float4* __restrict__ vel, float4* __restrict__ xvel,
+                                                         float* __restrict__ maxvsig, const DevParams P)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    CSum x{0.0f, 0.0f}, y{0.0f, 0.0f}, z{0.0f, 0.0f};
+    for (int s = 0; s < nslices; ++s) {
+        const float4 a = part[2 * ((size_t)s * n + i)], c = part[2 * ((size_t)s * n + i) + 1];
+        x.add(a.x); y.add(a.y); z.add(a.z);
+        x.c += c.x; y.c += c.y; z.c += c.z;
+    }
+    float ax = x.value(), ay = y.value(), az = z.value();
+    if (finish) {
+        const float4 p = posm[i];
+        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        environment_acceleration(P, p.x, p.y, p.z, v.x, v.y, v.z, ax, ay, az);
+        if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (maxvsig) maxvsig[i] = 0.0f;
+    }
+    acc[i] = make_float4(ax, ay, az, 0.0f);
 }
 
 // Persistent sweep over a batch of independent instances.  Item w =
@@ -313,8 +356,14 @@ hipError_t launch_t(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 {
     const int waves = (L.n + 64 * NI - 1) / (64 * NI);
     const int blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
-    hipLaunchKernelGGL((sweep_kernel<NI, GRAV, NBR, POT, SOFT>), dim3(blocks), dim3(kBlock), 0, s, L.posm, L.perm,
-                       L.n, L.acc, L.nbr, L.nbr_cnt, L.pot, L.kmax, L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+    const int ns = L.nslices > 1 ? L.nslices : 1;
+    hipLaunchKernelGGL((sweep_kernel<NI, GRAV, NBR, POT, SOFT>), dim3(blocks, ns), dim3(kBlock), 0, s, L.posm, L.perm,
+                       L.n, L.acc, L.nbr, L.nbr_cnt, L.pot, L.kmax, L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P,
+                       ns > 1 ? L.slice_len : L.n, ns > 1 ? L.part : nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ns == 1 || !(GRAV || L.finish)) return e;
+    hipLaunchKernelGGL(combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.part, ns, L.n, L.posm,
+                       L.acc, L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
     return hipGetLastError();
 }
 

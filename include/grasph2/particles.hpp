@@ -152,23 +152,28 @@ inline grasph_params& settings()
     return p;
 }
 
-// process-wide context on device 0 (created on first use)
+// process-wide context on device 0 (created on first use, released by
+// shutdown(); call that before returning from main so device memory is
+// freed while the HIP runtime is still up)
+inline grasph_ctx*& context_slot()
+{
+    static grasph_ctx* c = nullptr;
+    return c;
+}
 inline grasph_ctx* context()
 {
-    struct Holder {
-        grasph_ctx* c = nullptr;
-        Holder()
-        {
-            if (grasph_create(0, &c) != 0)
-                throw std::runtime_error(std::string("grasph_create: ") + grasph_last_error(nullptr));
-        }
-        ~Holder()
-        {
-            if (c) grasph_destroy(c);
-        }
-    };
-    static Holder h;
-    return h.c;
+    grasph_ctx*& c = context_slot();
+    if (!c && grasph_create(0, &c) != 0) {
+        c = nullptr;
+        throw std::runtime_error(std::string("grasph_create: ") + grasph_last_error(nullptr));
+    }
+    return c;
+}
+inline void shutdown()
+{
+    grasph_ctx*& c = context_slot();
+    if (c) grasph_destroy(c);
+    c = nullptr;
 }
 
 inline void check(int r)

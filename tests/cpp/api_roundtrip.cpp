@@ -55,6 +55,8 @@ int main(int argc, char** argv)
         p.dstress = s;
         hpb.storeParticle(i, p);
     }
+    int rc = 0;
+    {
     DevT pb(n);
     pb = hpb;
     try {
@@ -66,9 +68,14 @@ int main(int argc, char** argv)
         grasph::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (const std::exception& e) {
         fprintf(stderr, "error: %s\n", e.what());
-        return 1;
+        rc = 1;
     }
-    hpb = pb;
+    if (rc == 0) hpb = pb;
+    }  // device buffer freed here
+    if (rc != 0) {
+        grasph::shutdown();
+        return rc;
+    }
     grasph_particles v = hpb.view();
     FILE* fo = fopen(argv[2], "wb");
     if (!fo) return 2;
@@ -79,5 +86,6 @@ int main(int argc, char** argv)
     // a loaded particle round-trips through the interface
     auto p = hpb.loadParticle<POS, ACC>(0);
     printf("timestep %g particle0 acc %g %g %g\n", (double)getCurrentTimestep(), p.acc.x, p.acc.y, p.acc.z);
+    grasph::shutdown();
     return 0;
 }

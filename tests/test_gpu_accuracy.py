@@ -1,0 +1,125 @@
+"""Accuracy of every workload at its real size against the FP64 oracle, and
+the energy drift of its run.
+
+acceleration: the HIP path's acc after one computeDerivatives against the
+FP64 oracle evaluated on a sample of particles through the whole chain
+(density on the sample's neighbours' neighbours, strain / Balsara on its
+neighbours, acceleration on the sample), errors as in tests/_util.py, bound
+1e-5 (north_star).  The FP32 oracle on the same sample is reported beside it:
+that is the reference's own FP32 arithmetic.
+energy: FP64 invariants (kinetic with the half-kick velocities synchronised,
+softened gravitational potential, compressional + elastic) over the
+workload's step count with integrate(pb, true) as the reference's main loop;
+the drift is reported and bounded loosely (viscosity and plasticity
+dissipate, so SPH runs do not conserve it).  Results go to stdout and, when
+the folder exists, gpurun_out/accuracy.json.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+import grasph2_amd as g
+from grasph2_amd import ics
+from oracle import pyoracle as O
+from tests._util import scalar_errors, vec_errors
+
+pytestmark = pytest.mark.gpu
+
+RESULTS = {}
+
+
+def sampled_oracle(prm, h, sample, prec):
+    """Oracle acceleration (and density) of `sample`, computed through the
+    whole derivative chain on the particles it depends on."""
+    f = prm.flags
+    if not (f & g.SPH):
+        return O.calc_acceleration(prm, h, prec, sample=sample)
+    pos = np.asarray(h["posm"][:, :3], dtype=np.float64)
+    tree = cKDTree(pos)
+    r = float(prm.H) * 1.0001
+
+    def ring(idx):
+        nb = tree.query_ball_point(pos[idx], r)
+        return np.unique(np.concatenate([np.asarray(idx)] + [np.asarray(x, dtype=np.int64) for x in nb]))
+
+    u1 = ring(sample)
+    u2 = ring(u1)
+    cur = dict(h)
+    if not (f & g.INTEGRATE_DENSITY):
+        cur["density"] = O.calc_density(prm, cur, prec, sample=u2)["density"]
+    if (f & g.SOLIDS) or (f & g.INTEGRATE_DENSITY) or (f & g.BALSARA_SWITCH):
+        o = O.calc_balsara(prm, cur, prec, sample=u1)
+        cur["balsara"] = o["balsara"] if "balsara" in o else cur.get("balsara")
+    out = O.calc_acceleration(prm, cur, prec, sample=sample)
+    out["density"] = cur["density"]
+    return out
+
+
+def record(name, **kv):
+    RESULTS.setdefault(name, {}).update(kv)
+    print("[accuracy]", name, json.dumps(kv), flush=True)
+    if os.path.isdir("gpurun_out"):
+        with open("gpurun_out/accuracy.json", "w") as fh:
+            json.dump(RESULTS, fh, indent=1)
+
+
+def compact_sample(pos, count, seed):
+    """`count` particles nearest to a random particle: a compact cluster keeps
+    the oracle's dependency rings small at large N."""
+    rng = np.random.default_rng(seed)
+    c = pos[rng.integers(len(pos))]
+    d = np.linalg.norm(pos - c, axis=1)
+    return np.sort(np.argsort(d, kind="stable")[:count])
+
+
+WORKLOADS = [
+    # name, builder, sample size, compact, steps for the drift
+    ("plummer", lambda: ics.plummer(16384), 2048, False, 100),
+    ("liquid", lambda: ics.liquid_blob(65536), 1024, False, 200),
+    ("collision", lambda: ics.collision(131072), 1024, False, 100),
+    ("large", lambda: ics.large_sphere(1048576), 4096, True, 20),
+]
+
+
+@pytest.mark.parametrize("name,build,ns,compact,steps", WORKLOADS, ids=[w[0] for w in WORKLOADS])
+def test_workload_accuracy(gpu_ctx, name, build, ns, compact, steps):
+    h, prm = build()
+    n = h["posm"].shape[0]
+    O.set_threads(min(16, os.cpu_count() or 1))
+    d = g.DeviceParticles.from_host(h)
+    gpu_ctx.compute_derivatives(d, prm)
+    g.hip.synchronize()
+    got = d.download()
+    pos = h["posm"][:, :3].astype(np.float64)
+    sample = compact_sample(pos, ns, 1) if compact else np.sort(
+        np.random.default_rng(1).choice(n, size=ns, replace=False))
+    ref64 = sampled_oracle(prm, h, sample, "f64")
+    ref32 = sampled_oracle(prm, h, sample, "f32")
+    mx, nr = vec_errors(got["acc"][sample], ref64["acc"][sample])
+    mx32, nr32 = vec_errors(ref32["acc"][sample], ref64["acc"][sample])
+    res = {"n": n, "sample": int(ns), "acc_max_rel": mx, "acc_norm_rel": nr, "fp32_oracle_acc_max_rel": mx32,
+           "fp32_oracle_acc_norm_rel": nr32}
+    if prm.flags & g.SPH:
+        dmx, dnr = scalar_errors(got["density"][sample], ref64["density"][sample])
+        res.update(density_max_rel=dmx, density_norm_rel=dnr)
+    # energy over the run (reference main loop: integrate(pb, true) every step)
+    dt = prm.fixed_timestep
+    e = []
+    for k in range(steps + 1):
+        if k > 0:
+            gpu_ctx.integrate(d, prm, True)
+            gpu_ctx.compute_derivatives(d, prm)
+        if k == 0 or k == steps or k % max(1, steps // 4) == 0:
+            inv = gpu_ctx.invariants(d, prm, half_kick_dt=0.5 * dt, with_potential=True)
+            e.append(inv[g.INV_EKIN] + inv[g.INV_EPOT] + inv[g.INV_EINT])
+    d.free()
+    drift = abs(e[-1] - e[0]) / abs(e[0])
+    res.update(steps=steps, energy_start=e[0], energy_end=e[-1], energy_drift_rel=drift)
+    record(name, **res)
+    assert nr <= 1e-5, res
+    assert mx <= 1e-4, res
+    if name == "plummer":
+        assert drift < 1e-3, res
