@@ -13,7 +13,7 @@ from . import hip
 from ._capi import (  # noqa: F401
     ARTIFICIAL_STRESS, ARTIFICIAL_VISCOSITY, BALSARA_SWITCH, CLOHESSY_WILTSHIRE, CONSTANT_ACCELERATION,
     DEAL_WITH_NO_PARTNERS, INTEGRATE_DENSITY, PLASTICITY_MC, PLASTICITY_MIESE, SELF_GRAVITY, SIMPLE_BOX_BOUNDARY,
-    SOLIDS, SPH, XSPH, E_HIP, E_INVALID, E_NOMEM, E_UNSUPPORTED, NINV, INV_EKIN, INV_EPOT, INV_EINT, INV_PX, INV_PY,
+    SOLIDS, SPH, XSPH, VARIABLE_TIMESTEP, ACCELERATION_CRITERION, VELOCITY_CRITERION, E_HIP, E_INVALID, E_NOMEM, E_UNSUPPORTED, NINV, INV_EKIN, INV_EPOT, INV_EINT, INV_PX, INV_PY,
     INV_PZ, INV_LX, INV_LY, INV_LZ, INV_MASS, KERNEL_CLASSES, NKERNEL_CLASSES, Batch, GrasphError, Params, Particles,
     default_params, lib,
 )
@@ -165,6 +165,7 @@ class Context:
                                                       1 if not_first_step else 0, self._s(stream)))
 
     def current_timestep(self, prm):
+        """getCurrentTimestep(): fixed_timestep, or the context's current variable dt."""
         return self._lib.grasph_get_current_timestep(self.handle, ctypes.byref(prm))
 
     def invariants(self, parts, prm, half_kick_dt=0.0, with_potential=True, stream=None):
@@ -224,6 +225,11 @@ class Context:
         v = parts.view() if isinstance(parts, DeviceParticles) else parts
         return self._check(self._lib.grasph_autotune(self.handle, ctypes.byref(v), ctypes.byref(b),
                                                      cache_path.encode() if cache_path else None, self._s(stream)))
+
+    def batch_current_timesteps(self, b):
+        out = (ctypes.c_double * b.n_instances)()
+        self._check(self._lib.grasph_batch_get_current_timesteps(self.handle, ctypes.byref(b), out))
+        return np.array(out[:], dtype=np.float64)
 
     def batch_tuning(self):
         ni = (ctypes.c_int32 * 4)()

@@ -25,6 +25,9 @@ ARTIFICIAL_VISCOSITY = 1 << 10
 BALSARA_SWITCH = 1 << 11
 ARTIFICIAL_STRESS = 1 << 12
 XSPH = 1 << 13
+VARIABLE_TIMESTEP = 1 << 14
+ACCELERATION_CRITERION = 1 << 15
+VELOCITY_CRITERION = 1 << 16
 
 E_INVALID, E_HIP, E_UNSUPPORTED, E_NOMEM = -1, -2, -3, -4
 
@@ -58,6 +61,11 @@ class Params(ctypes.Structure):
         ("box_max", ctypes.c_float * 3),
         ("box_reflectiveness", ctypes.c_float),
         ("fixed_timestep", ctypes.c_float),
+        ("initial_timestep", ctypes.c_float),
+        ("min_timestep", ctypes.c_float),
+        ("max_timestep", ctypes.c_float),
+        ("accel_accuracy", ctypes.c_float),
+        ("velocity_accuracy", ctypes.c_float),
     ]
 
     def copy(self):
@@ -127,6 +135,7 @@ _SIGNATURES = {
     "grasph_autotune": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_char_p, ctypes.c_void_p]),
     "grasph_get_batch_tuning": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32),
                                                _P(ctypes.c_int32)]),
+    "grasph_batch_get_current_timesteps": (ctypes.c_int, [ctypes.c_void_p, _P(Batch), _P(ctypes.c_double)]),
 }
 
 _LIB = None

@@ -44,6 +44,7 @@ struct BatchState {
     int4* d_blk = nullptr;
     int n_blk = 0;
     int* d_queue = nullptr;
+    float* d_ts = nullptr;  // variable step: {current dt, next dt} per instance, zero when the tables are built
 
     // Morton order
     float* d_bbox = nullptr;
@@ -98,11 +99,11 @@ void drop_graph(BatchState& b)
 void free_tables(BatchState& b)
 {
     void* ps[] = {b.d_insts, b.d_prm, b.d_inst_of, b.d_items, b.d_blk, b.d_queue, b.d_bbox, b.d_keys, b.d_keys_alt,
-                  b.d_vals, b.d_perm, b.d_sort_tmp};
+                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts};
     for (void* p : ps) free_ptr(p);
     b.d_insts = nullptr; b.d_prm = nullptr; b.d_inst_of = nullptr; b.d_items = nullptr; b.d_blk = nullptr;
     b.d_queue = nullptr; b.d_bbox = nullptr; b.d_keys = nullptr; b.d_keys_alt = nullptr; b.d_vals = nullptr;
-    b.d_perm = nullptr; b.d_sort_tmp = nullptr;
+    b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr;
     b.sort_bytes = 0;
     b.order_valid = false;
     b.n_items = 0;
@@ -202,6 +203,8 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
             return r;
         std::vector<int> zero(64, 0);
         if ((r = upload(c, b.d_queue, zero, "batch work counter"))) return r;
+        std::vector<float> ts0(2 * (size_t)n_inst, 0.0f);
+        if ((r = upload(c, b.d_ts, ts0, "batch time-step state"))) return r;
         b.key_bits = 30;
         while ((1LL << (b.key_bits - 30)) < n_inst) ++b.key_bits;
         const size_t nt = (size_t)std::max<int64_t>(p->n, 1);
@@ -412,8 +415,11 @@ int batch_integrate_launch(Ctx& c, BatchState& b, const grasph_particles* p, int
     L.n = (int)b.ntotal;
     L.not_first_step = nfs;
     L.T = tables(b);
+    L.maxvsig = p->maxvsig;
+    L.n_instances = b.n_inst;
+    L.ts = (b.flags & GRASPH_VARIABLE_TIMESTEP) ? b.d_ts : nullptr;
     ProfScope ps(&c, s, KC_INTEGRATE);
-    hipError_t e = launch_integrate(L, make_dev_params(&b.params[0]), s);
+    hipError_t e = launch_integrate_step(L, make_dev_params(&b.params[0]), c.timestep_reset, s);
     if (e != hipSuccess) return hip_fail(c, e, "batched leapfrog launch");
     return 0;
 }
@@ -443,7 +449,7 @@ std::string graph_key(const Ctx& c, const BatchState& b, const grasph_particles*
     k << p->posm << ' ' << p->pos64 << ' ' << p->vel << ' ' << p->acc << ' ' << p->xvel << ' ' << p->density << ' '
       << p->density_dt << ' ' << p->balsara << ' ' << p->maxvsig << ' ' << p->dstress << ' ' << p->dstress_dt << ' '
       << (c.reorder_interval > 0) << ' ' << c.batch_wpb << ' ' << c.batch_bpc << ' ' << (const void*)b.d_nbr << ' '
-      << (const void*)b.d_jrec << ' ' << (const void*)b.d_items;
+      << (const void*)b.d_jrec << ' ' << (const void*)b.d_items << ' ' << c.timestep_reset;
     return k.str();
 }
 
@@ -614,6 +620,8 @@ int grasph_batch_step_loop(grasph_ctx* ctx, const grasph_particles* p, const gra
             s.perm_n = 0;
         }
         s.sweep_ni = c.sweep_ni;
+        s.sweep_slices = c.sweep_slices;
+        s.timestep_reset = c.timestep_reset;
         s.profile = false;
     }
     for (int step = 0; step < n_steps; ++step)
@@ -708,6 +716,29 @@ int grasph_get_batch_tuning(const grasph_ctx* ctx, int32_t ni[4], int32_t* waves
         for (int q = 0; q < 4; ++q) ni[q] = c.batch_ni[q];
     if (waves_per_block) *waves_per_block = c.batch_wpb;
     if (blocks_per_cu) *blocks_per_cu = c.batch_bpc;
+    return 0;
+}
+
+int grasph_batch_get_current_timesteps(grasph_ctx* ctx, const grasph_batch* b, double* dt)
+{
+    if (!ctx) return GRASPH_E_INVALID;
+    Ctx& c = ctx->c;
+    if (!b || !b->offsets || !b->params || !dt || b->n_instances < 1)
+        return grasph::fail(c, GRASPH_E_INVALID, "batch, its arrays or dt is NULL");
+    const int n = b->n_instances;
+    if (!(b->params[0].flags & GRASPH_VARIABLE_TIMESTEP)) {
+        for (int k = 0; k < n; ++k) dt[k] = (double)b->params[k].fixed_timestep;
+        return 0;
+    }
+    const grasph::BatchState* B = c.batch;
+    const bool same = B && B->d_ts && B->n_inst == n && std::equal(B->offsets.begin(), B->offsets.end(), b->offsets) &&
+                      std::memcmp(B->params.data(), b->params, n * sizeof(grasph_params)) == 0;
+    if (!same) return grasph::fail(c, GRASPH_E_INVALID, "this batch has not been integrated on this context");
+    std::vector<float> ts(2 * (size_t)n);
+    if (hipSetDevice(c.device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(ts.data(), B->d_ts, ts.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return grasph::hip_fail(c, hipGetLastError(), "reading the batch time steps");
+    for (int k = 0; k < n; ++k) dt[k] = (double)ts[2 * (size_t)k];
     return 0;
 }
 

@@ -228,6 +228,7 @@ int check_integrate_arrays(Ctx& c, const grasph_particles* p, uint32_t f)
         if (f & GRASPH_PLASTICITY_MC) { GRASPH_NEED(p->density, "density"); }
     }
     if ((f & GRASPH_SPH) && (f & GRASPH_XSPH)) GRASPH_NEED(p->xvel, "xvel");
+    if ((f & GRASPH_VARIABLE_TIMESTEP) && (f & GRASPH_VELOCITY_CRITERION)) GRASPH_NEED(p->maxvsig, "maxvsig");
     return 0;
 }
 
@@ -329,10 +330,10 @@ void grasph_params_default(grasph_params* p)
 {
     if (!p) return;
     std::memset(p, 0, sizeof *p);
-    // settings/settings.h with FIXED_TIMESTEP_LEAPFROG in place of the
-    // variable-step integrator (:49-62)
+    // settings/settings.h, including its variable-step integrator (:52-62)
     p->flags = GRASPH_SELF_GRAVITY | GRASPH_SPH | GRASPH_INTEGRATE_DENSITY | GRASPH_SOLIDS | GRASPH_PLASTICITY_MC |
-               GRASPH_ARTIFICIAL_VISCOSITY | GRASPH_BALSARA_SWITCH;
+               GRASPH_ARTIFICIAL_VISCOSITY | GRASPH_BALSARA_SWITCH | GRASPH_VARIABLE_TIMESTEP |
+               GRASPH_ACCELERATION_CRITERION | GRASPH_VELOCITY_CRITERION;
     p->dimension = 3;
     const double count = 16384.0, radius = 0.1, tmass = 1.0;
     const double pradius = radius * std::pow(count, -1.0 / 3.0);  // :97-98
@@ -358,7 +359,12 @@ void grasph_params_default(grasph_params* p)
         p->box_max[d] = 1.2f;
     }
     p->box_reflectiveness = 0.5f;
-    p->fixed_timestep = 0.002f;
+    p->fixed_timestep = 0.002f;     // :50
+    p->initial_timestep = 0.002f;   // :56
+    p->min_timestep = 0.00002f;     // :57
+    p->max_timestep = 0.02f;        // :58
+    p->accel_accuracy = 0.01f;      // :60
+    p->velocity_accuracy = 0.09f;   // :62
 }
 
 int grasph_create(int device, grasph_ctx** out)
@@ -390,6 +396,16 @@ int grasph_create(int device, grasph_ctx** out)
         return GRASPH_E_NOMEM;
     }
     ctx->c.device = device;
+    // the variable-step leapfrog's {current dt, next dt}: zero, as the
+    // reference's __device__ currentTS / nextTS start
+    if (hipMalloc((void**)&ctx->c.ts, 2 * sizeof(float)) != hipSuccess ||
+        hipMemset(ctx->c.ts, 0, 2 * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (ctx->c.ts) (void)hipFree(ctx->c.ts);
+        delete ctx;
+        grasph::g_create_err = "device allocation of the time-step state failed";
+        return GRASPH_E_NOMEM;
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->c.cus = cus;
@@ -408,7 +424,7 @@ int grasph_destroy(grasph_ctx* ctx)
     grasph::destroy_batch_state(c.batch);
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
-                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part};
+                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part, c.ts};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {
@@ -436,6 +452,9 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sweep_i_per_lane") {
         if (value != 1 && value != 2) return grasph::fail(c, GRASPH_E_INVALID, "sweep_i_per_lane must be 1 or 2");
         c.sweep_ni = (int)value;
+    } else if (k == "timestep_reset") {
+        if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "timestep_reset must be 0 or 1");
+        c.timestep_reset = value == 1;
     } else if (k == "sweep_slices") {
         if (value < 0 || value > 64) return grasph::fail(c, GRASPH_E_INVALID, "sweep_slices must be in [0, 64]");
         c.sweep_slices = (int)value;
@@ -494,17 +513,29 @@ int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_pa
     L.dstress_dt = p->dstress_dt;
     L.n = (int)p->n;
     L.not_first_step = not_first_step;
+    L.maxvsig = p->maxvsig;
+    L.n_instances = 1;
+    L.ts = (prm->flags & GRASPH_VARIABLE_TIMESTEP) ? c.ts : nullptr;
     hipStream_t s = (hipStream_t)stream;
     grasph::ProfScope ps(&c, s, grasph::KC_INTEGRATE);
-    hipError_t e = grasph::launch_integrate(L, grasph::make_dev_params(prm), s);
+    hipError_t e = grasph::launch_integrate_step(L, grasph::make_dev_params(prm), c.timestep_reset, s);
     if (e != hipSuccess) return grasph::hip_fail(c, e, "leapfrog launch");
     return 0;
 }
 
 double grasph_get_current_timestep(const grasph_ctx* ctx, const grasph_params* prm)
 {
-    (void)ctx;
-    return prm ? (double)prm->fixed_timestep : 0.0;
+    if (!prm) return 0.0;
+    if (!(prm->flags & GRASPH_VARIABLE_TIMESTEP)) return (double)prm->fixed_timestep;  // integration.h:248-249
+    if (!ctx || !ctx->c.ts) return 0.0;
+    // integration.h:250-253: the device's current dt, after the queued work
+    float dt = 0.0f;
+    if (hipSetDevice(ctx->c.device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(&dt, ctx->c.ts, sizeof dt, hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1.0;
+    }
+    return (double)dt;
 }
 
 int grasph_reduce_invariants(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm,

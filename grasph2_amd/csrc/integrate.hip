@@ -1,5 +1,5 @@
-// integrate.hip -- fixed-step leapfrog kick-drift with plasticity, and the
-// FP64 global invariants (gfx950).
+// integrate.hip -- leapfrog kick-drift with plasticity (fixed and variable
+// step), and the FP64 global invariants (gfx950).
 //
 // leapfrog: fixedTsLeapfrog::do_for_each (src/integration.h:82-125) with
 // doPlasticity (:24-31) and handleBoundaryConditions (:33-73); one particle
@@ -8,12 +8,26 @@
 // rewritten from it; without them it drifts the FP32 positions as the
 // reference does.  One kernel body serves one instance or a batch (block
 // descriptors and per-instance parameters from tables).
+//
+// Variable step (src/integration.h:127-217): the same kernel, with dt read
+// on the device from the instance's {current dt, next dt} pair and a half
+// kick, does variableTsLeapfrog_useNewTimestep; vts_kick_kernel does
+// variableTsLeapfrog_getNewTimestep (half kick with the current dt, then the
+// minimum of the dt criteria into next dt with one atomic minimum per block
+// -- non-negative floats order like their bit patterns).  No host round trip.
 #include "internal.hpp"
 
 namespace grasph {
 namespace {
 
 constexpr int kBlock = 256;
+
+// dt of the second variable-step kernel, integration.h:170
+__device__ __forceinline__ float vts_dt(const float* t, const DevParams& P)
+{
+    const float next = t[1];
+    return (next < P.ts_min) ? P.ts_min : next;
+}
 
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ posm, double* __restrict__ pos64,
@@ -24,19 +38,25 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
                                                           float* __restrict__ dstress,
                                                           const float* __restrict__ dstress_dt, const int n,
                                                           const int not_first_step, const BlockTables T,
-                                                          const DevParams P0)
+                                                          const DevParams P0, float* __restrict__ ts)
 {
     const SlotMap m = slot_map<BATCH>(T, n);
     const int i = m.slot0 + threadIdx.x;  // the integrator walks particles in index order
     if (i >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
-    const float dt = P.dt;
+    float dt = P.dt;
+    float kick_factor = (not_first_step) ? 1.0f : 0.5f;  // integration.h:95
+    if (ts) {
+        float* t = ts + 2 * m.inst;
+        dt = vts_dt(t, P);
+        kick_factor = 0.5f;  // integration.h:182
+        if (i == m.off) t[0] = dt;  // currentTS = dt (:171-172); every block only reads t[1]
+    }
     const bool sph = P.flags & GRASPH_SPH;
 
     const float4 a = acc[i];
     float4 v = vel[i];
-    // integration.h:95
-    const float kick = dt * ((not_first_step) ? 1.0f : 0.5f);
+    const float kick = dt * kick_factor;
     v.x = v.x + a.x * kick;
     v.y = v.y + a.y * kick;
     v.z = v.z + a.z * kick;
@@ -104,6 +124,60 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
     }
     posm[i] = p;
     vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// variableTsLeapfrog_getNewTimestep (src/integration.h:130-165).  The
+// criteria are rounded as written (no contraction), so dt matches an FP32
+// evaluation of the same formulas bit for bit.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void vts_kick_kernel(float4* __restrict__ vel, const float4* __restrict__ acc,
+                                                          const float* __restrict__ maxvsig, const int n,
+                                                          const BlockTables T, const DevParams P0,
+                                                          float* __restrict__ ts)
+{
+    __shared__ float red[kBlock / 64];
+    const SlotMap m = slot_map<BATCH>(T, n);
+    const int i = m.slot0 + threadIdx.x;
+    const DevParams P = BATCH ? T.prm[m.inst] : P0;
+    float* t = ts + 2 * m.inst;
+    float mindt = P.ts_max;  // :134
+    if (i < m.slot_end) {
+        const float dt = t[0];
+        const float4 a = acc[i];
+        float4 v = vel[i];
+        const float h = dt * 0.5f;  // :144
+        v.x = v.x + a.x * h;
+        v.y = v.y + a.y * h;
+        v.z = v.z + a.z * h;
+        vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+        if (P.flags & GRASPH_ACCELERATION_CRITERION) {  // :149-151
+            const float a2 = __fadd_rn(__fadd_rn(__fmul_rn(a.x, a.x), __fmul_rn(a.y, a.y)), __fmul_rn(a.z, a.z));
+            const float la = __fsqrt_rn(a2);
+            mindt = fminf(mindt, __fsqrt_rn(__fdiv_rn(__fmul_rn(P.acc_acc, P.H), la)));
+        }
+        if (P.flags & GRASPH_VELOCITY_CRITERION)  // :152-154
+            mindt = fminf(mindt, __fdiv_rn(__fmul_rn(P.vel_acc, P.H), maxvsig[i]));
+    }
+    // block minimum, then one atomic minimum on the bit pattern (:158-168)
+    for (int off = 32; off >= 1; off >>= 1) mindt = fminf(mindt, __shfl_xor(mindt, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mindt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = red[0];
+        for (int w = 1; w < kBlock / 64; ++w) r = fminf(r, red[w]);
+        atomicMin((int*)&t[1], __float_as_int(r));
+    }
+}
+
+// next dt := initial_timestep (integration.h:233-234) or max_timestep
+// (the "timestep_reset" option), one thread per instance
+__global__ void vts_set_next_kernel(float* __restrict__ ts, const int ninst, const DevParams* __restrict__ prm,
+                                    const DevParams P0, const int which)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    const DevParams& P = prm ? prm[k] : P0;
+    ts[2 * k + 1] = which == 0 ? P.ts_init : P.ts_max;
 }
 
 // internal energy per unit mass of the EOS, u(rho) = int_{rho0}^{rho} p / rho'^2 drho'
@@ -201,12 +275,49 @@ hipError_t launch_integrate(const IntegLaunch& L, const DevParams& P, hipStream_
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
         hipLaunchKernelGGL(leapfrog_kernel<true>, dim3(L.T.nblocks), dim3(kBlock), 0, s, L.posm, L.pos64, L.vel, L.acc,
-                           L.xvel, L.density, L.density_dt, L.dstress, L.dstress_dt, L.n, L.not_first_step, L.T, P);
+                           L.xvel, L.density, L.density_dt, L.dstress, L.dstress_dt, L.n, L.not_first_step, L.T, P,
+                           L.ts);
     else
         hipLaunchKernelGGL(leapfrog_kernel<false>, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.posm,
                            L.pos64, L.vel, L.acc, L.xvel, L.density, L.density_dt, L.dstress, L.dstress_dt, L.n,
-                           L.not_first_step, L.T, P);
+                           L.not_first_step, L.T, P, L.ts);
     return hipGetLastError();
+}
+
+hipError_t launch_vts_kick(const IntegLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    if (!L.ts) return hipErrorInvalidValue;
+    if (L.T.nblocks > 0)
+        hipLaunchKernelGGL(vts_kick_kernel<true>, dim3(L.T.nblocks), dim3(kBlock), 0, s, L.vel, L.acc, L.maxvsig,
+                           L.n, L.T, P, L.ts);
+    else
+        hipLaunchKernelGGL(vts_kick_kernel<false>, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.vel,
+                           L.acc, L.maxvsig, L.n, L.T, P, L.ts);
+    return hipGetLastError();
+}
+
+hipError_t launch_vts_set_next(const IntegLaunch& L, const DevParams& P, int which, hipStream_t s)
+{
+    if (!L.ts || L.n_instances <= 0) return hipErrorInvalidValue;
+    const DevParams* prm = L.T.nblocks > 0 ? L.T.prm : nullptr;
+    hipLaunchKernelGGL(vts_set_next_kernel, dim3((L.n_instances + 255) / 256), dim3(256), 0, s, L.ts, L.n_instances,
+                       prm, P, which);
+    return hipGetLastError();
+}
+
+// integrate(pb, notFirstStep), src/integration.h:219-244
+hipError_t launch_integrate_step(const IntegLaunch& L, const DevParams& P, bool timestep_reset, hipStream_t s)
+{
+    if (!L.ts) return launch_integrate(L, P, s);
+    hipError_t e;
+    if (L.not_first_step) {
+        if (timestep_reset && (e = launch_vts_set_next(L, P, 1, s)) != hipSuccess) return e;
+        if ((e = launch_vts_kick(L, P, s)) != hipSuccess) return e;
+    } else if ((e = launch_vts_set_next(L, P, 0, s)) != hipSuccess) {
+        return e;
+    }
+    return launch_integrate(L, P, s);
 }
 
 int inv_blocks(int n)

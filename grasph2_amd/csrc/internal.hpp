@@ -37,6 +37,10 @@ struct Ctx {
     int batch_ni[4] = {1, 1, 1, 1};
     int batch_wpb = 4;
     int batch_bpc = 8;
+    bool timestep_reset = false;  // variable step: restart the dt minimum at max_timestep each step
+
+    // variable-step leapfrog state {current dt, next dt}, zero at creation
+    float* ts = nullptr;
 
     // Morton order of the i side (slot t -> particle perm[t])
     int* perm = nullptr;
@@ -244,8 +248,21 @@ struct IntegLaunch {
     int n;
     int not_first_step;
     BlockTables T;  // batch tables (T.nblocks == 0 for one instance)
+    // variable-step leapfrog: per instance {current dt, next dt}; null for the fixed step
+    float* ts;
+    const float* maxvsig;  // read by the velocity criterion
+    int n_instances;
 };
+// Fixed step: fixedTsLeapfrog.  With L.ts: the second kernel of the
+// variable step (kick + drift with dt = max(next dt, min_timestep)).
 hipError_t launch_integrate(const IntegLaunch& L, const DevParams& P, hipStream_t s);
+// First kernel of the variable step: half kick with the current dt and the
+// dt criteria folded into the next dt (atomic minimum per block).
+hipError_t launch_vts_kick(const IntegLaunch& L, const DevParams& P, hipStream_t s);
+// next dt of every instance := initial_timestep (which 0) or max_timestep (which 1)
+hipError_t launch_vts_set_next(const IntegLaunch& L, const DevParams& P, int which, hipStream_t s);
+// Both stages of one integrate(pb, notFirstStep) call; timestep_reset as the option.
+hipError_t launch_integrate_step(const IntegLaunch& L, const DevParams& P, bool timestep_reset, hipStream_t s);
 
 struct InvLaunch {
     const float4* posm;
@@ -318,6 +335,13 @@ inline int check_params(Ctx& c, const grasph_params* prm)
                     "src/computeDerivatives.h:267)");
     if ((prm->flags & GRASPH_PLASTICITY_MIESE) && (prm->flags & GRASPH_PLASTICITY_MC))
         return fail(c, GRASPH_E_INVALID, "choose up to one plasticity model (settings.h:203-205)");
+    if (prm->flags & GRASPH_VARIABLE_TIMESTEP) {
+        const float v[5] = {prm->initial_timestep, prm->min_timestep, prm->max_timestep, prm->accel_accuracy,
+                            prm->velocity_accuracy};
+        for (float x : v)
+            if (!(x >= 0.0f) || !(x < 3.0e38f))
+                return fail(c, GRASPH_E_INVALID, "variable time-step settings must be finite and >= 0");
+    }
     return 0;
 }
 

@@ -23,6 +23,8 @@ struct DevParams {
     float cax, cay, caz;
     float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, refl;
     float cand;  // d2 = r2 + H2 threshold that admits every pair with r2 <= H2
+    // variable-step leapfrog, settings/settings.h:56-62
+    float ts_init, ts_min, ts_max, acc_acc, vel_acc;
 };
 
 // src/sph/kernel.h:41-60
@@ -157,6 +159,8 @@ __host__ inline DevParams make_dev_params(const grasph_params* p)
     d.bminx = p->box_min[0]; d.bminy = p->box_min[1]; d.bminz = p->box_min[2];
     d.bmaxx = p->box_max[0]; d.bmaxy = p->box_max[1]; d.bmaxz = p->box_max[2];
     d.refl = p->box_reflectiveness;
+    d.ts_init = p->initial_timestep; d.ts_min = p->min_timestep; d.ts_max = p->max_timestep;
+    d.acc_acc = p->accel_accuracy; d.vel_acc = p->velocity_accuracy;
     // d2 = fma-chain(r2) + H2 carries at most a few ulp of relative error, so
     // pairs with r2 <= H2 all satisfy d2 <= 2 H2 (1 + 2^-18).
     d.cand = 2.0f * d.H2 * (1.0f + 1.0f / 262144.0f);

@@ -87,7 +87,10 @@ def _fill(h, pos, mass, vel):
 
 def lattice_sphere(n, radius=1.0, jitter=0.0, rng=None):
     """n points of a cubic lattice filling a sphere: the n lattice sites
-    closest to the centre, spacing chosen so the sphere holds n sites."""
+    closest to the centre, spacing chosen so the sphere holds n sites
+    (no points and spacing None for n = 0)."""
+    if n <= 0:
+        return np.zeros((0, 3)), None
     d = (4.0 / 3.0 * math.pi * radius ** 3 / n) ** (1.0 / 3.0)
     m = int(math.ceil(radius / d)) + 2
     ax = (np.arange(-m, m + 1) * d)
@@ -161,8 +164,9 @@ SOLID_FIELDS = ("posm", "pos64", "vel", "acc", "density", "balsara", "maxvsig", 
 def _collision_arrays(n_total, rng, speed, angle_deg, jitter=0.1):
     na = n_total // 2
     nb = n_total - na
-    pa, d = lattice_sphere(na, 1.0, jitter=jitter, rng=rng)
-    pb, _ = lattice_sphere(nb, 1.0, jitter=jitter, rng=rng)
+    pa, da = lattice_sphere(na, 1.0, jitter=jitter, rng=rng)
+    pb, db = lattice_sphere(nb, 1.0, jitter=jitter, rng=rng)
+    d = da if da is not None else db
     H = 2.5 * d
     sep = 2.0 + H
     pa = pa + np.array([-sep / 2, 0.0, 0.0])

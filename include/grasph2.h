@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GRASPH_VERSION 10000 /* major*10000 + minor*100 + patch */
+#define GRASPH_VERSION 10100 /* major*10000 + minor*100 + patch */
 
 /* ---- status codes ------------------------------------------------------ */
 #define GRASPH_OK 0
@@ -53,7 +53,10 @@ extern "C" {
 #define GRASPH_BALSARA_SWITCH        (1u << 11) /* BALSARA_SWITCH            settings.h:159 */
 #define GRASPH_ARTIFICIAL_STRESS     (1u << 12) /* ARTIFICIAL_STRESS         settings.h:163 */
 #define GRASPH_XSPH                  (1u << 13) /* XSPH                      settings.h:170 */
-#define GRASPH_ALL_FLAGS             ((1u << 14) - 1u)
+#define GRASPH_VARIABLE_TIMESTEP     (1u << 14) /* VARIABLE_TIMESTEP_LEAPFROG settings.h:55 (else FIXED_...) */
+#define GRASPH_ACCELERATION_CRITERION (1u << 15) /* ACCELERATION_CRITERION   settings.h:59  */
+#define GRASPH_VELOCITY_CRITERION    (1u << 16) /* VELOCITY_CRITERION        settings.h:61  */
+#define GRASPH_ALL_FLAGS             ((1u << 17) - 1u)
 
 /*
  * Runtime replacement of the compile-time constants of settings/settings.h.
@@ -83,12 +86,17 @@ typedef struct grasph_params {
     float box_max[3];        /*                                 settings.h:148 */
     float box_reflectiveness;/*                                 settings.h:149 */
     float fixed_timestep;    /* dt of the fixed-step leapfrog   settings.h:50  */
+    float initial_timestep;  /* variable step: first dt         settings.h:56  */
+    float min_timestep;      /*   smallest dt                   settings.h:57  */
+    float max_timestep;      /*   largest dt                    settings.h:58  */
+    float accel_accuracy;    /*   dt <= sqrt(accel_accuracy H / |a|)   :60   */
+    float velocity_accuracy; /*   dt <= velocity_accuracy H / maxvsig  :62   */
 } grasph_params;
 
 /* Fill *p with the defaults of settings/settings.h: self-gravity, SPH,
- * integrated density, solids, Mohr-Coulomb plasticity, artificial viscosity
- * and the Balsara switch, with the fixed-step leapfrog (dt 0.002) standing in
- * for the variable-step integrator the reference selects there. */
+ * integrated density, solids, Mohr-Coulomb plasticity, artificial viscosity,
+ * the Balsara switch and the variable-step leapfrog with both time-step
+ * criteria. */
 void grasph_params_default(grasph_params* p);
 
 /*
@@ -135,7 +143,12 @@ int grasph_destroy(grasph_ctx* ctx);
 
 /* Tuning knobs.  Keys: "max_neighbours" (default 160), "reorder_interval"
  * (Morton re-sort every k derivative evaluations, default 1; 0 = never),
- * "profile" (1 = record HIP events around each kernel). */
+ * "profile" (1 = record HIP events around each kernel), "sweep_i_per_lane"
+ * (1 or 2), "sweep_slices" (j-split of the sweep, 0 = automatic),
+ * "batch_waves_per_block", "batch_blocks_per_cu" (batched sweep launch
+ * shape), "timestep_reset" (1 = the variable-step leapfrog starts each
+ * step's dt minimum from max_timestep, so dt can grow again; 0 = the
+ * reference's running minimum, the default). */
 int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
 
 /* ---- the hot path ------------------------------------------------------- */
@@ -159,12 +172,23 @@ int grasph_calc_balsara_density_dt_dstress_dt(grasph_ctx* ctx, const grasph_part
 int grasph_calc_acceleration(grasph_ctx* ctx, const grasph_particles* p,
                              const grasph_params* prm, void* stream);
 
-/* integrate(pb, notFirstStep) with FIXED_TIMESTEP_LEAPFROG
- * src/integration.h:219-244 -> fixedTsLeapfrog src/integration.h:82-125 */
+/* integrate(pb, notFirstStep)                 src/integration.h:219-244
+ * FIXED_TIMESTEP_LEAPFROG: fixedTsLeapfrog, src/integration.h:82-125.
+ * GRASPH_VARIABLE_TIMESTEP: the two kernels of VARIABLE_TIMESTEP_LEAPFROG,
+ * src/integration.h:127-217.  With not_first_step, a half kick with the
+ * current dt that also folds the particles' dt criteria into the next dt
+ * (a minimum that, as in the reference, is not reset between steps: dt
+ * never grows unless the option "timestep_reset" is set); without it the
+ * next dt is set to initial_timestep.  Then a half kick and the drift with
+ * dt = max(next dt, min_timestep), which becomes the current dt.  The two
+ * values live in the context (zero at creation, like the reference's
+ * __device__ currentTS / nextTS), so the calls stay asynchronous. */
 int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p,
                      const grasph_params* prm, int not_first_step, void* stream);
 
-/* getCurrentTimestep()                         src/integration.h:246-255 */
+/* getCurrentTimestep()                         src/integration.h:246-255
+ * fixed_timestep, or with GRASPH_VARIABLE_TIMESTEP the context's current dt
+ * (waits for the device, as the reference's cudaMemcpyFromSymbol does). */
 double grasph_get_current_timestep(const grasph_ctx* ctx, const grasph_params* prm);
 
 /* ---- batched ensembles (no counterpart in the reference) ---------------
@@ -187,9 +211,14 @@ typedef struct grasph_batch {
  * launch per neighbour pass over all instances. */
 int grasph_batch_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b,
                                      void* stream);
-/* integrate(pb, notFirstStep) for every instance, one launch. */
+/* integrate(pb, notFirstStep) for every instance, one launch (two with the
+ * variable step, whose per-instance dt state lives with the batch's tables:
+ * zero when a batch layout or its params are first seen). */
 int grasph_batch_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b, int not_first_step,
                            void* stream);
+/* getCurrentTimestep() of every instance of the batch last prepared with
+ * exactly these offsets and params: dt[k] (waits for the device). */
+int grasph_batch_get_current_timesteps(grasph_ctx* ctx, const grasph_batch* b, double* dt);
 /* n_steps of (computeDerivatives, integrate(true)) for every instance; each
  * step is one replay of a captured HIP graph. */
 int grasph_batch_step(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b, int n_steps, void* stream);

@@ -98,6 +98,7 @@ template <typename T> struct Consts {
     T mateps, matexp, normalsep, xsph, cw_n, dt;
     V3<T> const_acc, bmin, bmax;
     T refl;
+    T ts_init, ts_min, ts_max, acc_acc, vel_acc;  // settings.h:56-62
     int dim;
     uint32_t f;
     bool has(uint32_t b) const { return (f & b) != 0; }
@@ -135,6 +136,8 @@ template <typename T> Consts<T> make_consts(const grasph_params* p)
     c.bmin = {T(p->box_min[0]), T(p->box_min[1]), T(p->box_min[2])};
     c.bmax = {T(p->box_max[0]), T(p->box_max[1]), T(p->box_max[2])};
     c.refl = T(p->box_reflectiveness);
+    c.ts_init = T(p->initial_timestep); c.ts_min = T(p->min_timestep); c.ts_max = T(p->max_timestep);
+    c.acc_acc = T(p->accel_accuracy); c.vel_acc = T(p->velocity_accuracy);
     return c;
 }
 
@@ -447,17 +450,20 @@ template <typename T> void calc_acceleration(const Consts<T>& C, const Parts<T>&
     }
 }
 
-/* ---- integration.h:24-125 ----------------------------------------------- */
-template <typename T> void fixed_ts_leapfrog(const Consts<T>& C, const Parts<T>& P, int not_first_step)
+/* ---- integration.h:24-125 (and the kick-drift of :168-217) --------------- */
+/* kick v += a (dt kick_factor), drift, density and stress update, plasticity,
+ * box boundary: fixedTsLeapfrog::do_for_each (:89-125) with kick_factor 1
+ * (not first step) or 1/2, variableTsLeapfrog_useNewTimestep (:175-216)
+ * with kick_factor 1/2 -- the two bodies are the same statements. */
+template <typename T> void leapfrog_kick_drift(const Consts<T>& C, const Parts<T>& P, const T dt, const T kick_factor)
 {
     const bool sph = C.has(GRASPH_SPH), xsph = C.has(GRASPH_XSPH), intd = C.has(GRASPH_INTEGRATE_DENSITY),
                solids = C.has(GRASPH_SOLIDS), box = C.has(GRASPH_SIMPLE_BOX_BOUNDARY);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < P.n; ++i) {
-        const T dt = C.dt;
         V3<T> pos = P.pos(i), vel = P.v4(P.vel, i);
         const V3<T> acc = P.v4(P.acc, i);
-        vel = vel + acc * (dt * ((not_first_step) ? T(1.0) : T(0.5)));
+        vel = vel + acc * (dt * kick_factor);
         if (xsph && sph)
             pos = pos + (vel + C.xsph * P.v4(P.xvel, i)) * dt;
         else
@@ -494,6 +500,50 @@ template <typename T> void fixed_ts_leapfrog(const Consts<T>& C, const Parts<T>&
         P.posm[4 * i] = pos.x; P.posm[4 * i + 1] = pos.y; P.posm[4 * i + 2] = pos.z;
         P.s4(P.vel, i, vel);
     }
+}
+
+template <typename T> void fixed_ts_leapfrog(const Consts<T>& C, const Parts<T>& P, int not_first_step)
+{
+    leapfrog_kick_drift(C, P, C.dt, (not_first_step) ? T(1.0) : T(0.5));  // integration.h:95
+}
+
+/* variableTsLeapfrog_getNewTimestep (integration.h:130-165): half kick with
+ * the current dt ts[0], then ts[1] = min(ts[1], max_timestep, criteria of
+ * every particle) -- the block minima and the atomicMin of the reference
+ * give exactly this minimum.  Criteria: sqrt(accel_accuracy H / |a|) and
+ * velocity_accuracy H / maxvsig (:149-154). */
+template <typename T> void vts_get_new_timestep(const Consts<T>& C, const Parts<T>& P, T* ts)
+{
+    const T dt = ts[0];
+    const T accH = C.acc_acc * C.H, velH = C.vel_acc * C.H;
+    T mindt = C.ts_max;
+    for (int64_t i = 0; i < P.n; ++i) {
+        V3<T> vel = P.v4(P.vel, i);
+        const V3<T> acc = P.v4(P.acc, i);
+        vel = vel + acc * (dt * T(0.5));
+        P.s4(P.vel, i, vel);
+        if (C.has(GRASPH_ACCELERATION_CRITERION)) mindt = std::fmin(mindt, std::sqrt(accH / length(acc)));
+        if (C.has(GRASPH_VELOCITY_CRITERION)) mindt = std::fmin(mindt, velH / P.maxvsig[i]);
+    }
+    if (P.n > 0) ts[1] = std::fmin(ts[1], mindt);
+}
+
+/* variableTsLeapfrog_useNewTimestep (integration.h:167-217) */
+template <typename T> void vts_use_new_timestep(const Consts<T>& C, const Parts<T>& P, T* ts)
+{
+    const T dt = (ts[1] < C.ts_min) ? C.ts_min : ts[1];
+    ts[0] = dt;
+    leapfrog_kick_drift(C, P, dt, T(0.5));
+}
+
+/* integrate(pb, notFirstStep) with VARIABLE_TIMESTEP_LEAPFROG (:225-243) */
+template <typename T> void variable_ts_leapfrog(const Consts<T>& C, const Parts<T>& P, int not_first_step, T* ts)
+{
+    if (not_first_step)
+        vts_get_new_timestep(C, P, ts);
+    else
+        ts[1] = C.ts_init;
+    vts_use_new_timestep(C, P, ts);
 }
 
 template <typename T> int validate(const grasph_params* p)
@@ -666,6 +716,21 @@ int oracle_fixed_ts_leapfrog_f64(const grasph_params* prm, const oracle_particle
     int v = validate<double>(prm);
     if (v) return v;
     fixed_ts_leapfrog(make_consts<double>(prm), parts_from<double>(p), nfs);
+    return 0;
+}
+/* ts = {currentTS, nextTS} in the precision of the suffix, updated in place */
+int oracle_variable_ts_leapfrog_f32(const grasph_params* prm, const oracle_particles_t* p, int nfs, float* ts)
+{
+    int v = validate<float>(prm);
+    if (v) return v;
+    variable_ts_leapfrog(make_consts<float>(prm), parts_from<float>(p), nfs, ts);
+    return 0;
+}
+int oracle_variable_ts_leapfrog_f64(const grasph_params* prm, const oracle_particles_t* p, int nfs, double* ts)
+{
+    int v = validate<double>(prm);
+    if (v) return v;
+    variable_ts_leapfrog(make_consts<double>(prm), parts_from<double>(p), nfs, ts);
     return 0;
 }
 

@@ -45,6 +45,9 @@ def lib():
             fn = getattr(L, "oracle_fixed_ts_leapfrog_" + prec)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_void_p, P, ctypes.c_int]
+            fn = getattr(L, "oracle_variable_ts_leapfrog_" + prec)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_void_p, P, ctypes.c_int, ctypes.c_void_p]
             fn = getattr(L, "oracle_potential_" + prec)
             fn.restype = None
             fn.argtypes = [ctypes.c_void_p, P, ctypes.c_void_p]
@@ -151,6 +154,32 @@ def leapfrog(prm, h, not_first_step=True, prec="f64"):
     b = _Bound(h, prec)
     _check(getattr(lib(), "oracle_fixed_ts_leapfrog_" + prec)(ctypes.byref(prm), ctypes.byref(b.view),
                                                               1 if not_first_step else 0))
+    return b.arrays
+
+
+class TimestepState:
+    """The reference's __device__ currentTS / nextTS (src/integration.h:127-128),
+    zero until the first integrate call, in the oracle's precision."""
+
+    def __init__(self, prec="f64"):
+        self.prec = prec
+        self.ts = np.zeros(2, dtype=np.float32 if prec == "f32" else np.float64)
+
+    @property
+    def current(self):
+        return float(self.ts[0])
+
+    @property
+    def next(self):
+        return float(self.ts[1])
+
+
+def variable_leapfrog(prm, h, state, not_first_step=True):
+    """integrate(pb, notFirstStep) with VARIABLE_TIMESTEP_LEAPFROG; updates
+    `state` (a TimestepState) in place and returns the new arrays."""
+    b = _Bound(h, state.prec)
+    _check(getattr(lib(), "oracle_variable_ts_leapfrog_" + state.prec)(
+        ctypes.byref(prm), ctypes.byref(b.view), 1 if not_first_step else 0, state.ts.ctypes.data))
     return b.arrays
 
 

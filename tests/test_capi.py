@@ -38,9 +38,9 @@ def test_python_binding_covers_the_header():
 
 
 def test_version_and_struct_layout():
-    assert g.lib().grasph_version() == 10000
-    # grasph_params: 2 ints + 15 floats + 3*3 floats + 2 floats
-    assert ctypes.sizeof(g.Params) == 4 * (2 + 15 + 9 + 2)
+    assert g.lib().grasph_version() == 10100
+    # grasph_params: 2 ints + 15 floats + 3*3 floats + 2 floats + 5 variable-step floats
+    assert ctypes.sizeof(g.Params) == 4 * (2 + 15 + 9 + 2 + 5)
     assert ctypes.sizeof(g.Particles) == 8 * 12
 
 
@@ -53,8 +53,12 @@ def test_defaults_follow_settings_h():
     assert p.SOUNDSPEED == pytest.approx(math.sqrt(1024 / rho0), rel=1e-6)
     assert p.BULK == 1024 and p.shear == 128 and p.tanfr == pytest.approx(1.0)
     assert p.flags == (g.SELF_GRAVITY | g.SPH | g.INTEGRATE_DENSITY | g.SOLIDS | g.PLASTICITY_MC |
-                       g.ARTIFICIAL_VISCOSITY | g.BALSARA_SWITCH)
+                       g.ARTIFICIAL_VISCOSITY | g.BALSARA_SWITCH | g.VARIABLE_TIMESTEP |
+                       g.ACCELERATION_CRITERION | g.VELOCITY_CRITERION)
     assert p.fixed_timestep == pytest.approx(0.002)
+    # settings.h:56-62
+    assert (p.initial_timestep, p.min_timestep, p.max_timestep) == pytest.approx((0.002, 0.00002, 0.02))
+    assert (p.accel_accuracy, p.velocity_accuracy) == pytest.approx((0.01, 0.09))
     assert list(p.constant_acceleration) == [0.0, -1.0, 0.0]
 
 

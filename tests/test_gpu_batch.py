@@ -56,7 +56,18 @@ def assert_same(got_list, ref_list, fields):
             assert np.array_equal(a[f], b[f]), "instance %d field %s" % (k, f)
 
 
-def test_batched_derivatives_match_single_instances(gpu_ctx):
+@pytest.fixture
+def unsplit(gpu_ctx):
+    """The single-instance sweep without its j-split (the batched sweep never
+    splits an instance's j range), so both sum in the same order; the
+    launch-loop contexts take the option from the parent context."""
+    gpu_ctx.set_option("sweep_slices", 1)
+    yield gpu_ctx
+    gpu_ctx.set_option("sweep_slices", 0)
+
+
+def test_batched_derivatives_match_single_instances(unsplit):
+    gpu_ctx = unsplit
     insts, params, host, offsets = make_batch()
     b = gpu_ctx.batch(offsets, params)
     d = g.DeviceParticles.from_host(host)
@@ -71,7 +82,8 @@ def test_batched_derivatives_match_single_instances(gpu_ctx):
     assert nr <= 1e-5 and mx <= 1e-4, (mx, nr)
 
 
-def test_graph_steps_match_loop_and_eager(gpu_ctx):
+def test_graph_steps_match_loop_and_eager(unsplit):
+    gpu_ctx = unsplit
     insts, params, host, offsets = make_batch(seed=1)
     b = gpu_ctx.batch(offsets, params)
     dg = g.DeviceParticles.from_host(host)
@@ -132,3 +144,82 @@ def test_batch_rejects_mixed_flags(gpu_ctx):
         gpu_ctx.batch_compute_derivatives(d, b)
     assert e.value.code == g.E_INVALID
     d.free()
+
+
+def test_split_single_instance_agrees_with_batch(gpu_ctx):
+    """With its j-split the single-instance sweep sums gravity in another
+    order: equal to FP32 rounding, not bit for bit."""
+    insts, params, host, offsets = make_batch(sizes=(2500, 4100, 9000), seed=3)
+    b = gpu_ctx.batch(offsets, params)
+    d = g.DeviceParticles.from_host(host)
+    gpu_ctx.batch_compute_derivatives(d, b)
+    g.hip.synchronize()
+    got = g.unpack_batch(d.download(), offsets)
+    d.free()
+    ref = single_results(gpu_ctx, insts, params)
+    for a, r in zip(got, ref):
+        mx, nr = vec_errors(a["acc"], r["acc"])
+        assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
+        assert np.array_equal(a["density"], r["density"])  # list passes: same neighbours, same order
+
+
+def _variable(params, initial=0.05):
+    out = []
+    for p in params:
+        q = p.copy()
+        q.flags |= g.VARIABLE_TIMESTEP | g.ACCELERATION_CRITERION | g.VELOCITY_CRITERION
+        q.initial_timestep, q.min_timestep, q.max_timestep = initial, 1e-4, 0.1
+        q.accel_accuracy, q.velocity_accuracy = 0.01, 0.09
+        out.append(q)
+    return out
+
+
+@pytest.mark.parametrize("reset", [0, 1])
+def test_batched_variable_step_matches_single_instances(unsplit, reset):
+    """Per-instance dt state: the batched integrator (eager and graph steps)
+    equals each instance integrated alone on its own context, dt included."""
+    gpu_ctx = unsplit
+    insts, params, host, offsets = make_batch(sizes=(300, 1000, 2500, 64, 1), seed=4)
+    params = _variable(params)
+    gpu_ctx.set_option("timestep_reset", reset)
+    try:
+        b = gpu_ctx.batch(offsets, params)
+        de = g.DeviceParticles.from_host(host)
+        dg = g.DeviceParticles.from_host(host)
+        gpu_ctx.batch_compute_derivatives(de, b)
+        gpu_ctx.batch_integrate(de, b, False)
+        for _ in range(3):
+            gpu_ctx.batch_compute_derivatives(de, b)
+            gpu_ctx.batch_integrate(de, b, True)
+        dts = gpu_ctx.batch_current_timesteps(b)
+        got = g.unpack_batch(de.download(), offsets)
+        # graph steps on a second batch object (fresh dt state): first step eager, then replays
+        b2 = gpu_ctx.batch(offsets, [p.copy() for p in params])
+        gpu_ctx.batch_compute_derivatives(dg, b2)
+        gpu_ctx.batch_integrate(dg, b2, False)
+        gpu_ctx.batch_step(dg, b2, 3)
+        g.hip.synchronize()
+        assert np.array_equal(gpu_ctx.batch_current_timesteps(b2), dts)
+        graph = g.unpack_batch(dg.download(), offsets)
+        assert_same(graph, got, FIELDS_S)
+        de.free()
+        dg.free()
+        for k, (h, prm) in enumerate(zip(insts, params)):
+            c1 = g.Context(0)
+            c1.set_option("sweep_slices", 1)
+            c1.set_option("timestep_reset", reset)
+            d1 = g.DeviceParticles.from_host(h)
+            c1.compute_derivatives(d1, prm)
+            c1.integrate(d1, prm, False)
+            for _ in range(3):
+                c1.compute_derivatives(d1, prm)
+                c1.integrate(d1, prm, True)
+            assert c1.current_timestep(prm) == dts[k], k
+            one = d1.download()
+            d1.free()
+            c1.close()
+            for f in FIELDS_S:
+                assert np.array_equal(one[f], got[k][f]), "instance %d field %s" % (k, f)
+        assert len(set(dts.tolist())) > 1  # the instances chose their own dt
+    finally:
+        gpu_ctx.set_option("timestep_reset", 0)

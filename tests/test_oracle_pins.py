@@ -378,6 +378,82 @@ def test_leapfrog_stress_density_boundary():
     assert o["vel"][1, 0] == pytest.approx(-0.5)  # reflected with reflectiveness 0.5
 
 
+# ---------------------------------------------------------------- variable step
+def _vts_params(H=0.05):
+    prm = ics.plummer_params(H=H, dt=0.002)
+    prm.flags = g.VARIABLE_TIMESTEP | g.ACCELERATION_CRITERION | g.VELOCITY_CRITERION
+    prm.initial_timestep, prm.min_timestep, prm.max_timestep = 0.002, 2e-5, 0.02
+    prm.accel_accuracy, prm.velocity_accuracy = 0.01, 0.09
+    return prm
+
+
+def _vts_state(acc, maxvsig):
+    n = len(acc)
+    h = make(np.zeros((n, 3)), np.ones(n), fields=("posm", "vel", "acc", "maxvsig"))
+    h["acc"][:, :3] = acc
+    h["maxvsig"][:] = maxvsig
+    return h
+
+
+def test_variable_step_criteria_by_hand():
+    """Parameters are FP32 (as in grasph_params), hence the 1e-6 tolerances."""
+    prm = _vts_params()
+    st = O.TimestepState("f64")
+    h = _vts_state([[3.0, 4.0, 0.0], [0.0, 0.0, 0.0]], [0.0, 2.0])
+    O.variable_leapfrog(prm, h, st, not_first_step=False)
+    assert (st.current, st.next) == pytest.approx((0.002, 0.002), rel=1e-7)  # first step: initial_timestep
+    # |a| = 5: sqrt(0.01 * 0.05 / 5) = 0.01; 0.09 * 0.05 / 2 = 0.00225; running min stays 0.002
+    O.variable_leapfrog(prm, h, st)
+    assert st.current == pytest.approx(0.002, rel=1e-7)
+    h = _vts_state([[300.0, 400.0, 0.0], [0.0, 0.0, 0.0]], [0.0, 2.0])  # sqrt(1e-6) = 0.001
+    O.variable_leapfrog(prm, h, st)
+    assert st.current == pytest.approx(0.001, rel=1e-6)
+    h = _vts_state([[0.0, 0.0, 0.0]], [100.0])  # 0.09 * 0.05 / 100 = 4.5e-5
+    O.variable_leapfrog(prm, h, st)
+    assert st.current == pytest.approx(4.5e-5, rel=1e-6)
+    h = _vts_state([[0.0, 0.0, 0.0]], [0.0])  # no criterion binds: dt never grows back
+    O.variable_leapfrog(prm, h, st)
+    assert st.current == pytest.approx(4.5e-5, rel=1e-6)
+    h = _vts_state([[5e8, 0.0, 0.0]], [0.0])  # sqrt(1e-12) = 1e-6 < min_timestep
+    O.variable_leapfrog(prm, h, st)
+    assert st.next == pytest.approx(1e-6, rel=1e-6) and st.current == pytest.approx(2e-5, rel=1e-6)
+
+
+def test_variable_step_from_zero_state_uses_min_timestep():
+    """integrate(pb, true) on the very first call (as main.cu:257 does): the
+    zero-initialised next dt wins the minimum, so dt = min_timestep."""
+    prm = _vts_params()
+    for prec in ("f64", "f32"):
+        st = O.TimestepState(prec)
+        h = _vts_state([[1.0, 0.0, 0.0]], [1.0])
+        O.variable_leapfrog(prm, h, st, not_first_step=True)
+        assert st.next == 0.0
+        assert st.current == pytest.approx(2e-5, rel=1e-6)
+
+
+def test_variable_step_exact_for_constant_acceleration():
+    """Kick-drift-kick with synchronised half kicks integrates constant
+    acceleration exactly for any dt sequence: x = x0 + v0 t + a t^2 / 2."""
+    prm = _vts_params(H=0.1)
+    a = np.array([0.0, -2.0, 0.5])
+    x0, v0 = np.array([0.3, 0.2, -0.1]), np.array([1.0, 0.5, 0.0])
+    h = make(x0[None], np.array([1.0]), vel=v0[None], fields=("posm", "vel", "acc", "maxvsig"))
+    h["acc"][0, :3] = a
+    x0, v0, a = (np.asarray(x, dtype=np.float32).astype(np.float64) for x in (x0, v0, a))  # as stored
+    st = O.TimestepState("f64")
+    t = 0.0
+    dts = []
+    for k, vsig in enumerate([0.0, 2.0, 3.0, 5.0, 8.0, 8.0, 13.0]):
+        h["maxvsig"][:] = vsig  # shrinks dt step by step through the velocity criterion
+        o = O.variable_leapfrog(prm, h, st, not_first_step=(k > 0))
+        h["posm"], h["vel"] = o["posm"], o["vel"]
+        t += st.current
+        dts.append(st.current)
+        assert np.allclose(h["posm"][0, :3], x0 + v0 * t + 0.5 * a * t * t, rtol=0, atol=1e-13)
+        assert np.allclose(h["vel"][0, :3], v0 + a * (t - 0.5 * st.current), rtol=0, atol=1e-13)
+    assert len(set(dts)) >= 4 and all(x >= y for x, y in zip(dts, dts[1:]))
+
+
 # ---------------------------------------------------------------- invariants
 def test_invariants_two_body():
     prm = ics.plummer_params(H=0.5)
