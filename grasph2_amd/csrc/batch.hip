@@ -45,6 +45,7 @@ struct BatchState {
     int n_blk = 0;
     int* d_queue = nullptr;
     float* d_ts = nullptr;  // variable step: {current dt, next dt} per instance, zero when the tables are built
+    float* d_um = nullptr;  // {uniform, mass} per instance, written before each gravity sweep
 
     // Morton order
     float* d_bbox = nullptr;
@@ -99,11 +100,11 @@ void drop_graph(BatchState& b)
 void free_tables(BatchState& b)
 {
     void* ps[] = {b.d_insts, b.d_prm, b.d_inst_of, b.d_items, b.d_blk, b.d_queue, b.d_bbox, b.d_keys, b.d_keys_alt,
-                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts};
+                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um};
     for (void* p : ps) free_ptr(p);
     b.d_insts = nullptr; b.d_prm = nullptr; b.d_inst_of = nullptr; b.d_items = nullptr; b.d_blk = nullptr;
     b.d_queue = nullptr; b.d_bbox = nullptr; b.d_keys = nullptr; b.d_keys_alt = nullptr; b.d_vals = nullptr;
-    b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr;
+    b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr; b.d_um = nullptr;
     b.sort_bytes = 0;
     b.order_valid = false;
     b.n_items = 0;
@@ -204,7 +205,9 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
         std::vector<int> zero(64, 0);
         if ((r = upload(c, b.d_queue, zero, "batch work counter"))) return r;
         std::vector<float> ts0(2 * (size_t)n_inst, 0.0f);
-        if ((r = upload(c, b.d_ts, ts0, "batch time-step state"))) return r;
+        if ((r = upload(c, b.d_ts, ts0, "batch time-step state")) ||
+            (r = upload(c, b.d_um, ts0, "batch mass records")))
+            return r;
         b.key_bits = 30;
         while ((1LL << (b.key_bits - 30)) < n_inst) ++b.key_bits;
         const size_t nt = (size_t)std::max<int64_t>(p->n, 1);
@@ -334,6 +337,11 @@ int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStrea
     bool soft = true;
     for (const auto& q : b.params) soft = soft && q.H > 0.0f;
     L.soft = soft;
+    if (L.grav) {
+        e = launch_mass_check(L.posm, b.d_insts, b.n_inst, 0, b.d_um, s);
+        if (e != hipSuccess) return hip_fail(c, e, "batch mass check launch");
+        L.umass = b.d_um;
+    }
     ProfScope ps(&c, s, KC_SWEEP);
     e = launch_batch_sweep(L, P, s);
     if (e != hipSuccess) return hip_fail(c, e, "batched sweep launch");

@@ -283,6 +283,13 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     L.nslices = slices;
     L.slice_len = slice_len;
     L.part = c.part;
+    if (sweep_grav) {
+        // equal masses let the sweep factor the mass out of the gravity sum
+        ProfScope ps(&c, s, KC_OTHER);
+        hipError_t e = launch_mass_check(L.posm, nullptr, 1, L.n, c.umass, s);
+        if (e != hipSuccess) return hip_fail(c, e, "mass check launch");
+        L.umass = c.umass;
+    }
     if ((r = run_sweep(c, L, P, s))) return r;
 
     NbrLaunch N = nbr_base(c, p);
@@ -399,11 +406,13 @@ int grasph_create(int device, grasph_ctx** out)
     // the variable-step leapfrog's {current dt, next dt}: zero, as the
     // reference's __device__ currentTS / nextTS start
     if (hipMalloc((void**)&ctx->c.ts, 2 * sizeof(float)) != hipSuccess ||
-        hipMemset(ctx->c.ts, 0, 2 * sizeof(float)) != hipSuccess) {
+        hipMemset(ctx->c.ts, 0, 2 * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&ctx->c.umass, 2 * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         if (ctx->c.ts) (void)hipFree(ctx->c.ts);
+        if (ctx->c.umass) (void)hipFree(ctx->c.umass);
         delete ctx;
-        grasph::g_create_err = "device allocation of the time-step state failed";
+        grasph::g_create_err = "device allocation of the context state failed";
         return GRASPH_E_NOMEM;
     }
     int cus = 0;
@@ -424,7 +433,7 @@ int grasph_destroy(grasph_ctx* ctx)
     grasph::destroy_batch_state(c.batch);
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
-                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part, c.ts};
+                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part, c.ts, c.umass};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {

@@ -41,6 +41,8 @@ struct Ctx {
 
     // variable-step leapfrog state {current dt, next dt}, zero at creation
     float* ts = nullptr;
+    // {uniform, mass} of the particles, written before each gravity sweep
+    float* umass = nullptr;
 
     // Morton order of the i side (slot t -> particle perm[t])
     int* perm = nullptr;
@@ -118,6 +120,7 @@ struct SweepLaunch {
     int nslices;        // j-split: slices of slice_len j (lists and counts per slice)
     int slice_len;
     float4* part;       // [nslices][n][2] partial sums when nslices > 1
+    const float* umass; // {uniform, mass} from launch_mass_check, or null
 };
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s);
 
@@ -183,8 +186,13 @@ struct BatchSweepLaunch {
     int blocks;           // persistent grid
     int waves_per_block;  // 1, 2 or 4
     bool grav, want_nbr, soft;
+    const float* umass;   // {uniform, mass} per instance, or null
 };
 hipError_t launch_batch_sweep(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s);
+// umass[2k] = 1 when every particle of instance k (insts null: the n
+// particles of one instance) has the same mass, umass[2k + 1] = that mass
+hipError_t launch_mass_check(const float4* posm, const BatchInst* insts, int n_inst, int n, float* umass,
+                             hipStream_t s);
 hipError_t launch_reset_queue(int* queue, int n, hipStream_t s);
 
 // Batched Morton order: per-instance bounding boxes, keys (instance, Z-order

@@ -87,8 +87,10 @@ __device__ __forceinline__ void neighbour_pair(const float4 pj, const int j, con
 }
 
 // One chunk of kChunk j against the lane's NI particles: gravity (and
-// potential) on every pair, then one wave-uniform candidate test.
-template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+// potential) on every pair, then one wave-uniform candidate test.  UM (all
+// partners of equal mass): the mass is factored out of the sum and applied
+// at the fold, one multiply per pair fewer.
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT, bool UM>
 __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, const float (&xi)[NI],
                                       const float (&yi)[NI], const float (&zi)[NI], Acc<NI>& a, const int slot,
                                       const float H2, const float cand, const Rare& R)
@@ -107,8 +109,14 @@ __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, 
             float inv = 0.0f;
             if (GRAV || POT) inv = __builtin_amdgcn_rsqf(d2[q][c]);
             if (GRAV) {
-                float s = pj[c].w * inv;
-                s = s * (inv * inv);
+                float s;
+                if (UM) {
+                    s = inv * inv;
+                    s = s * inv;
+                } else {
+                    s = pj[c].w * inv;
+                    s = s * (inv * inv);
+                }
                 if (!SOFT) s = (d2[q][c] > 0.0f) ? s : 0.0f;  // r^2 > 0 (computeDerivatives.h:226)
                 a.ax[q] = fmaf(dx, s, a.ax[q]);
                 a.ay[q] = fmaf(dy, s, a.ay[q]);
@@ -135,13 +143,17 @@ __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, 
     }
 }
 
-template <int NI, bool GRAV, bool POT>
-__device__ __forceinline__ void fold(Acc<NI>& a)
+template <int NI, bool GRAV, bool POT, bool UM>
+__device__ __forceinline__ void fold(Acc<NI>& a, const float m)
 {
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         if (GRAV) {
-            a.sx[q].add(a.ax[q]); a.sy[q].add(a.ay[q]); a.sz[q].add(a.az[q]);
+            if (UM) {
+                a.sx[q].add(a.ax[q] * m); a.sy[q].add(a.ay[q] * m); a.sz[q].add(a.az[q] * m);
+            } else {
+                a.sx[q].add(a.ax[q]); a.sy[q].add(a.ay[q]); a.sz[q].add(a.az[q]);
+            }
             a.ax[q] = a.ay[q] = a.az[q] = 0.0f;
         }
         if (POT) {
@@ -151,12 +163,13 @@ __device__ __forceinline__ void fold(Acc<NI>& a)
     }
 }
 
-// Sweep of the lane's NI slots against j in [j0, j1).
-template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+// Sweep of the lane's NI slots against j in [j0, j1); m is the common
+// partner mass when UM.
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT, bool UM>
 __device__ __forceinline__ void sweep_range(const float4* __restrict__ posm, const int j0, const int j1,
                                             const float (&xi)[NI], const float (&yi)[NI], const float (&zi)[NI],
                                             Acc<NI>& a, const int slot, const float H2, const float cand,
-                                            const Rare& R)
+                                            const Rare& R, const float m)
 {
     // Two SGPR buffers in ping-pong: while one chunk is computed the next one
     // loads into the other buffer, so no register copies are needed.
@@ -173,39 +186,40 @@ __device__ __forceinline__ void sweep_range(const float4* __restrict__ posm, con
         const int jb = j0 + k * kChunk;
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) B[c] = posm[jb + kChunk + c];
-        chunk<NI, GRAV, NBR, POT, SOFT>(A, jb, xi, yi, zi, a, slot, H2, cand, R);
-        if ((k % kFoldChunks) == kFoldChunks - 1) fold<NI, GRAV, POT>(a);
+        chunk<NI, GRAV, NBR, POT, SOFT, UM>(A, jb, xi, yi, zi, a, slot, H2, cand, R);
+        if ((k % kFoldChunks) == kFoldChunks - 1) fold<NI, GRAV, POT, UM>(a, m);
         // next A: chunk k + 2, or chunk k + 1 again when k + 2 is past the end
         const int ja = j0 + ((k + 2 < nchunks) ? (k + 2) : (k + 1)) * kChunk;
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) A[c] = posm[ja + c];
-        chunk<NI, GRAV, NBR, POT, SOFT>(B, jb + kChunk, xi, yi, zi, a, slot, H2, cand, R);
-        if (((k + 1) % kFoldChunks) == kFoldChunks - 1) fold<NI, GRAV, POT>(a);
+        chunk<NI, GRAV, NBR, POT, SOFT, UM>(B, jb + kChunk, xi, yi, zi, a, slot, H2, cand, R);
+        if (((k + 1) % kFoldChunks) == kFoldChunks - 1) fold<NI, GRAV, POT, UM>(a, m);
     }
-    if (k < nchunks) chunk<NI, GRAV, NBR, POT, SOFT>(A, j0 + k * kChunk, xi, yi, zi, a, slot, H2, cand, R);
+    if (k < nchunks) chunk<NI, GRAV, NBR, POT, SOFT, UM>(A, j0 + k * kChunk, xi, yi, zi, a, slot, H2, cand, R);
     // tail: pad the last partial chunk with far-away massless partners
     if (nfull < j1) {
         float4 t[kChunk];
 #pragma unroll
         for (int c = 0; c < kChunk; ++c)
             t[c] = (nfull + c < j1) ? posm[nfull + c] : make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
-        chunk<NI, GRAV, NBR, POT, SOFT>(t, nfull, xi, yi, zi, a, slot, H2, cand, R);
+        chunk<NI, GRAV, NBR, POT, SOFT, UM>(t, nfull, xi, yi, zi, a, slot, H2, cand, R);
     }
-    fold<NI, GRAV, POT>(a);
+    fold<NI, GRAV, POT, UM>(a, m);
 }
 
 // Load the lane's particles (slots s0 + 64 q below s_end), run the sweep
 // over [j0, j1) and store the results -- or, with part set (one slice of a
 // j-split sweep), the compensated partial sums (sum, compensation) of the
 // slice, combined later in slice order.
-template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
+template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT, bool UM>
 __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, const int* __restrict__ perm,
                                             const int s0, const int s_end, const int j0, const int j1,
                                             float4* __restrict__ acc, int* __restrict__ nbr_cnt,
                                             float* __restrict__ pot, const int finish,
                                             const float4* __restrict__ vel, float4* __restrict__ xvel,
                                             float* __restrict__ maxvsig, const DevParams& P, const float H2,
-                                            const float cand, const Rare& R, float4* __restrict__ part = nullptr)
+                                            const float cand, const Rare& R, const float m,
+                                            float4* __restrict__ part = nullptr)
 {
     float xi[NI], yi[NI], zi[NI];
     int ii[NI];
@@ -228,7 +242,7 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, con
         a.sx[q] = a.sy[q] = a.sz[q] = a.sphi[q] = CSum{0.0f, 0.0f};
         a.cnt[q] = 0;
     }
-    sweep_range<NI, GRAV, NBR, POT, SOFT>(posm, j0, j1, xi, yi, zi, a, s0, H2, cand, R);
+    sweep_range<NI, GRAV, NBR, POT, SOFT, UM>(posm, j0, j1, xi, yi, zi, a, s0, H2, cand, R, m);
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         if (!act[q]) continue;
@@ -266,7 +280,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict_
                                                        const int kmax, const int finish,
                                                        const float4* __restrict__ vel, float4* __restrict__ xvel,
                                                        float* __restrict__ maxvsig, const DevParams P,
-                                                       const int slice_len, float4* __restrict__ part)
+                                                       const int slice_len, float4* __restrict__ part,
+                                                       const float* __restrict__ umass)
 {
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -277,9 +292,39 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict_
     int j1 = min(n, j0 + slice_len);
     if (!(GRAV || NBR || POT)) j1 = j0;  // nothing pairwise to do: skip the j loop
     const Rare R = make_rare(nbr + (size_t)sl * kmax * n, (size_t)n, kmax, P.H2);
-    sweep_slots<NI, GRAV, NBR, POT, SOFT>(posm, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot, finish, vel,
-                                          xvel, maxvsig, P, P.H2, P.cand, R,
-                                          part ? part + 2 * (size_t)sl * n : nullptr);
+    float4* const pp = part ? part + 2 * (size_t)sl * n : nullptr;
+    if constexpr (GRAV && !POT) {
+        if (umass && umass[0] != 0.0f) {  // {uniform, mass} of the partners, wave-uniform
+            sweep_slots<NI, GRAV, NBR, POT, SOFT, true>(posm, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot,
+                                                        finish, vel, xvel, maxvsig, P, P.H2, P.cand, R, umass[1], pp);
+            return;
+        }
+    }
+    sweep_slots<NI, GRAV, NBR, POT, SOFT, false>(posm, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot,
+                                                 finish, vel, xvel, maxvsig, P, P.H2, P.cand, R, 1.0f, pp);
+}
+
+// {1, m} when every particle of [off, off + n) has mass m (bit for bit),
+// else {0, m}; one block per instance (blockIdx.x; insts null: one instance
+// [0, n)).  Written every evaluation, so no reset is needed.
+__global__ __launch_bounds__(1024) void mass_check_kernel(const float4* __restrict__ posm,
+                                                          const BatchInst* __restrict__ insts, const int n,
+                                                          float* __restrict__ umass)
+{
+    const int k = blockIdx.x;
+    const int off = insts ? insts[k].off : 0, cnt = insts ? insts[k].n : n;
+    if (cnt <= 0) {
+        if (threadIdx.x == 0) { umass[2 * k] = 0.0f; umass[2 * k + 1] = 0.0f; }
+        return;  // the whole block leaves: no barrier below is reached
+    }
+    const float m0 = posm[off].w;
+    int bad = 0;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) bad |= __float_as_int(posm[off + i].w) != __float_as_int(m0);
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) {
+        umass[2 * k] = bad ? 0.0f : 1.0f;
+        umass[2 * k + 1] = m0;
+    }
 }
 
 // Sum the slices' partial gravity in slice order (compensated) and finish
@@ -321,7 +366,8 @@ __global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __res
                                                              const BatchInst* __restrict__ insts,
                                                              int* __restrict__ queue, float4* __restrict__ acc,
                                                              int* __restrict__ nbr, int* __restrict__ nbr_cnt,
-                                                             const int ntotal, const int kmax, const DevParams P)
+                                                             const int ntotal, const int kmax, const DevParams P,
+                                                             const float* __restrict__ umass)
 {
     const int lane = threadIdx.x & 63;
     const Rare R0 = make_rare(nbr, (size_t)ntotal, kmax, 0.0f);
@@ -334,12 +380,22 @@ __global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __res
         const BatchInst I = insts[it.x];
         Rare R = R0;
         R.h2 = vgpr_f32(I.H2);
-        if (it.z == 2)
-            sweep_slots<2, GRAV, NBR, false, SOFT>(posm, perm, it.y + lane, I.off + I.n, I.off, I.off + I.n, acc,
-                                                   nbr_cnt, nullptr, 0, nullptr, nullptr, nullptr, P, I.H2, I.cand, R);
-        else
-            sweep_slots<1, GRAV, NBR, false, SOFT>(posm, perm, it.y + lane, I.off + I.n, I.off, I.off + I.n, acc,
-                                                   nbr_cnt, nullptr, 0, nullptr, nullptr, nullptr, P, I.H2, I.cand, R);
+        const int s0 = it.y + lane, e = I.off + I.n;
+        if (GRAV && umass && umass[2 * it.x] != 0.0f) {
+            const float m = umass[2 * it.x + 1];
+            if (it.z == 2)
+                sweep_slots<2, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+                                                             nullptr, nullptr, nullptr, P, I.H2, I.cand, R, m);
+            else
+                sweep_slots<1, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+                                                             nullptr, nullptr, nullptr, P, I.H2, I.cand, R, m);
+        } else if (it.z == 2) {
+            sweep_slots<2, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+                                                          nullptr, nullptr, nullptr, P, I.H2, I.cand, R, 1.0f);
+        } else {
+            sweep_slots<1, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+                                                          nullptr, nullptr, nullptr, P, I.H2, I.cand, R, 1.0f);
+        }
     }
 }
 
@@ -359,7 +415,7 @@ hipError_t launch_t(const SweepLaunch& L, const DevParams& P, hipStream_t s)
     const int ns = L.nslices > 1 ? L.nslices : 1;
     hipLaunchKernelGGL((sweep_kernel<NI, GRAV, NBR, POT, SOFT>), dim3(blocks, ns), dim3(kBlock), 0, s, L.posm, L.perm,
                        L.n, L.acc, L.nbr, L.nbr_cnt, L.pot, L.kmax, L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P,
-                       ns > 1 ? L.slice_len : L.n, ns > 1 ? L.part : nullptr);
+                       ns > 1 ? L.slice_len : L.n, ns > 1 ? L.part : nullptr, L.umass);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ns == 1 || !(GRAV || L.finish)) return e;
     hipLaunchKernelGGL(combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.part, ns, L.n, L.posm,
@@ -392,7 +448,7 @@ hipError_t launch_b(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s
 {
     hipLaunchKernelGGL((batch_sweep_kernel<GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0, s,
                        L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal, L.kmax,
-                       P);
+                       P, L.umass);
     return hipGetLastError();
 }
 
@@ -402,6 +458,14 @@ hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
     return L.ni == 2 ? launch_ni<2>(L, P, s) : launch_ni<1>(L, P, s);
+}
+
+hipError_t launch_mass_check(const float4* posm, const BatchInst* insts, int n_inst, int n, float* umass,
+                             hipStream_t s)
+{
+    if (n_inst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mass_check_kernel, dim3(n_inst), dim3(1024), 0, s, posm, insts, n, umass);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset_queue(int* queue, int n, hipStream_t s)
