@@ -70,6 +70,22 @@ def test_gravity_matches_fp64(gpu_ctx, n):
     check_vec(out["acc"], ref["acc"])
 
 
+@pytest.mark.parametrize("uniform", [True, False])
+def test_gravity_equal_and_unequal_masses(gpu_ctx, uniform):
+    """Equal masses take the sweep that factors the mass out of the sum;
+    one particle heavier by one ulp takes the general one.  Both against
+    the FP64 oracle, unsoftened and softened."""
+    h, prm = ics.plummer(3000, seed=11)
+    h["posm"][:, 3] = np.float32(1.0 / 3000)
+    if not uniform:
+        h["posm"][1234, 3] = np.nextafter(h["posm"][1234, 3], np.float32(1))
+    for H in (0.0, 0.01):
+        prm.H = H
+        out = gpu_run(gpu_ctx, h, prm)
+        ref = O.calc_acceleration(prm, h, "f64")
+        check_vec(out["acc"], ref["acc"])
+
+
 def test_gravity_unsoftened_and_coincident(gpu_ctx):
     h, prm = ics.plummer(600, seed=3)
     prm.H = 0.0
