@@ -12,9 +12,9 @@ value = pair interactions per second = N (N - 1) / (seconds per step): every
 step evaluates softened gravity on all ordered pairs i != j.
 roofline: the all-pairs sweep kernel (the dominant one), FP32 vector ALU
 bound; its algorithmic FLOPs per launch are 20 per gravity pair (FMA = 2,
-rsqrt = 1) over N (N - 1) pairs plus 9 per neighbour pair for the density
-sum, divided by its mean duration from HIP events recorded around every
-launch on the launch stream over the timed steps.
+rsqrt = 1) over N (N - 1) pairs (its neighbour search rides on the same r^2
+and adds no counted FLOPs), divided by its mean duration from HIP events
+recorded around every launch on the launch stream over the timed steps.
 cpu_baseline: the CPU oracle in FP32 (the same op structure as the
 reference's host path) on a bounded sample of target particles i against
 all j, all three passes, OpenMP threads as reported.
@@ -159,13 +159,14 @@ def main():
     value = pairs / (ms_step * 1e-3)
 
     sweep_ms = kt["sweep"][0] / max(kt["sweep"][1], 1)
-    sweep_flops = FLOP_GRAVITY_PAIR * pairs + (FLOP_DENSITY_NBR * nb_pairs if sph else 0)
+    sweep_flops = FLOP_GRAVITY_PAIR * pairs
     achieved = sweep_flops / (sweep_ms * 1e-3) / 1e12
     integ_ms = kt["integrate"][0] / max(kt["integrate"][1], 1)
     integ_bytes = n * (INTEGRATOR_BYTES_SOLID if solid else INTEGRATOR_BYTES_NBODY)
     step_flops = sweep_flops
     if sph:
-        step_flops += nb_pairs * (FLOP_STRAIN_NBR + (FLOP_ACCEL_NBR_SOLID if solid else FLOP_ACCEL_NBR_LIQUID))
+        step_flops += nb_pairs * (FLOP_DENSITY_NBR + FLOP_STRAIN_NBR +
+                                  (FLOP_ACCEL_NBR_SOLID if solid else FLOP_ACCEL_NBR_LIQUID))
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "sweep_%s_%d_traffic.json" % (args.workload, n))
