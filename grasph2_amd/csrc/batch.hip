@@ -34,6 +34,7 @@ struct BatchState {
     std::vector<grasph_params> params;
     uint32_t flags = 0;
     int tuned_ni[4] = {0, 0, 0, 0};  // i per lane the items table was built with
+    int items_ni_max = 1;            // largest ni among the items
 
     // device tables
     BatchInst* d_insts = nullptr;
@@ -136,9 +137,11 @@ int build_items(Ctx& c, BatchState& b)
         return (b.offsets[x + 1] - b.offsets[x]) > (b.offsets[y + 1] - b.offsets[y]);
     });
     std::vector<int4> items;
+    b.items_ni_max = 1;
     for (int k : order) {
         const int64_t off = b.offsets[k], n = b.offsets[k + 1] - off;
         const int ni = c.batch_ni[bucket_of(n)];
+        if (n > 0) b.items_ni_max = std::max(b.items_ni_max, ni);
         for (int64_t s = off; s < off + n; s += 64 * ni) items.push_back(make_int4(k, (int)s, ni, 0));
     }
     b.n_items = (int)items.size();
@@ -332,6 +335,7 @@ int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStrea
     L.kmax = b.kmax;
     L.waves_per_block = c.batch_wpb;
     L.blocks = device_cus(c) * c.batch_bpc;
+    L.ni_max = b.items_ni_max;
     L.grav = f & GRASPH_SELF_GRAVITY;
     L.want_nbr = f & GRASPH_SPH;
     bool soft = true;

@@ -187,6 +187,7 @@ struct BatchSweepLaunch {
     int waves_per_block;  // 1, 2 or 4
     bool grav, want_nbr, soft;
     const float* umass;   // {uniform, mass} per instance, or null
+    int ni_max;           // largest ni of the items (1 or 2)
 };
 hipError_t launch_batch_sweep(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s);
 // umass[2k] = 1 when every particle of instance k (insts null: the n

@@ -358,8 +358,10 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(const float4* __restric
 // (inst, slot0, ni) covers slots [slot0, slot0 + 64 ni) of instance inst
 // with ni particles per lane (chosen per size bucket); the instance's
 // particles are [off, off + n) and its slots are the same range of perm.
-// Every wavefront leaves the loop once the counter passes n_items.
-template <bool GRAV, bool NBR, bool SOFT>
+// Every wavefront leaves the loop once the counter passes n_items.  NIMAX
+// is the largest ni of the items: the two-per-lane code needs about twice
+// the registers, so a batch without such items gets the leaner kernel.
+template <int NIMAX, bool GRAV, bool NBR, bool SOFT>
 __global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __restrict__ posm,
                                                              const int* __restrict__ perm,
                                                              const int4* __restrict__ items, const int n_items,
@@ -383,14 +385,14 @@ __global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __res
         const int s0 = it.y + lane, e = I.off + I.n;
         if (GRAV && umass && umass[2 * it.x] != 0.0f) {
             const float m = umass[2 * it.x + 1];
-            if (it.z == 2)
-                sweep_slots<2, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+            if (NIMAX == 2 && it.z == 2)
+                sweep_slots<NIMAX, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                              nullptr, nullptr, nullptr, P, I.H2, I.cand, R, m);
             else
                 sweep_slots<1, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                              nullptr, nullptr, nullptr, P, I.H2, I.cand, R, m);
-        } else if (it.z == 2) {
-            sweep_slots<2, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+        } else if (NIMAX == 2 && it.z == 2) {
+            sweep_slots<NIMAX, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                           nullptr, nullptr, nullptr, P, I.H2, I.cand, R, 1.0f);
         } else {
             sweep_slots<1, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
@@ -446,9 +448,14 @@ hipError_t launch_ni(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 template <bool GRAV, bool NBR, bool SOFT>
 hipError_t launch_b(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_sweep_kernel<GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0, s,
-                       L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal, L.kmax,
-                       P, L.umass);
+    if (L.ni_max == 2)
+        hipLaunchKernelGGL((batch_sweep_kernel<2, GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0,
+                           s, L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal,
+                           L.kmax, P, L.umass);
+    else
+        hipLaunchKernelGGL((batch_sweep_kernel<1, GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0,
+                           s, L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal,
+                           L.kmax, P, L.umass);
     return hipGetLastError();
 }
 
@@ -478,6 +485,7 @@ hipError_t launch_batch_sweep(const BatchSweepLaunch& L, const DevParams& P, hip
 {
     if (L.n_items <= 0) return hipSuccess;
     if (L.waves_per_block < 1 || L.waves_per_block > kBlock / 64) return hipErrorInvalidValue;
+    if (L.ni_max != 1 && L.ni_max != 2) return hipErrorInvalidValue;
     const int key = (L.grav ? 4 : 0) | (L.want_nbr ? 2 : 0) | (L.soft ? 1 : 0);
     switch (key) {
         case 1: return launch_b<false, false, true>(L, P, s);
