@@ -47,6 +47,10 @@ struct BatchState {
     int* d_queue = nullptr;
     float* d_ts = nullptr;  // variable step: {current dt, next dt} per instance, zero when the tables are built
     float* d_um = nullptr;  // {uniform, mass} per instance, written before each gravity sweep
+    // slot-ordered copies of positions, velocities and densities (SPH with a Morton order)
+    float4* d_xs = nullptr;
+    float4* d_vs = nullptr;
+    float* d_ds = nullptr;
 
     // Morton order
     float* d_bbox = nullptr;
@@ -101,11 +105,12 @@ void drop_graph(BatchState& b)
 void free_tables(BatchState& b)
 {
     void* ps[] = {b.d_insts, b.d_prm, b.d_inst_of, b.d_items, b.d_blk, b.d_queue, b.d_bbox, b.d_keys, b.d_keys_alt,
-                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um};
+                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um, b.d_xs, b.d_vs, b.d_ds};
     for (void* p : ps) free_ptr(p);
     b.d_insts = nullptr; b.d_prm = nullptr; b.d_inst_of = nullptr; b.d_items = nullptr; b.d_blk = nullptr;
     b.d_queue = nullptr; b.d_bbox = nullptr; b.d_keys = nullptr; b.d_keys_alt = nullptr; b.d_vals = nullptr;
     b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr; b.d_um = nullptr;
+    b.d_xs = nullptr; b.d_vs = nullptr; b.d_ds = nullptr;
     b.sort_bytes = 0;
     b.order_valid = false;
     b.n_items = 0;
@@ -218,7 +223,10 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
             hipMalloc((void**)&b.d_keys, nt * sizeof(uint64_t)) != hipSuccess ||
             hipMalloc((void**)&b.d_keys_alt, nt * sizeof(uint64_t)) != hipSuccess ||
             hipMalloc((void**)&b.d_vals, nt * sizeof(int)) != hipSuccess ||
-            hipMalloc((void**)&b.d_perm, nt * sizeof(int)) != hipSuccess) {
+            hipMalloc((void**)&b.d_perm, nt * sizeof(int)) != hipSuccess ||
+            hipMalloc((void**)&b.d_xs, nt * sizeof(float4)) != hipSuccess ||
+            hipMalloc((void**)&b.d_vs, nt * sizeof(float4)) != hipSuccess ||
+            hipMalloc((void**)&b.d_ds, nt * sizeof(float)) != hipSuccess) {
             (void)hipGetLastError();
             return fail(c, GRASPH_E_NOMEM, "device allocation of the batch order failed");
         }
@@ -314,6 +322,28 @@ int device_cus(Ctx& c)
     return cus;
 }
 
+// The slot order of the sweep and the passes: the Morton order for SPH (the
+// neighbour work wants it), none for gravity alone.
+const int* slot_order(const Ctx& c, const BatchState& b)
+{
+    return ((b.flags & GRASPH_SPH) && b.order_valid) ? order_of(c, b) : nullptr;
+}
+
+// slot-ordered copies (positions; velocities and, when no density pass
+// writes them, densities) for the current order
+int batch_gather(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
+{
+    const int* perm = slot_order(c, b);
+    if (!perm) return 0;
+    const bool dens_pass = !(b.flags & GRASPH_INTEGRATE_DENSITY);
+    ProfScope ps(&c, s, KC_REORDER);
+    hipError_t e = launch_gather(perm, (int)b.ntotal, (const float4*)p->posm, (const float4*)p->vel,
+                                 dens_pass ? nullptr : p->density, b.d_xs, b.d_vs, b.d_ds, s);
+    if (e != hipSuccess) return hip_fail(c, e, "batched slot-order gather launch");
+    return 0;
+}
+
+// the persistent sweep; with a slot order, batch_gather must have run for the current positions
 int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
 {
     const uint32_t f = b.flags;
@@ -322,8 +352,8 @@ int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStrea
     if (e != hipSuccess) return hip_fail(c, e, "batch work counter reset");
     BatchSweepLaunch L;
     std::memset(&L, 0, sizeof L);
-    L.posm = (const float4*)p->posm;
-    L.perm = order_of(c, b);
+    L.perm = slot_order(c, b);
+    L.xs = L.perm ? b.d_xs : (const float4*)p->posm;
     L.items = b.d_items;
     L.n_items = b.n_items;
     L.insts = b.d_insts;
@@ -342,7 +372,7 @@ int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStrea
     for (const auto& q : b.params) soft = soft && q.H > 0.0f;
     L.soft = soft;
     if (L.grav) {
-        e = launch_mass_check(L.posm, b.d_insts, b.n_inst, 0, b.d_um, s);
+        e = launch_mass_check((const float4*)p->posm, b.d_insts, b.n_inst, 0, b.d_um, s);
         if (e != hipSuccess) return hip_fail(c, e, "batch mass check launch");
         L.umass = b.d_um;
     }
@@ -361,16 +391,19 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     const bool strain_pass = sph && ((f & GRASPH_SOLIDS) || (f & GRASPH_INTEGRATE_DENSITY) || (f & GRASPH_BALSARA_SWITCH));
     const DevParams P = make_dev_params(&b.params[0]);  // flags; values come from the per-instance table
     int r;
+    if ((r = batch_gather(c, b, p, s))) return r;
     if ((sph || grav) && (r = launch_sweep_only(c, b, p, s))) return r;
 
+    const int* perm = slot_order(c, b);
     NbrLaunch N;
     std::memset(&N, 0, sizeof N);
-    N.posm = (const float4*)p->posm;
-    N.vel = (const float4*)p->vel;
-    N.density = p->density;
+    N.posm = perm ? b.d_xs : (const float4*)p->posm;
+    N.vel = perm ? b.d_vs : (const float4*)p->vel;
+    N.density = perm ? b.d_ds : p->density;
+    N.density_slots = perm ? b.d_ds : nullptr;
     N.dstress = p->dstress;
     N.balsara_in = p->balsara;
-    N.perm = order_of(c, b);
+    N.perm = perm;
     N.nbr = b.d_nbr;
     N.nbr_cnt = b.d_cnt;
     N.n = (int)b.ntotal;
@@ -461,7 +494,8 @@ std::string graph_key(const Ctx& c, const BatchState& b, const grasph_particles*
     k << p->posm << ' ' << p->pos64 << ' ' << p->vel << ' ' << p->acc << ' ' << p->xvel << ' ' << p->density << ' '
       << p->density_dt << ' ' << p->balsara << ' ' << p->maxvsig << ' ' << p->dstress << ' ' << p->dstress_dt << ' '
       << (c.reorder_interval > 0) << ' ' << c.batch_wpb << ' ' << c.batch_bpc << ' ' << (const void*)b.d_nbr << ' '
-      << (const void*)b.d_jrec << ' ' << (const void*)b.d_items << ' ' << c.timestep_reset;
+      << (const void*)b.d_jrec << ' ' << (const void*)b.d_items << ' ' << (const void*)b.d_xs << ' '
+      << c.timestep_reset;
     return k.str();
 }
 
@@ -681,6 +715,7 @@ int grasph_autotune(grasph_ctx* ctx, const grasph_particles* p, const grasph_bat
     // launch shape with one particle per lane everywhere
     for (int q = 0; q < 4; ++q) c.batch_ni[q] = 1;
     if ((r = grasph::prepare(c, p, b))) return r;
+    if ((r = grasph::batch_gather(c, B, p, s))) return r;  // the timed sweeps read the slot-ordered copies
     const int shapes[][2] = {{1, 8}, {1, 16}, {2, 4}, {2, 8}, {4, 2}, {4, 4}, {4, 8}};
     double best = 1e30;
     int bw = 4, bb = 8;

@@ -273,9 +273,33 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     }
     const DevParams P = make_dev_params(prm);
 
+    // slot-ordered copies of what the sweep and the passes read
+    const int* perm = need_list ? order_of(c) : nullptr;
+    const bool list_reads_vel = strain_pass || (accel_pass && sph);
+    const bool list_reads_density = list_reads_vel;
+    const float4* xs = (const float4*)p->posm;
+    const float4* vs = (const float4*)p->vel;
+    const float* ds = p->density;
+    if (perm) {
+        const bool gvel = list_reads_vel && p->vel;
+        const bool gdens = list_reads_density && p->density && !dens_pass;  // else the density pass writes them
+        if ((r = grow(c, c.xs, c.xs_cap, n, "slot-ordered positions")) ||
+            (gvel && (r = grow(c, c.vs, c.vs_cap, n, "slot-ordered velocities"))) ||
+            ((list_reads_density || dens_pass) && (r = grow(c, c.ds, c.ds_cap, n, "slot-ordered densities"))))
+            return r;
+        ProfScope ps(&c, s, KC_REORDER);
+        hipError_t e = launch_gather(perm, (int)n, (const float4*)p->posm, gvel ? (const float4*)p->vel : nullptr,
+                                     gdens ? p->density : nullptr, c.xs, c.vs, c.ds, s);
+        if (e != hipSuccess) return hip_fail(c, e, "slot-order gather launch");
+        xs = c.xs;
+        vs = gvel ? c.vs : nullptr;
+        ds = c.ds;
+    }
+
     // one all-pairs sweep
     SweepLaunch L = sweep_base(c, p);
-    if (!need_list) L.perm = nullptr;
+    L.perm = perm;
+    L.xs = xs;
     L.grav = sweep_grav;
     L.want_nbr = need_list;
     L.want_pot = false;
@@ -293,6 +317,11 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     if ((r = run_sweep(c, L, P, s))) return r;
 
     NbrLaunch N = nbr_base(c, p);
+    N.perm = perm;
+    N.posm = xs;
+    N.vel = vs;
+    N.density = ds;
+    N.density_slots = perm ? c.ds : nullptr;
     N.nslices = slices;
     N.slice_len = slice_len;
     if (dens_pass) {
@@ -433,7 +462,8 @@ int grasph_destroy(grasph_ctx* ctx)
     grasph::destroy_batch_state(c.batch);
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
-                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part, c.ts, c.umass};
+                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part, c.ts, c.umass,
+                    c.xs,   c.vs,      c.ds};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {

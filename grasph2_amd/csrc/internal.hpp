@@ -67,9 +67,16 @@ struct Ctx {
     float4* part = nullptr;   // partial sums of a j-split sweep
     int64_t part_cap = 0;
 
-    // pass-C per-particle record: sigma/rho^2 (6), balsara, density
+    // pass-C per-particle record: sigma/rho^2 (6), balsara, density (slot order)
     float4* jrec = nullptr;
     int64_t jrec_cap = 0;
+    // slot-ordered copies of positions, velocities and densities
+    float4* xs = nullptr;
+    int64_t xs_cap = 0;
+    float4* vs = nullptr;
+    int64_t vs_cap = 0;
+    float* ds = nullptr;
+    int64_t ds_cap = 0;
 
     // potential and reduction partials
     float* pot = nullptr;
@@ -103,7 +110,8 @@ struct ProfScope {
 
 // ---- launchers (return hipError_t of the launch) -------------------------
 struct SweepLaunch {
-    const float4* posm;
+    const float4* posm;  // particles (index order)
+    const float4* xs;    // the same in slot order (xs[t] = posm[perm[t]]), or null when perm is null
     const int* perm;
     int n;
     float4* acc;        // gravity (written when grav)
@@ -172,7 +180,7 @@ __device__ inline SlotMap slot_map(const BlockTables& T, int n)
 }
 
 struct BatchSweepLaunch {
-    const float4* posm;
+    const float4* xs;   // particles in slot order (posm itself when perm is null)
     const int* perm;
     const int4* items;  // (instance, first slot, i per lane, 0) per work item, longest instance first
     int n_items;
@@ -216,10 +224,15 @@ struct BatchOrder {
 size_t batch_order_tmp_bytes(int ntotal, int key_bits);
 hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s);
 
+// Neighbour passes.  posm, vel and density are read in slot order (slot t
+// is particle perm[t]; with perm null the caller's arrays themselves), the
+// per-particle outputs go to particle perm[t] of the caller's arrays, and
+// the lists hold slot indices.
 struct NbrLaunch {
     const float4* posm;
     const float4* vel;
     const float* density;
+    float* density_slots;  // density pass: slot-ordered copy of its output (null when perm is null)
     const float* dstress;
     const float* balsara_in;
     const int* perm;
@@ -293,6 +306,10 @@ hipError_t launch_invariants(const InvLaunch& L, const grasph_params& prm, hipSt
 // Z-order curve over the bounding box of posm.
 size_t reorder_tmp_bytes(int n);
 hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s);
+// Slot-ordered copies for the sweep and the list passes: xs[t] = posm[perm[t]],
+// and vs / ds from vel / density when those are given (null: skipped).
+hipError_t launch_gather(const int* perm, int n, const float4* posm, const float4* vel, const float* density,
+                         float4* xs, float4* vs, float* ds, hipStream_t s);
 
 // environmentAcceleration and the zero XVEL / MAXVSIG stores of
 // calcAcceleration for configurations without SPH (batched form).

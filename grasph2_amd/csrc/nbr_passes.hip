@@ -11,12 +11,15 @@
 //          environmentAcceleration (:300-312), added to the gravity the
 //          sweep left in acc with compensated (TwoSum) accumulation.
 //
-// One particle per lane in Morton order (slot t -> i = perm[t]); the list is
-// stored slot-major, nbr[k * nstride + t], so a wavefront's loads of its
-// k-th neighbours are contiguous.  Lists are in increasing j, the
-// reference's summation order.  A particle whose count exceeds the list
-// capacity falls back to a scan over its instance with the exact neighbour
-// test, so a full list never loses partners.
+// One particle per lane in Morton order (slot t -> i = perm[t]).  Inputs
+// are read through slot-ordered copies (positions, velocities, densities,
+// and the record, which is written in slot order), so a neighbour's data
+// sits near its neighbours' in memory; outputs go to particle i of the
+// caller's arrays.  The list is stored slot-major, nbr[k * nstride + t], so
+// a wavefront's loads of its k-th neighbours are contiguous, and holds slot
+// indices in increasing order.  A particle whose count exceeds the list
+// capacity falls back to a scan over its instance's slots with the exact
+// neighbour test, so a full list never loses partners.
 //
 // Every kernel serves one instance (SlotMap from the kernel arguments) or a
 // batch (SlotMap and parameters of the block's instance from tables).
@@ -90,7 +93,8 @@ template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
                                                          const Lists Ls,
                                                          const size_t nstride, const int kmax,
-                                                         float* __restrict__ density, const BlockTables T,
+                                                         float* __restrict__ density,
+                                                         float* __restrict__ density_slots, const BlockTables T,
                                                          const DevParams P0)
 {
     const SlotMap m = slot_map<BATCH>(T, (int)nstride);
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = posm[i];
+    const float4 pi = posm[t];
     CSum rho{0.0f, 0.0f};
     for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = posm[j];
@@ -108,6 +112,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     float r = rho.value();
     if ((P.flags & GRASPH_DEAL_WITH_NO_PARTNERS) && r < 0.001f) r = P.rho0 * 0.01f;  // :85-88
     density[i] = r;
+    if (density_slots) density_slots[t] = r;
 }
 
 template <bool BATCH, bool SOLIDS, bool BALS, bool INTD>
@@ -122,9 +127,9 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = posm[i];
-    const float4 vi = vel[i];
-    const float rhoi = density[i];
+    const float4 pi = posm[t];
+    const float4 vi = vel[t];
+    const float rhoi = density[t];
 
     float exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;  // strain rate (symmetric)
     float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
@@ -198,13 +203,13 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     if (SOLIDS) {
         const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
         const float d2 = rhoi * rhoi;
-        jrec[2 * (size_t)i] = make_float4((S.m[0][0] - pres) / d2, (S.m[1][1] - pres) / d2, (S.m[2][2] - pres) / d2,
+        jrec[2 * (size_t)t] = make_float4((S.m[0][0] - pres) / d2, (S.m[1][1] - pres) / d2, (S.m[2][2] - pres) / d2,
                                           S.m[0][1] / d2);
-        jrec[2 * (size_t)i + 1] = make_float4(S.m[0][2] / d2, S.m[1][2] / d2, bal, rhoi);
+        jrec[2 * (size_t)t + 1] = make_float4(S.m[0][2] / d2, S.m[1][2] / d2, bal, rhoi);
     } else {
         const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
-        jrec[2 * (size_t)i] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
-        jrec[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
+        jrec[2 * (size_t)t] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
+        jrec[2 * (size_t)t + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
     }
 }
 
@@ -221,19 +226,19 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(const float* __restrict__ 
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float rhoi = density[i];
+    const float rhoi = density[t];
     const float bal = balsara ? balsara[i] : 0.0f;
     if (P.flags & GRASPH_SOLIDS) {
         const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
         const float d2 = rhoi * rhoi;
-        jrec[2 * (size_t)i] =
+        jrec[2 * (size_t)t] =
             make_float4((dstress[m.plane6(0, i)] - pres) / d2, (dstress[m.plane6(1, i)] - pres) / d2,
                         (dstress[m.plane6(2, i)] - pres) / d2, dstress[m.plane6(3, i)] / d2);
-        jrec[2 * (size_t)i + 1] = make_float4(dstress[m.plane6(4, i)] / d2, dstress[m.plane6(5, i)] / d2, bal, rhoi);
+        jrec[2 * (size_t)t + 1] = make_float4(dstress[m.plane6(4, i)] / d2, dstress[m.plane6(5, i)] / d2, bal, rhoi);
     } else {
         const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
-        jrec[2 * (size_t)i] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
-        jrec[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
+        jrec[2 * (size_t)t] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
+        jrec[2 * (size_t)t + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
     }
 }
 
@@ -249,9 +254,9 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = posm[i];
-    const float4 vi = vel[i];
-    const float4 ra = jrec[2 * (size_t)i], rb = jrec[2 * (size_t)i + 1];
+    const float4 pi = posm[t];
+    const float4 vi = vel[t];
+    const float4 ra = jrec[2 * (size_t)t], rb = jrec[2 * (size_t)t + 1];
     const float rhoi = rb.w, bi = BALS ? rb.z : 1.0f;
     // sigma_i / rho_i^2 and, for liquids with artificial pressure, arts_i
     const float sxx = ra.x, syy = ra.y, szz = ra.z, sxy = ra.w, sxz = rb.x, syz = rb.y;
@@ -431,10 +436,10 @@ hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
         hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, lists_of(L),
-                           (size_t)L.n, L.kmax, density, L.T, P);
+                           (size_t)L.n, L.kmax, density, L.density_slots, L.T, P);
     else
         hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, lists_of(L),
-                           (size_t)L.n, L.kmax, density, L.T, P);
+                           (size_t)L.n, L.kmax, density, L.density_slots, L.T, P);
     return hipGetLastError();
 }
 

@@ -1,11 +1,13 @@
-// reorder.hip -- Morton (Z-order) ordering of the i side (gfx950).
+// reorder.hip -- Morton (Z-order) ordering of the particles (gfx950).
 //
-// The sweep and the neighbour passes map slot t to particle perm[t].  Along
-// a Z-order curve consecutive slots are spatial neighbours, so the 64 lanes
-// of a wavefront share most of their SPH partners: the neighbour branch of
-// the sweep is taken by few j per wavefront and the list gathers hit the
-// same cache lines.  The order only affects speed, never results: every
-// sum still runs over j in index order.
+// The sweep and the neighbour passes map slot t to particle perm[t] and
+// read the particles through slot-ordered copies (gather_kernel: positions,
+// velocities, densities).  Along a Z-order curve consecutive slots are
+// spatial neighbours, so the 64 lanes of a wavefront share most of their
+// SPH partners, consecutive j of the sweep are close together (few j chunks
+// hold neighbour candidates), and the list passes' gathers of neighbour data
+// hit the same cache lines.  The order changes the summation order of the
+// (compensated) sums, not which pairs are summed.
 #include <hipcub/hipcub.hpp>
 
 #include "internal.hpp"
@@ -132,7 +134,31 @@ __global__ __launch_bounds__(kBlock) void batch_keys(const float4* __restrict__ 
     vals[i] = i;
 }
 
+// slot-ordered copies: xs[t] = posm[perm[t]] and, where given, vs / ds from vel / density
+__global__ __launch_bounds__(kBlock) void gather_kernel(const int* __restrict__ perm, const int n,
+                                                        const float4* __restrict__ posm, const float4* __restrict__ vel,
+                                                        const float* __restrict__ density, float4* __restrict__ xs,
+                                                        float4* __restrict__ vs, float* __restrict__ ds)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int i = perm[t];
+    xs[t] = posm[i];
+    if (vel) vs[t] = vel[i];
+    if (density) ds[t] = density[i];
+}
+
 }  // namespace
+
+hipError_t launch_gather(const int* perm, int n, const float4* posm, const float4* vel, const float* density,
+                         float4* xs, float4* vs, float* ds, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (!perm || !posm || !xs || (vel && !vs) || (density && !ds)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gather_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, perm, n, posm, vel,
+                       density, xs, vs, ds);
+    return hipGetLastError();
+}
 
 size_t batch_order_tmp_bytes(int ntotal, int key_bits)
 {

@@ -7,12 +7,16 @@
 // SPH acceleration).  It replaces the tiled device loop
 // do_for_each_pair_fast_impl (src/particles/algorithms.h:270-309).
 //
-// Mapping: NI i particles per lane (slots t and t + 64 for NI = 2), i taken
-// in Morton order (perm) so that a wavefront's particles are spatial
-// neighbours and the neighbour branch is coherent.  All lanes visit the same
-// j at the same time, so the j stream (float4 x,y,z,m) is read with scalar
-// loads into SGPRs -- no LDS, no barriers -- in chunks of 8, two SGPR
-// buffers in ping-pong so the next chunk loads while this one computes.
+// Mapping: NI i particles per lane (slots t and t + 64 for NI = 2).  The
+// particle stream xs is in slot order: with a Morton order it is the
+// positions gathered as xs[t] = posm[perm[t]] (a wavefront's particles are
+// spatial neighbours, so the neighbour branch is coherent, and consecutive
+// j are close too, so few j chunks hold candidates), without one it is
+// posm itself.  Results go to particle perm[t]; the lists hold slot
+// indices.  All lanes visit the same j at the same time, so the j stream
+// (float4 x,y,z,m) is read with scalar loads into SGPRs -- no LDS, no
+// barriers -- in chunks of 8, two SGPR buffers in ping-pong so the next
+// chunk loads while this one computes.
 // Per pair: 3 sub, 3 FMA for r^2 + H^2, one v_rsq_f32, 3 mul, 3 FMA, and a
 // v_min towards one wave-uniform neighbour-candidate test per chunk; only
 // chunks where some lane has a candidate enter the (rare) neighbour path.
@@ -212,7 +216,7 @@ __device__ __forceinline__ void sweep_range(const float4* __restrict__ posm, con
 // j-split sweep), the compensated partial sums (sum, compensation) of the
 // slice, combined later in slice order.
 template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT, bool UM>
-__device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, const int* __restrict__ perm,
+__device__ __forceinline__ void sweep_slots(const float4* __restrict__ xs, const int* __restrict__ perm,
                                             const int s0, const int s_end, const int j0, const int j1,
                                             float4* __restrict__ acc, int* __restrict__ nbr_cnt,
                                             float* __restrict__ pot, const int finish,
@@ -228,10 +232,10 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, con
     for (int q = 0; q < NI; ++q) {
         const int t = s0 + 64 * q;
         act[q] = t < s_end;
-        ii[q] = act[q] ? (perm ? perm[t] : t) : 0;
+        ii[q] = act[q] ? (perm ? perm[t] : t) : 0;  // the particle of slot t
         xi[q] = yi[q] = zi[q] = __int_as_float(0x7fc00000);  // NaN: inactive slots never branch
         if (act[q]) {
-            const float4 p = posm[ii[q]];
+            const float4 p = xs[t];
             xi[q] = p.x; yi[q] = p.y; zi[q] = p.z;
         }
     }
@@ -242,7 +246,7 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, con
         a.sx[q] = a.sy[q] = a.sz[q] = a.sphi[q] = CSum{0.0f, 0.0f};
         a.cnt[q] = 0;
     }
-    sweep_range<NI, GRAV, NBR, POT, SOFT, UM>(posm, j0, j1, xi, yi, zi, a, s0, H2, cand, R, m);
+    sweep_range<NI, GRAV, NBR, POT, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, s0, H2, cand, R, m);
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         if (!act[q]) continue;
@@ -274,7 +278,7 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ posm, con
 }
 
 template <int NI, bool GRAV, bool NBR, bool POT, bool SOFT>
-__global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
+__global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict__ xs, const int* __restrict__ perm,
                                                        const int n, float4* __restrict__ acc, int* __restrict__ nbr,
                                                        int* __restrict__ nbr_cnt, float* __restrict__ pot,
                                                        const int kmax, const int finish,
@@ -295,12 +299,12 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict_
     float4* const pp = part ? part + 2 * (size_t)sl * n : nullptr;
     if constexpr (GRAV && !POT) {
         if (umass && umass[0] != 0.0f) {  // {uniform, mass} of the partners, wave-uniform
-            sweep_slots<NI, GRAV, NBR, POT, SOFT, true>(posm, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot,
+            sweep_slots<NI, GRAV, NBR, POT, SOFT, true>(xs, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot,
                                                         finish, vel, xvel, maxvsig, P, P.H2, P.cand, R, umass[1], pp);
             return;
         }
     }
-    sweep_slots<NI, GRAV, NBR, POT, SOFT, false>(posm, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot,
+    sweep_slots<NI, GRAV, NBR, POT, SOFT, false>(xs, perm, s0, n, j0, j1, acc, nbr_cnt + (size_t)sl * n, pot,
                                                  finish, vel, xvel, maxvsig, P, P.H2, P.cand, R, 1.0f, pp);
 }
 
@@ -362,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(const float4* __restric
 // is the largest ni of the items: the two-per-lane code needs about twice
 // the registers, so a batch without such items gets the leaner kernel.
 template <int NIMAX, bool GRAV, bool NBR, bool SOFT>
-__global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __restrict__ posm,
+__global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __restrict__ xs,
                                                              const int* __restrict__ perm,
                                                              const int4* __restrict__ items, const int n_items,
                                                              const BatchInst* __restrict__ insts,
@@ -386,16 +390,16 @@ __global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __res
         if (GRAV && umass && umass[2 * it.x] != 0.0f) {
             const float m = umass[2 * it.x + 1];
             if (NIMAX == 2 && it.z == 2)
-                sweep_slots<NIMAX, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+                sweep_slots<NIMAX, GRAV, NBR, false, SOFT, GRAV>(xs, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                              nullptr, nullptr, nullptr, P, I.H2, I.cand, R, m);
             else
-                sweep_slots<1, GRAV, NBR, false, SOFT, GRAV>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+                sweep_slots<1, GRAV, NBR, false, SOFT, GRAV>(xs, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                              nullptr, nullptr, nullptr, P, I.H2, I.cand, R, m);
         } else if (NIMAX == 2 && it.z == 2) {
-            sweep_slots<NIMAX, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+            sweep_slots<NIMAX, GRAV, NBR, false, SOFT, false>(xs, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                           nullptr, nullptr, nullptr, P, I.H2, I.cand, R, 1.0f);
         } else {
-            sweep_slots<1, GRAV, NBR, false, SOFT, false>(posm, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
+            sweep_slots<1, GRAV, NBR, false, SOFT, false>(xs, perm, s0, e, I.off, e, acc, nbr_cnt, nullptr, 0,
                                                           nullptr, nullptr, nullptr, P, I.H2, I.cand, R, 1.0f);
         }
     }
@@ -415,7 +419,8 @@ hipError_t launch_t(const SweepLaunch& L, const DevParams& P, hipStream_t s)
     const int waves = (L.n + 64 * NI - 1) / (64 * NI);
     const int blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
     const int ns = L.nslices > 1 ? L.nslices : 1;
-    hipLaunchKernelGGL((sweep_kernel<NI, GRAV, NBR, POT, SOFT>), dim3(blocks, ns), dim3(kBlock), 0, s, L.posm, L.perm,
+    hipLaunchKernelGGL((sweep_kernel<NI, GRAV, NBR, POT, SOFT>), dim3(blocks, ns), dim3(kBlock), 0, s,
+                       L.xs ? L.xs : L.posm, L.perm,
                        L.n, L.acc, L.nbr, L.nbr_cnt, L.pot, L.kmax, L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P,
                        ns > 1 ? L.slice_len : L.n, ns > 1 ? L.part : nullptr, L.umass);
     hipError_t e = hipGetLastError();
@@ -450,11 +455,11 @@ hipError_t launch_b(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s
 {
     if (L.ni_max == 2)
         hipLaunchKernelGGL((batch_sweep_kernel<2, GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0,
-                           s, L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal,
+                           s, L.xs, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal,
                            L.kmax, P, L.umass);
     else
         hipLaunchKernelGGL((batch_sweep_kernel<1, GRAV, NBR, SOFT>), dim3(L.blocks), dim3(64 * L.waves_per_block), 0,
-                           s, L.posm, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal,
+                           s, L.xs, L.perm, L.items, L.n_items, L.insts, L.queue, L.acc, L.nbr, L.nbr_cnt, L.ntotal,
                            L.kmax, P, L.umass);
     return hipGetLastError();
 }

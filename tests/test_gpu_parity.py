@@ -179,6 +179,9 @@ def test_standalone_passes(gpu_ctx):
 
 
 def test_order_and_list_capacity_do_not_change_results(gpu_ctx):
+    """The list-overflow fallback walks the same partners in the same slot
+    order (bit for bit); the identity order sums in another order (equal to
+    FP32 rounding of the compensated sums)."""
     h, prm = solid_case(3000, seed=41)
     base = gpu_run(gpu_ctx, h, prm)
     try:
@@ -191,8 +194,14 @@ def test_order_and_list_capacity_do_not_change_results(gpu_ctx):
         gpu_ctx.set_option("reorder_interval", 1)
         gpu_ctx.set_option("max_neighbours", 160)
     for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
-        assert np.array_equal(base[f], ident[f]), f
         assert np.array_equal(base[f], small[f]), f
+    mx, nr = vec_errors(ident["acc"], base["acc"])
+    assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
+    for f in ("density", "balsara", "maxvsig"):
+        mx, nr = scalar_errors(ident[f], base[f])
+        assert nr <= 1e-6, (f, mx, nr)
+    mx, nr = tensor_errors(ident["dstress_dt"], base["dstress_dt"])
+    assert nr <= 1e-5, (mx, nr)
 
 
 # ------------------------------------------------------------------ integrator
