@@ -43,7 +43,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="collision", choices=["collision", "plummer", "liquid", "large"])
     ap.add_argument("--n", type=int, default=0, help="override the particle count")
     ap.add_argument("--no-cpu-baseline", action="store_true")

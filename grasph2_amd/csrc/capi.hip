@@ -436,7 +436,7 @@ int grasph_create(int device, grasph_ctx** out)
     // reference's __device__ currentTS / nextTS start
     if (hipMalloc((void**)&ctx->c.ts, 2 * sizeof(float)) != hipSuccess ||
         hipMemset(ctx->c.ts, 0, 2 * sizeof(float)) != hipSuccess ||
-        hipMalloc((void**)&ctx->c.umass, 2 * sizeof(float)) != hipSuccess) {
+        hipMalloc((void**)&ctx->c.umass, (2 + 256) * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         if (ctx->c.ts) (void)hipFree(ctx->c.ts);
         if (ctx->c.umass) (void)hipFree(ctx->c.umass);

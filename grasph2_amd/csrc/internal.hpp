@@ -41,7 +41,8 @@ struct Ctx {
 
     // variable-step leapfrog state {current dt, next dt}, zero at creation
     float* ts = nullptr;
-    // {uniform, mass} of the particles, written before each gravity sweep
+    // {uniform, mass} of the particles, written before each gravity sweep,
+    // then the per-block flags of the check (2 + 256 floats)
     float* umass = nullptr;
 
     // Morton order of the i side (slot t -> particle perm[t])
@@ -199,7 +200,8 @@ struct BatchSweepLaunch {
 };
 hipError_t launch_batch_sweep(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s);
 // umass[2k] = 1 when every particle of instance k (insts null: the n
-// particles of one instance) has the same mass, umass[2k + 1] = that mass
+// particles of one instance) has the same mass, umass[2k + 1] = that mass.
+// With insts null, umass must hold 2 + 256 floats (per-block flags follow).
 hipError_t launch_mass_check(const float4* posm, const BatchInst* insts, int n_inst, int n, float* umass,
                              hipStream_t s);
 hipError_t launch_reset_queue(int* queue, int n, hipStream_t s);

@@ -44,13 +44,30 @@ __global__ __launch_bounds__(kBlock) void bbox_partial(const float4* __restrict_
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void bbox_final(float* __restrict__ part, const int nparts)
+// one block of kBlock threads folds the nparts (<= kBlock) block boxes
+__global__ __launch_bounds__(kBlock) void bbox_final(float* __restrict__ part, const int nparts)
 {
-    const int d = threadIdx.x;
-    if (d >= 6) return;
-    float v = (d < 3) ? INFINITY : -INFINITY;
-    for (int b = 0; b < nparts; ++b) v = (d < 3) ? fminf(v, part[b * 6 + d]) : fmaxf(v, part[b * 6 + d]);
-    part[nparts * 6 + d] = v;
+    __shared__ float sh[6][kBlock / 64];
+    const int b = threadIdx.x, lane = b & 63, w = b >> 6;
+    float v[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) v[d] = (b < nparts) ? part[b * 6 + d] : ((d < 3) ? INFINITY : -INFINITY);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            const float o = __shfl_xor(v[d], off, 64);
+            v[d] = (d < 3) ? fminf(v[d], o) : fmaxf(v[d], o);
+        }
+    if (lane == 0)
+#pragma unroll
+        for (int d = 0; d < 6; ++d) sh[d][w] = v[d];
+    __syncthreads();
+    if (b < 6) {
+        float r = sh[b][0];
+        for (int q = 1; q < kBlock / 64; ++q) r = (b < 3) ? fminf(r, sh[b][q]) : fmaxf(r, sh[b][q]);
+        part[nparts * 6 + b] = r;
+    }
 }
 
 __device__ inline uint32_t spread10(uint32_t v)
@@ -193,7 +210,8 @@ hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(bbox_partial, dim3(kBoxBlocks), dim3(kBlock), 0, s, posm, n, c.bbox);
-    hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, c.bbox, kBoxBlocks);
+    static_assert(kBoxBlocks <= kBlock, "bbox_final folds at most one box per thread");
+    hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, s, c.bbox, kBoxBlocks);
     const int blocks = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(morton_keys, dim3(blocks), dim3(kBlock), 0, s, posm, n, c.bbox + kBoxBlocks * 6, c.keys,
                        c.vals);

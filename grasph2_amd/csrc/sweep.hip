@@ -26,6 +26,8 @@
 // The batched kernel runs the same code persistently over a queue of work
 // items (instance, 64 NI slots) ordered longest instance first; each
 // wavefront takes the next item with one atomic add.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace grasph {
@@ -34,6 +36,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kChunk = 8;    // j per prefetch chunk and per neighbour-candidate test
 constexpr int kFold = 128;   // j per plain partial sum before the compensated add
+constexpr int kMassBlocks = 256;  // blocks of the single-instance mass check (flags after umass[0..1])
 
 template <int NI>
 struct Acc {
@@ -331,6 +334,33 @@ __global__ __launch_bounds__(1024) void mass_check_kernel(const float4* __restri
     }
 }
 
+// The same check for one large instance: block b flags a mismatch in its
+// grid-stride share, then one block folds the flags (every flag is written
+// each time, so nothing needs clearing).
+__global__ __launch_bounds__(kBlock) void mass_partial_kernel(const float4* __restrict__ posm, const int n,
+                                                              int* __restrict__ partial)
+{
+    const float m0 = posm[0].w;
+    int bad = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        bad |= __float_as_int(posm[i].w) != __float_as_int(m0);
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bad;
+}
+
+__global__ __launch_bounds__(kBlock) void mass_final_kernel(const float4* __restrict__ posm,
+                                                            const int* __restrict__ partial, const int nparts,
+                                                            float* __restrict__ umass)
+{
+    int bad = 0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) bad |= partial[b];
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) {
+        umass[0] = bad ? 0.0f : 1.0f;
+        umass[1] = posm[0].w;
+    }
+}
+
 // Sum the slices' partial gravity in slice order (compensated) and finish
 // the acceleration as a one-slice sweep would.
 __global__ __launch_bounds__(kBlock) void combine_kernel(const float4* __restrict__ part, const int nslices,
@@ -476,6 +506,13 @@ hipError_t launch_mass_check(const float4* posm, const BatchInst* insts, int n_i
                              hipStream_t s)
 {
     if (n_inst <= 0) return hipSuccess;
+    if (!insts && n > 16384) {  // one large instance: spread the scan over blocks, then fold their flags
+        const int blocks = std::min(kMassBlocks, (n + 4095) / 4096);
+        int* partial = (int*)(umass + 2);
+        hipLaunchKernelGGL(mass_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, posm, n, partial);
+        hipLaunchKernelGGL(mass_final_kernel, dim3(1), dim3(kBlock), 0, s, posm, partial, blocks, umass);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(mass_check_kernel, dim3(n_inst), dim3(1024), 0, s, posm, insts, n, umass);
     return hipGetLastError();
 }
