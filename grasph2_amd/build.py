@@ -53,19 +53,25 @@ def build_library(verbose=True):
     return LIB
 
 
+CPP_EXAMPLES = ("api_roundtrip", "snapshot_roundtrip")
+
+
 def build_cpp_example(verbose=True):
-    """Compile tests/cpp/api_roundtrip.cpp, the C++ template API driven like
-    the reference's main loop (host code only: g++ and the HIP runtime)."""
-    src = os.path.join(ROOT, "tests", "cpp", "api_roundtrip.cpp")
-    exe = os.path.join(ROOT, "tests", "cpp", "api_roundtrip")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lgrasph2_amd", "-L/opt/rocm/lib", "-lamdhip64",
-           "-Wl,-rpath,$ORIGIN/../../grasph2_amd", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("C++ example build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
-    if verbose:
-        print("built", exe)
+    """Compile the C++ programs under tests/cpp: api_roundtrip (the C++
+    template API driven like the reference's main loop) and
+    snapshot_roundtrip (snapshot files, host only).  Host code: g++ and the
+    HIP runtime."""
+    for name in CPP_EXAMPLES:
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+               "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lgrasph2_amd", "-L/opt/rocm/lib",
+               "-lamdhip64", "-pthread", "-Wl,-rpath,$ORIGIN/../../grasph2_amd", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("C++ example build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+        if verbose:
+            print("built", exe)
 
 
 def build_oracle(verbose=True):
