@@ -112,14 +112,16 @@ int ensure_nbr(Ctx& c, int64_t n, int slices)
     return 0;
 }
 
-// j-split of the single-instance sweep: enough slices for ~4 wavefronts per
-// SIMD, each slice at least 1024 j; slice length a multiple of 8.
+// j-split of the single-instance sweep: enough slices for ~16 wavefronts
+// per SIMD (measured: the sweep keeps gaining up to there, as more resident
+// waves hide the scalar-load and rare-path latency), each slice at least
+// 1024 j; slice length a multiple of 8.
 void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len)
 {
     int S = c.sweep_slices;
     if (S <= 0) {
         const int64_t waves = (n + 64 * ni - 1) / (64 * ni);
-        const int64_t target = 4 * 4 * (int64_t)c.cus;
+        const int64_t target = 16 * 4 * (int64_t)c.cus;
         S = (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + waves - 1) / waves));
         S = (int)std::max<int64_t>(1, std::min<int64_t>(S, n / 1024));
     }
@@ -315,6 +317,13 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         L.umass = c.umass;
     }
     if ((r = run_sweep(c, L, P, s))) return r;
+    if (need_list && slices > 1) {
+        // one list per particle for the passes (walking per-slice lists costs them up to 2x)
+        ProfScope ps(&c, s, KC_OTHER);
+        hipError_t e = launch_merge_lists(c.nbr, c.nbr_cnt, (int)n, c.max_neighbours, slices, s);
+        if (e != hipSuccess) return hip_fail(c, e, "list merge launch");
+        c.last_list_slices = 1;
+    }
 
     NbrLaunch N = nbr_base(c, p);
     N.perm = perm;
@@ -322,7 +331,7 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     N.vel = vs;
     N.density = ds;
     N.density_slots = perm ? c.ds : nullptr;
-    N.nslices = slices;
+    N.nslices = 1;  // merged above
     N.slice_len = slice_len;
     if (dens_pass) {
         ProfScope ps(&c, s, KC_DENSITY);

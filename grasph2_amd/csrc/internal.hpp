@@ -132,6 +132,9 @@ struct SweepLaunch {
     const float* umass; // {uniform, mass} from launch_mass_check, or null
 };
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s);
+// Fold the per-slice lists of a j-split sweep into one list per particle
+// (slice 0's arrays; a count above kmax means "scan the instance").
+hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s);
 
 // One instance of a batch, as the device kernels see it.
 struct BatchInst {

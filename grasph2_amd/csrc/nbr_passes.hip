@@ -79,8 +79,9 @@ __device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ po
         if (cnt <= kmax) {
             for (int k = 0; k < cnt; ++k) body(list[(size_t)k * nstride + t]);
         } else {
-            const int a = m.off + s * Ls.slice_len;
-            const int b = min(m.off + m.n, a + Ls.slice_len);
+            const int64_t a64 = (int64_t)m.off + (int64_t)s * Ls.slice_len;
+            const int a = (int)a64;
+            const int b = (int)min((int64_t)m.off + m.n, a64 + Ls.slice_len);
             for (int j = a; j < b; ++j) {
                 const float4 pj = posm[j];
                 if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);

@@ -334,6 +334,29 @@ __global__ __launch_bounds__(1024) void mass_check_kernel(const float4* __restri
     }
 }
 
+// Concatenate the per-slice neighbour lists of a j-split sweep into slice
+// 0's list, in slice order (so slot indices stay increasing), and leave the
+// total in slice 0's count.  A total above kmax (some slice overflowed, or
+// the slices together do) marks the particle for the exact-test scan of its
+// whole instance.  Each thread moves only its own slot's column.
+__global__ __launch_bounds__(kBlock) void merge_lists_kernel(int* __restrict__ nbr, int* __restrict__ cnt,
+                                                             const int n, const int kmax, const int nslices)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const size_t ns = (size_t)n;
+    int total = cnt[t];
+    for (int s = 1; s < nslices; ++s) {
+        const int c = cnt[(size_t)s * ns + t];
+        if (total + c <= kmax) {
+            const int* src = nbr + (size_t)s * kmax * ns;
+            for (int k = 0; k < c; ++k) nbr[(size_t)(total + k) * ns + t] = src[(size_t)k * ns + t];
+        }
+        total += c;  // once above kmax it stays above: the list is then ignored
+    }
+    cnt[t] = total;
+}
+
 // The same check for one large instance: block b flags a mismatch in its
 // grid-stride share, then one block folds the flags (every flag is written
 // each time, so nothing needs clearing).
@@ -514,6 +537,14 @@ hipError_t launch_mass_check(const float4* posm, const BatchInst* insts, int n_i
         return hipGetLastError();
     }
     hipLaunchKernelGGL(mass_check_kernel, dim3(n_inst), dim3(1024), 0, s, posm, insts, n, umass);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s)
+{
+    if (n <= 0 || nslices <= 1) return hipSuccess;
+    hipLaunchKernelGGL(merge_lists_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbr, cnt, n, kmax,
+                       nslices);
     return hipGetLastError();
 }
 
