@@ -47,10 +47,8 @@ struct BatchState {
     int* d_queue = nullptr;
     float* d_ts = nullptr;  // variable step: {current dt, next dt} per instance, zero when the tables are built
     float* d_um = nullptr;  // {uniform, mass} per instance, written before each gravity sweep
-    // slot-ordered copies of positions, velocities and densities (SPH with a Morton order)
+    // slot-ordered positions for the sweep (SPH with a Morton order)
     float4* d_xs = nullptr;
-    float4* d_vs = nullptr;
-    float* d_ds = nullptr;
 
     // Morton order
     float* d_bbox = nullptr;
@@ -70,8 +68,8 @@ struct BatchState {
     int* d_cnt = nullptr;
     int64_t nbr_cap = 0;  // slots * kmax allocated
     int kmax = 0;
-    float4* d_jrec = nullptr;
-    int64_t jrec_cap = 0;
+    SlotRec* d_rec = nullptr;  // slot records of the passes
+    int64_t rec_cap = 0;
 
     // graph of one step
     hipStream_t gs = nullptr;
@@ -105,12 +103,12 @@ void drop_graph(BatchState& b)
 void free_tables(BatchState& b)
 {
     void* ps[] = {b.d_insts, b.d_prm, b.d_inst_of, b.d_items, b.d_blk, b.d_queue, b.d_bbox, b.d_keys, b.d_keys_alt,
-                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um, b.d_xs, b.d_vs, b.d_ds};
+                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um, b.d_xs};
     for (void* p : ps) free_ptr(p);
     b.d_insts = nullptr; b.d_prm = nullptr; b.d_inst_of = nullptr; b.d_items = nullptr; b.d_blk = nullptr;
     b.d_queue = nullptr; b.d_bbox = nullptr; b.d_keys = nullptr; b.d_keys_alt = nullptr; b.d_vals = nullptr;
     b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr; b.d_um = nullptr;
-    b.d_xs = nullptr; b.d_vs = nullptr; b.d_ds = nullptr;
+    b.d_xs = nullptr;
     b.sort_bytes = 0;
     b.order_valid = false;
     b.n_items = 0;
@@ -224,9 +222,7 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
             hipMalloc((void**)&b.d_keys_alt, nt * sizeof(uint64_t)) != hipSuccess ||
             hipMalloc((void**)&b.d_vals, nt * sizeof(int)) != hipSuccess ||
             hipMalloc((void**)&b.d_perm, nt * sizeof(int)) != hipSuccess ||
-            hipMalloc((void**)&b.d_xs, nt * sizeof(float4)) != hipSuccess ||
-            hipMalloc((void**)&b.d_vs, nt * sizeof(float4)) != hipSuccess ||
-            hipMalloc((void**)&b.d_ds, nt * sizeof(float)) != hipSuccess) {
+            hipMalloc((void**)&b.d_xs, nt * sizeof(float4)) != hipSuccess) {
             (void)hipGetLastError();
             return fail(c, GRASPH_E_NOMEM, "device allocation of the batch order failed");
         }
@@ -256,11 +252,11 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
         b.nbr_cap = need;
         b.kmax = c.max_neighbours;
     }
-    if (b.jrec_cap < 2 * b.ntotal || !b.d_jrec) {
+    if (b.rec_cap < b.ntotal || !b.d_rec) {
         drop_graph(b);
-        int64_t cap = b.jrec_cap;
-        if ((r = grow(c, b.d_jrec, cap, 2 * b.ntotal, "batch pass records"))) return r;
-        b.jrec_cap = cap;
+        int64_t cap = b.rec_cap;
+        if ((r = grow(c, b.d_rec, cap, b.ntotal, "batch slot records"))) return r;
+        b.rec_cap = cap;
     }
     if (!b.gs) {
         if (hipStreamCreateWithFlags(&b.gs, hipStreamNonBlocking) != hipSuccess ||
@@ -329,16 +325,16 @@ const int* slot_order(const Ctx& c, const BatchState& b)
     return ((b.flags & GRASPH_SPH) && b.order_valid) ? order_of(c, b) : nullptr;
 }
 
-// slot-ordered copies (positions; velocities and, when no density pass
-// writes them, densities) for the current order
+// slot-ordered positions (with a slot order) and the passes' slot records
+// (positions, velocities and, when no density pass writes them, densities)
 int batch_gather(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
 {
+    if (!(b.flags & GRASPH_SPH)) return 0;
     const int* perm = slot_order(c, b);
-    if (!perm) return 0;
     const bool dens_pass = !(b.flags & GRASPH_INTEGRATE_DENSITY);
     ProfScope ps(&c, s, KC_REORDER);
     hipError_t e = launch_gather(perm, (int)b.ntotal, (const float4*)p->posm, (const float4*)p->vel,
-                                 dens_pass ? nullptr : p->density, b.d_xs, b.d_vs, b.d_ds, s);
+                                 dens_pass ? nullptr : p->density, perm ? b.d_xs : nullptr, b.d_rec, s);
     if (e != hipSuccess) return hip_fail(c, e, "batched slot-order gather launch");
     return 0;
 }
@@ -397,10 +393,9 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     const int* perm = slot_order(c, b);
     NbrLaunch N;
     std::memset(&N, 0, sizeof N);
-    N.posm = perm ? b.d_xs : (const float4*)p->posm;
-    N.vel = perm ? b.d_vs : (const float4*)p->vel;
-    N.density = perm ? b.d_ds : p->density;
-    N.density_slots = perm ? b.d_ds : nullptr;
+    N.posm = (const float4*)p->posm;
+    N.vel = (const float4*)p->vel;
+    N.rec = b.d_rec;
     N.dstress = p->dstress;
     N.balsara_in = p->balsara;
     N.perm = perm;
@@ -411,7 +406,6 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     N.balsara = p->balsara;
     N.density_dt = p->density_dt;
     N.dstress_dt = p->dstress_dt;
-    N.jrec = b.d_jrec;
     N.acc = (float4*)p->acc;
     N.xvel = (float4*)p->xvel;
     N.maxvsig = p->maxvsig;
@@ -494,7 +488,7 @@ std::string graph_key(const Ctx& c, const BatchState& b, const grasph_particles*
     k << p->posm << ' ' << p->pos64 << ' ' << p->vel << ' ' << p->acc << ' ' << p->xvel << ' ' << p->density << ' '
       << p->density_dt << ' ' << p->balsara << ' ' << p->maxvsig << ' ' << p->dstress << ' ' << p->dstress_dt << ' '
       << (c.reorder_interval > 0) << ' ' << c.batch_wpb << ' ' << c.batch_bpc << ' ' << (const void*)b.d_nbr << ' '
-      << (const void*)b.d_jrec << ' ' << (const void*)b.d_items << ' ' << (const void*)b.d_xs << ' '
+      << (const void*)b.d_rec << ' ' << (const void*)b.d_items << ' ' << (const void*)b.d_xs << ' '
       << c.timestep_reset;
     return k.str();
 }
@@ -548,7 +542,7 @@ void destroy_batch_state(BatchState* b)
     free_tables(*b);
     free_ptr(b->d_nbr);
     free_ptr(b->d_cnt);
-    free_ptr(b->d_jrec);
+    free_ptr(b->d_rec);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     if (b->gs) (void)hipStreamDestroy(b->gs);

@@ -158,7 +158,7 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
     std::memset(&L, 0, sizeof L);
     L.posm = (const float4*)p->posm;
     L.vel = (const float4*)p->vel;
-    L.density = p->density;
+    L.rec = c.rec;
     L.dstress = p->dstress;
     L.balsara_in = p->balsara;
     L.perm = order_of(c);
@@ -169,7 +169,6 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
     L.balsara = p->balsara;
     L.density_dt = p->density_dt;
     L.dstress_dt = p->dstress_dt;
-    L.jrec = c.jrec;
     L.acc = (float4*)p->acc;
     L.xvel = (float4*)p->xvel;
     L.maxvsig = p->maxvsig;
@@ -270,32 +269,22 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     if (slices > 1 && sweep_grav) {
         if ((r = grow(c, c.part, c.part_cap, 2 * n * slices, "sweep partial sums"))) return r;
     }
-    if ((accel_pass && sph) || strain_pass) {
-        if ((r = grow(c, c.jrec, c.jrec_cap, 2 * n, "pass record"))) return r;
-    }
     const DevParams P = make_dev_params(prm);
 
-    // slot-ordered copies of what the sweep and the passes read
+    // slot-ordered positions for the sweep and the passes' slot records
     const int* perm = need_list ? order_of(c) : nullptr;
-    const bool list_reads_vel = strain_pass || (accel_pass && sph);
-    const bool list_reads_density = list_reads_vel;
     const float4* xs = (const float4*)p->posm;
-    const float4* vs = (const float4*)p->vel;
-    const float* ds = p->density;
-    if (perm) {
-        const bool gvel = list_reads_vel && p->vel;
-        const bool gdens = list_reads_density && p->density && !dens_pass;  // else the density pass writes them
-        if ((r = grow(c, c.xs, c.xs_cap, n, "slot-ordered positions")) ||
-            (gvel && (r = grow(c, c.vs, c.vs_cap, n, "slot-ordered velocities"))) ||
-            ((list_reads_density || dens_pass) && (r = grow(c, c.ds, c.ds_cap, n, "slot-ordered densities"))))
+    if (need_list) {
+        const bool reads_vel = strain_pass || (accel_pass && sph);
+        const bool gdens = reads_vel && !dens_pass;  // else the density pass fills the records' densities
+        if ((perm && (r = grow(c, c.xs, c.xs_cap, n, "slot-ordered positions"))) ||
+            (r = grow(c, c.rec, c.rec_cap, n, "slot records")))
             return r;
         ProfScope ps(&c, s, KC_REORDER);
-        hipError_t e = launch_gather(perm, (int)n, (const float4*)p->posm, gvel ? (const float4*)p->vel : nullptr,
-                                     gdens ? p->density : nullptr, c.xs, c.vs, c.ds, s);
+        hipError_t e = launch_gather(perm, (int)n, (const float4*)p->posm, reads_vel ? (const float4*)p->vel : nullptr,
+                                     gdens ? p->density : nullptr, perm ? c.xs : nullptr, c.rec, s);
         if (e != hipSuccess) return hip_fail(c, e, "slot-order gather launch");
-        xs = c.xs;
-        vs = gvel ? c.vs : nullptr;
-        ds = c.ds;
+        if (perm) xs = c.xs;
     }
 
     // one all-pairs sweep
@@ -327,12 +316,7 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
 
     NbrLaunch N = nbr_base(c, p);
     N.perm = perm;
-    N.posm = xs;
-    N.vel = vs;
-    N.density = ds;
-    N.density_slots = perm ? c.ds : nullptr;
-    N.nslices = 1;  // merged above
-    N.slice_len = slice_len;
+    N.rec = c.rec;
     if (dens_pass) {
         ProfScope ps(&c, s, KC_DENSITY);
         hipError_t e = launch_density(N, p->density, P, s);
@@ -471,8 +455,8 @@ int grasph_destroy(grasph_ctx* ctx)
     grasph::destroy_batch_state(c.batch);
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
-                    c.nbr,  c.nbr_cnt, c.jrec,    c.pot,  c.red,      c.part, c.ts, c.umass,
-                    c.xs,   c.vs,      c.ds};
+                    c.nbr,  c.nbr_cnt, c.rec,     c.pot,  c.red,      c.part, c.ts, c.umass,
+                    c.xs};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {

@@ -68,16 +68,11 @@ struct Ctx {
     float4* part = nullptr;   // partial sums of a j-split sweep
     int64_t part_cap = 0;
 
-    // pass-C per-particle record: sigma/rho^2 (6), balsara, density (slot order)
-    float4* jrec = nullptr;
-    int64_t jrec_cap = 0;
-    // slot-ordered copies of positions, velocities and densities
+    // slot-ordered positions for the sweep, and the slot records of the passes
     float4* xs = nullptr;
     int64_t xs_cap = 0;
-    float4* vs = nullptr;
-    int64_t vs_cap = 0;
-    float* ds = nullptr;
-    int64_t ds_cap = 0;
+    struct SlotRec* rec = nullptr;
+    int64_t rec_cap = 0;
 
     // potential and reduction partials
     float* pot = nullptr;
@@ -135,6 +130,13 @@ hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 // Fold the per-slice lists of a j-split sweep into one list per particle
 // (slice 0's arrays; a count above kmax means "scan the instance").
 hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s);
+
+// Per-slot record of the neighbour passes (64 bytes, slot order): position
+// and mass; velocity and density; the acceleration pass's j-side terms
+// sigma/rho^2 (xx yy zz xy | xz yz), balsara, density.
+struct SlotRec {
+    float4 pm, vr, ra, rb;
+};
 
 // One instance of a batch, as the device kernels see it.
 struct BatchInst {
@@ -229,15 +231,15 @@ struct BatchOrder {
 size_t batch_order_tmp_bytes(int ntotal, int key_bits);
 hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s);
 
-// Neighbour passes.  posm, vel and density are read in slot order (slot t
-// is particle perm[t]; with perm null the caller's arrays themselves), the
-// per-particle outputs go to particle perm[t] of the caller's arrays, and
-// the lists hold slot indices.
+// Neighbour passes.  Slot t is particle perm[t] (perm null: t); inputs of
+// the lists' partners come from the slot records, per-particle outputs go
+// to particle perm[t] of the caller's arrays, and the lists hold slot
+// indices (one list per particle).  posm / vel (index order) serve the
+// finish kernel of configurations without SPH.
 struct NbrLaunch {
     const float4* posm;
     const float4* vel;
-    const float* density;
-    float* density_slots;  // density pass: slot-ordered copy of its output (null when perm is null)
+    SlotRec* rec;
     const float* dstress;
     const float* balsara_in;
     const int* perm;
@@ -248,14 +250,11 @@ struct NbrLaunch {
     float* balsara;
     float* density_dt;
     float* dstress_dt;
-    float4* jrec;
     float4* acc;
     float4* xvel;
     float* maxvsig;
     bool acc_has_gravity;
     BlockTables T;  // batch tables (T.nblocks == 0 for one instance)
-    int nslices;    // lists per slice of the j-split sweep (0 or 1: one list)
-    int slice_len;
 };
 hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P, hipStream_t s);
 hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s);
@@ -311,10 +310,11 @@ hipError_t launch_invariants(const InvLaunch& L, const grasph_params& prm, hipSt
 // Z-order curve over the bounding box of posm.
 size_t reorder_tmp_bytes(int n);
 hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s);
-// Slot-ordered copies for the sweep and the list passes: xs[t] = posm[perm[t]],
-// and vs / ds from vel / density when those are given (null: skipped).
+// Slot-ordered copies for the sweep and the list passes (perm null: the
+// identity): xs[t] = posm[perm[t]] when xs is given, and the slot records'
+// position/mass and velocity/density (vel or density null: zeros there).
 hipError_t launch_gather(const int* perm, int n, const float4* posm, const float4* vel, const float* density,
-                         float4* xs, float4* vs, float* ds, hipStream_t s);
+                         float4* xs, SlotRec* rec, hipStream_t s);
 
 // environmentAcceleration and the zero XVEL / MAXVSIG stores of
 // calcAcceleration for configurations without SPH (batched form).

@@ -11,15 +11,16 @@
 //          environmentAcceleration (:300-312), added to the gravity the
 //          sweep left in acc with compensated (TwoSum) accumulation.
 //
-// One particle per lane in Morton order (slot t -> i = perm[t]).  Inputs
-// are read through slot-ordered copies (positions, velocities, densities,
-// and the record, which is written in slot order), so a neighbour's data
-// sits near its neighbours' in memory; outputs go to particle i of the
-// caller's arrays.  The list is stored slot-major, nbr[k * nstride + t], so
-// a wavefront's loads of its k-th neighbours are contiguous, and holds slot
-// indices in increasing order.  A particle whose count exceeds the list
-// capacity falls back to a scan over its instance's slots with the exact
-// neighbour test, so a full list never loses partners.
+// One particle per lane in Morton order (slot t -> i = perm[t]).  Every
+// input a pass reads for a neighbour sits in one 64-byte slot record,
+// rec[j] = {x y z m, vx vy vz rho, sigma/rho^2 (6) balsara rho}, in slot
+// order: one neighbour costs one contiguous gather, and a particle's
+// neighbours' records are close together in memory.  Outputs go to
+// particle i of the caller's arrays.  The list is stored slot-major,
+// nbr[k * nstride + t], so a wavefront's loads of its k-th neighbours are
+// contiguous, and holds slot indices in increasing order.  A particle whose
+// count exceeds the list capacity falls back to a scan over its instance's
+// slots with the exact neighbour test, so a full list never loses partners.
 //
 // Every kernel serves one instance (SlotMap from the kernel arguments) or a
 // batch (SlotMap and parameters of the block's instance from tables).
@@ -57,45 +58,34 @@ __device__ inline float dsdt_entry(const Tens& ed, const Tens& rd, const Tens& S
     return o;
 }
 
-// Lists of a j-split sweep: one list per slice of slice_len j, walked in
-// slice order (so j stays increasing).
 struct Lists {
     const int* nbr;
     const int* cnt;
-    int nslices;
-    int slice_len;
 };
 
-// Walk particle i's neighbours: the lists, or for a slice whose list
-// overflowed, a scan of that slice's j with the exact test.
+// Walk slot t's neighbours: the list, or when it overflowed, a scan of the
+// instance's slots with the exact test (same partners, same order).
 template <typename F>
-__device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ posm, const Lists& Ls,
+__device__ __forceinline__ void for_each_neighbour(const SlotRec* __restrict__ rec, const Lists& Ls,
                                                    const size_t nstride, const int t, const int kmax,
                                                    const float4 pi, const SlotMap& m, const float H2, F&& body)
 {
-    for (int s = 0; s < Ls.nslices; ++s) {
-        const int* list = Ls.nbr + (size_t)s * kmax * nstride;
-        const int cnt = Ls.cnt[(size_t)s * nstride + t];
-        if (cnt <= kmax) {
-            for (int k = 0; k < cnt; ++k) body(list[(size_t)k * nstride + t]);
-        } else {
-            const int64_t a64 = (int64_t)m.off + (int64_t)s * Ls.slice_len;
-            const int a = (int)a64;
-            const int b = (int)min((int64_t)m.off + m.n, a64 + Ls.slice_len);
-            for (int j = a; j < b; ++j) {
-                const float4 pj = posm[j];
-                if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
-            }
+    const int cnt = Ls.cnt[t];
+    if (cnt <= kmax) {
+        for (int k = 0; k < cnt; ++k) body(Ls.nbr[(size_t)k * nstride + t]);
+    } else {
+        for (int j = m.off; j < m.off + m.n; ++j) {
+            const float4 pj = rec[j].pm;
+            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
         }
     }
 }
 
 template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ posm, const int* __restrict__ perm,
+__global__ __launch_bounds__(kBlock) void density_kernel(SlotRec* __restrict__ rec, const int* __restrict__ perm,
                                                          const Lists Ls,
                                                          const size_t nstride, const int kmax,
-                                                         float* __restrict__ density,
-                                                         float* __restrict__ density_slots, const BlockTables T,
+                                                         float* __restrict__ density, const BlockTables T,
                                                          const DevParams P0)
 {
     const SlotMap m = slot_map<BATCH>(T, (int)nstride);
@@ -103,50 +93,49 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = posm[t];
+    const float4 pi = rec[t].pm;
     CSum rho{0.0f, 0.0f};
-    for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const float4 pj = posm[j];
+    for_each_neighbour(rec, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
+        const float4 pj = rec[j].pm;
         const float r = sqrtf(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
         rho.add(pj.w * w_spline(r, P.H, P.Wpref));
     });
     float r = rho.value();
     if ((P.flags & GRASPH_DEAL_WITH_NO_PARTNERS) && r < 0.001f) r = P.rho0 * 0.01f;  // :85-88
     density[i] = r;
-    if (density_slots) density_slots[t] = r;
+    rec[t].vr.w = r;  // the strain / record passes read densities from the slot records
 }
 
 template <bool BATCH, bool SOLIDS, bool BALS, bool INTD>
 __global__ __launch_bounds__(kBlock) void strain_kernel(
-    const float4* __restrict__ posm, const float4* __restrict__ vel, const float* __restrict__ density,
-    const float* __restrict__ dstress, const int* __restrict__ perm, const Lists Ls, const size_t nstride, const int kmax, float* __restrict__ balsara,
-    float* __restrict__ density_dt, float* __restrict__ dstress_dt, float4* __restrict__ jrec, const BlockTables T,
-    const DevParams P0)
+    SlotRec* __restrict__ rec, const float* __restrict__ dstress, const int* __restrict__ perm, const Lists Ls,
+    const size_t nstride, const int kmax, float* __restrict__ balsara, float* __restrict__ density_dt,
+    float* __restrict__ dstress_dt, const BlockTables T, const DevParams P0)
 {
     const SlotMap m = slot_map<BATCH>(T, (int)nstride);
     const int t = m.slot0 + threadIdx.x;
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = posm[t];
-    const float4 vi = vel[t];
-    const float rhoi = density[t];
+    const float4 pi = rec[t].pm;
+    const float4 vi = rec[t].vr;
+    const float rhoi = vi.w;
 
     float exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;  // strain rate (symmetric)
     float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 
-    for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const float4 pj = posm[j];
+    for_each_neighbour(rec, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
+        const float4 pj = rec[j].pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
         const float r = sqrtf(r2);
         const float dw = dw_spline(r, P.H, P.dWpref);
         const float s = dw / r;
         const float gx = s * rx, gy = s * ry, gz = s * rz;
-        const float4 vj = vel[j];
+        const float4 vj = rec[j].vr;
         const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-        const float mj = pj.w, rhoj = density[j];
+        const float mj = pj.w, rhoj = vj.w;
         if (SOLIDS) {  // models.h:225-248
             const float tmp = -0.5f * mj / rhoj;
             exx += tmp * (vx * gx + vx * gx);
@@ -204,49 +193,45 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     if (SOLIDS) {
         const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
         const float d2 = rhoi * rhoi;
-        jrec[2 * (size_t)t] = make_float4((S.m[0][0] - pres) / d2, (S.m[1][1] - pres) / d2, (S.m[2][2] - pres) / d2,
-                                          S.m[0][1] / d2);
-        jrec[2 * (size_t)t + 1] = make_float4(S.m[0][2] / d2, S.m[1][2] / d2, bal, rhoi);
+        rec[t].ra = make_float4((S.m[0][0] - pres) / d2, (S.m[1][1] - pres) / d2, (S.m[2][2] - pres) / d2,
+                                S.m[0][1] / d2);
+        rec[t].rb = make_float4(S.m[0][2] / d2, S.m[1][2] / d2, bal, rhoi);
     } else {
         const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
-        jrec[2 * (size_t)t] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
-        jrec[2 * (size_t)t + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
+        rec[t].ra = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
+        rec[t].rb = make_float4(0.0f, 0.0f, bal, rhoi);
     }
 }
 
 // record only (no strain pass in this configuration)
 template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void prep_kernel(const float* __restrict__ density,
-                                                      const float* __restrict__ dstress,
+__global__ __launch_bounds__(kBlock) void prep_kernel(SlotRec* __restrict__ rec, const float* __restrict__ dstress,
                                                       const float* __restrict__ balsara, const int* __restrict__ perm,
-                                                      const size_t nstride, float4* __restrict__ jrec,
-                                                      const BlockTables T, const DevParams P0)
+                                                      const size_t nstride, const BlockTables T, const DevParams P0)
 {
     const SlotMap m = slot_map<BATCH>(T, (int)nstride);
     const int t = m.slot0 + threadIdx.x;
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float rhoi = density[t];
+    const float rhoi = rec[t].vr.w;
     const float bal = balsara ? balsara[i] : 0.0f;
     if (P.flags & GRASPH_SOLIDS) {
         const float pres = eos_murnaghan(rhoi, P.rho0, P.BULK, P.dBULKdP);
         const float d2 = rhoi * rhoi;
-        jrec[2 * (size_t)t] =
-            make_float4((dstress[m.plane6(0, i)] - pres) / d2, (dstress[m.plane6(1, i)] - pres) / d2,
-                        (dstress[m.plane6(2, i)] - pres) / d2, dstress[m.plane6(3, i)] / d2);
-        jrec[2 * (size_t)t + 1] = make_float4(dstress[m.plane6(4, i)] / d2, dstress[m.plane6(5, i)] / d2, bal, rhoi);
+        rec[t].ra = make_float4((dstress[m.plane6(0, i)] - pres) / d2, (dstress[m.plane6(1, i)] - pres) / d2,
+                                (dstress[m.plane6(2, i)] - pres) / d2, dstress[m.plane6(3, i)] / d2);
+        rec[t].rb = make_float4(dstress[m.plane6(4, i)] / d2, dstress[m.plane6(5, i)] / d2, bal, rhoi);
     } else {
         const float sig = -eos_liquid(rhoi, P.rho0, P.c2);
-        jrec[2 * (size_t)t] = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
-        jrec[2 * (size_t)t + 1] = make_float4(0.0f, 0.0f, bal, rhoi);
+        rec[t].ra = make_float4(sig / (rhoi * rhoi), 0.0f, 0.0f, 0.0f);
+        rec[t].rb = make_float4(0.0f, 0.0f, bal, rhoi);
     }
 }
 
 template <bool BATCH, bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
 __global__ __launch_bounds__(kBlock) void accel_kernel(
-    const float4* __restrict__ posm, const float4* __restrict__ vel, const float4* __restrict__ jrec,
-    const int* __restrict__ perm, const Lists Ls, const size_t nstride,
+    const SlotRec* __restrict__ rec, const int* __restrict__ perm, const Lists Ls, const size_t nstride,
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
     const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
@@ -255,9 +240,9 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = posm[t];
-    const float4 vi = vel[t];
-    const float4 ra = jrec[2 * (size_t)t], rb = jrec[2 * (size_t)t + 1];
+    const float4 pi = rec[t].pm;
+    const float4 vi = rec[t].vr;
+    const float4 ra = rec[t].ra, rb = rec[t].rb;
     const float rhoi = rb.w, bi = BALS ? rb.z : 1.0f;
     // sigma_i / rho_i^2 and, for liquids with artificial pressure, arts_i
     const float sxx = ra.x, syy = ra.y, szz = ra.z, sxy = ra.w, sxz = rb.x, syz = rb.y;
@@ -271,15 +256,16 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    for_each_neighbour(posm, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const float4 pj = posm[j];
+    for_each_neighbour(rec, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
+        const SlotRec q = rec[j];
+        const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
         const float r = sqrtf(r2);
         const float dw = dw_spline(r, P.H, P.dWpref);
         const float s = dw / r;
         const float gx = s * rx, gy = s * ry, gz = s * rz;
-        const float4 qa = jrec[2 * (size_t)j], qb = jrec[2 * (size_t)j + 1];
+        const float4 qa = q.ra, qb = q.rb;
         const float rhoj = qb.w;
         const float mj = pj.w;
         float fx, fy, fz;
@@ -300,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
             fy = mj * (st * gy);
             fz = mj * (st * gz);
         }
-        const float4 vj = vel[j];
+        const float4 vj = q.vr;
         const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
         if (AV) {  // :277-289
             const float II = artificial_viscosity(BALS ? &mv : nullptr, P.alpha, rhoi, rhoj, vx, vy, vz, rx, ry, rz, r,
@@ -353,10 +339,7 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const float4* __restrict
     if (maxvsig) maxvsig[i] = 0.0f;
 }
 
-inline Lists lists_of(const NbrLaunch& L)
-{
-    return Lists{L.nbr, L.nbr_cnt, L.nslices > 1 ? L.nslices : 1, L.nslices > 1 ? L.slice_len : (1 << 30)};
-}
+inline Lists lists_of(const NbrLaunch& L) { return Lists{L.nbr, L.nbr_cnt}; }
 
 inline dim3 grid_of(const NbrLaunch& L)
 {
@@ -369,9 +352,8 @@ hipError_t strain_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
     const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
                in = P.flags & GRASPH_INTEGRATE_DENSITY;
 #define GRASPH_STRAIN(A, B, C)                                                                                     \
-    hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.density,   \
-                       L.dstress, L.perm, lists_of(L), (size_t)L.n, L.kmax, L.balsara, L.density_dt,           \
-                       L.dstress_dt, L.jrec, L.T, P)
+    hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.perm,  \
+                       lists_of(L), (size_t)L.n, L.kmax, L.balsara, L.density_dt, L.dstress_dt, L.T, P)
     const int key = (so ? 4 : 0) | (ba ? 2 : 0) | (in ? 1 : 0);
     switch (key) {
         case 0: GRASPH_STRAIN(false, false, false); break;
@@ -394,9 +376,9 @@ hipError_t accel_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
                av = P.flags & GRASPH_ARTIFICIAL_VISCOSITY, xs = P.flags & GRASPH_XSPH,
                ar = P.flags & GRASPH_ARTIFICIAL_STRESS;
 #define GRASPH_ACCEL(A, B, C, D, E)                                                                                  \
-    hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.jrec,  \
-                       L.perm, lists_of(L), (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig,                      \
-                       L.acc_has_gravity ? 1 : 0, L.T, P)
+    hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.rec, L.perm,         \
+                       lists_of(L), (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig, L.acc_has_gravity ? 1 : 0,  \
+                       L.T, P)
     // SOLIDS + ARTIFICIAL_STRESS is rejected before launch (the reference does not compile it).
     const int key = (so ? 16 : 0) | (ba ? 8 : 0) | (av ? 4 : 0) | (xs ? 2 : 0) | (ar ? 1 : 0);
     switch (key) {
@@ -436,11 +418,11 @@ hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, lists_of(L),
-                           (size_t)L.n, L.kmax, density, L.density_slots, L.T, P);
+        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.perm, lists_of(L),
+                           (size_t)L.n, L.kmax, density, L.T, P);
     else
-        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.perm, lists_of(L),
-                           (size_t)L.n, L.kmax, density, L.density_slots, L.T, P);
+        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.perm, lists_of(L),
+                           (size_t)L.n, L.kmax, density, L.T, P);
     return hipGetLastError();
 }
 
@@ -454,11 +436,11 @@ hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(prep_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.density, L.dstress, L.balsara_in,
-                           L.perm, (size_t)L.n, L.jrec, L.T, P);
+        hipLaunchKernelGGL(prep_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.balsara_in, L.perm,
+                           (size_t)L.n, L.T, P);
     else
-        hipLaunchKernelGGL(prep_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.density, L.dstress, L.balsara_in,
-                           L.perm, (size_t)L.n, L.jrec, L.T, P);
+        hipLaunchKernelGGL(prep_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.balsara_in,
+                           L.perm, (size_t)L.n, L.T, P);
     return hipGetLastError();
 }
 

@@ -151,29 +151,34 @@ __global__ __launch_bounds__(kBlock) void batch_keys(const float4* __restrict__ 
     vals[i] = i;
 }
 
-// slot-ordered copies: xs[t] = posm[perm[t]] and, where given, vs / ds from vel / density
+// slot-ordered copies: xs[t] = posm[perm[t]] (when xs is given) and the
+// slot records' {x y z m} and {vx vy vz rho}
 __global__ __launch_bounds__(kBlock) void gather_kernel(const int* __restrict__ perm, const int n,
                                                         const float4* __restrict__ posm, const float4* __restrict__ vel,
                                                         const float* __restrict__ density, float4* __restrict__ xs,
-                                                        float4* __restrict__ vs, float* __restrict__ ds)
+                                                        SlotRec* __restrict__ rec)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
-    const int i = perm[t];
-    xs[t] = posm[i];
-    if (vel) vs[t] = vel[i];
-    if (density) ds[t] = density[i];
+    const int i = perm ? perm[t] : t;
+    const float4 p = posm[i];
+    if (xs) xs[t] = p;
+    if (rec) {
+        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        rec[t].pm = p;
+        rec[t].vr = make_float4(v.x, v.y, v.z, density ? density[i] : 0.0f);
+    }
 }
 
 }  // namespace
 
 hipError_t launch_gather(const int* perm, int n, const float4* posm, const float4* vel, const float* density,
-                         float4* xs, float4* vs, float* ds, hipStream_t s)
+                         float4* xs, SlotRec* rec, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    if (!perm || !posm || !xs || (vel && !vs) || (density && !ds)) return hipErrorInvalidValue;
+    if (!posm || (xs && !perm)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(gather_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, perm, n, posm, vel,
-                       density, xs, vs, ds);
+                       density, xs, rec);
     return hipGetLastError();
 }
 
