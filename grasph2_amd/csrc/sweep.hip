@@ -17,8 +17,9 @@
 // (float4 x,y,z,m) is read with scalar loads into SGPRs -- no LDS, no
 // barriers -- in chunks of 8, two SGPR buffers in ping-pong so the next
 // chunk loads while this one computes.
-// Per pair: 3 sub, 3 FMA for r^2 + H^2, one v_rsq_f32, 3 mul, 3 FMA, and a
-// v_min towards one wave-uniform neighbour-candidate test per chunk; only
+// Per pair: 3 sub, 3 FMA for r^2 + H^2, v_rsq_f32 and v_rcp_f32, 1 mul (2
+// with unequal masses), 3 FMA, and a v_min towards one wave-uniform
+// neighbour-candidate test per chunk; only
 // chunks where some lane has a candidate enter the (rare) neighbour path.
 // Sums are plain FP32 over blocks of 128 j, then compensated (TwoSum) into
 // the running total.
@@ -116,13 +117,15 @@ __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, 
             float inv = 0.0f;
             if (GRAV || POT) inv = __builtin_amdgcn_rsqf(d2[q][c]);
             if (GRAV) {
+                // r^-3 = rsqrt(d2) * rcp(d2): the second factor from the transcendental
+                // unit, which runs beside the FMA pipe, instead of a VALU multiply
+                const float rcp = __builtin_amdgcn_rcpf(d2[q][c]);
                 float s;
                 if (UM) {
-                    s = inv * inv;
-                    s = s * inv;
+                    s = inv * rcp;
                 } else {
                     s = pj[c].w * inv;
-                    s = s * (inv * inv);
+                    s = s * rcp;
                 }
                 if (!SOFT) s = (d2[q][c] > 0.0f) ? s : 0.0f;  // r^2 > 0 (computeDerivatives.h:226)
                 a.ax[q] = fmaf(dx, s, a.ax[q]);
@@ -345,14 +348,20 @@ __global__ __launch_bounds__(kBlock) void merge_lists_kernel(int* __restrict__ n
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
     const size_t ns = (size_t)n;
-    int total = cnt[t];
+    // all counts first (independent loads), then the copies
+    constexpr int kMax = 16;
+    int c[kMax];
+#pragma unroll
+    for (int s = 0; s < kMax; ++s) c[s] = (s < nslices) ? cnt[(size_t)s * ns + t] : 0;
+    int total = c[0];
     for (int s = 1; s < nslices; ++s) {
-        const int c = cnt[(size_t)s * ns + t];
-        if (total + c <= kmax) {
+        const int cs = (s < kMax) ? c[s] : cnt[(size_t)s * ns + t];
+        if (total + cs <= kmax) {
             const int* src = nbr + (size_t)s * kmax * ns;
-            for (int k = 0; k < c; ++k) nbr[(size_t)(total + k) * ns + t] = src[(size_t)k * ns + t];
+            int* dst = nbr + (size_t)total * ns;
+            for (int k = 0; k < cs; ++k) dst[(size_t)k * ns + t] = src[(size_t)k * ns + t];
         }
-        total += c;  // once above kmax it stays above: the list is then ignored
+        total += cs;  // once above kmax it stays above: the list is then ignored
     }
     cnt[t] = total;
 }
