@@ -17,9 +17,10 @@
 // (float4 x,y,z,m) is read with scalar loads into SGPRs -- no LDS, no
 // barriers -- in chunks of 8, two SGPR buffers in ping-pong so the next
 // chunk loads while this one computes.
-// Per pair: 3 sub, 3 FMA for r^2 + H^2, v_rsq_f32 and v_rcp_f32, 1 mul (2
-// with unequal masses), 3 FMA, and a v_min towards one wave-uniform
-// neighbour-candidate test per chunk; only
+// Per pair: 3 sub, 3 FMA for r^2 + H^2, one v_rsq_f32, 2 mul (3 with
+// unequal masses), 3 FMA, and a v_min towards one wave-uniform
+// neighbour-candidate test per chunk (a second transcendental per pair in
+// place of a multiply, rsqrt * rcp, measured 12% slower); only
 // chunks where some lane has a candidate enter the (rare) neighbour path.
 // Sums are plain FP32 over blocks of 128 j, then compensated (TwoSum) into
 // the running total.
@@ -117,15 +118,13 @@ __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, 
             float inv = 0.0f;
             if (GRAV || POT) inv = __builtin_amdgcn_rsqf(d2[q][c]);
             if (GRAV) {
-                // r^-3 = rsqrt(d2) * rcp(d2): the second factor from the transcendental
-                // unit, which runs beside the FMA pipe, instead of a VALU multiply
-                const float rcp = __builtin_amdgcn_rcpf(d2[q][c]);
                 float s;
                 if (UM) {
-                    s = inv * rcp;
+                    s = inv * inv;
+                    s = s * inv;
                 } else {
                     s = pj[c].w * inv;
-                    s = s * rcp;
+                    s = s * (inv * inv);
                 }
                 if (!SOFT) s = (d2[q][c] > 0.0f) ? s : 0.0f;  // r^2 > 0 (computeDerivatives.h:226)
                 a.ax[q] = fmaf(dx, s, a.ax[q]);
