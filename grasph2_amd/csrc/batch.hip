@@ -395,6 +395,7 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     std::memset(&N, 0, sizeof N);
     N.posm = (const float4*)p->posm;
     N.vel = (const float4*)p->vel;
+    N.xs = perm ? b.d_xs : (const float4*)p->posm;
     N.rec = b.d_rec;
     N.dstress = p->dstress;
     N.balsara_in = p->balsara;

@@ -316,6 +316,7 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
 
     NbrLaunch N = nbr_base(c, p);
     N.perm = perm;
+    N.xs = xs;
     N.rec = c.rec;
     if (dens_pass) {
         ProfScope ps(&c, s, KC_DENSITY);

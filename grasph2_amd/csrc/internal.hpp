@@ -239,6 +239,7 @@ hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s);
 struct NbrLaunch {
     const float4* posm;
     const float4* vel;
+    const float4* xs;  // positions in slot order (posm itself when perm is null)
     SlotRec* rec;
     const float* dstress;
     const float* balsara_in;

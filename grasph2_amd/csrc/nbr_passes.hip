@@ -66,23 +66,26 @@ struct Lists {
 // Walk slot t's neighbours: the list, or when it overflowed, a scan of the
 // instance's slots with the exact test (same partners, same order).
 template <typename F>
-__device__ __forceinline__ void for_each_neighbour(const SlotRec* __restrict__ rec, const Lists& Ls,
+__device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ xs, const Lists& Ls,
                                                    const size_t nstride, const int t, const int kmax,
                                                    const float4 pi, const SlotMap& m, const float H2, F&& body)
 {
     const int cnt = Ls.cnt[t];
     if (cnt <= kmax) {
+        // unrolled so the loads of the next partners issue before this one's arithmetic
+#pragma unroll 4
         for (int k = 0; k < cnt; ++k) body(Ls.nbr[(size_t)k * nstride + t]);
     } else {
         for (int j = m.off; j < m.off + m.n; ++j) {
-            const float4 pj = rec[j].pm;
+            const float4 pj = xs[j];
             if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
         }
     }
 }
 
 template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void density_kernel(SlotRec* __restrict__ rec, const int* __restrict__ perm,
+__global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ xs, SlotRec* __restrict__ rec,
+                                                         const int* __restrict__ perm,
                                                          const Lists Ls,
                                                          const size_t nstride, const int kmax,
                                                          float* __restrict__ density, const BlockTables T,
@@ -93,10 +96,10 @@ __global__ __launch_bounds__(kBlock) void density_kernel(SlotRec* __restrict__ r
     if (t >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
-    const float4 pi = rec[t].pm;
+    const float4 pi = xs[t];  // positions from the compact slot-ordered array (16 B per partner)
     CSum rho{0.0f, 0.0f};
-    for_each_neighbour(rec, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const float4 pj = rec[j].pm;
+    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
+        const float4 pj = xs[j];
         const float r = sqrtf(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
         rho.add(pj.w * w_spline(r, P.H, P.Wpref));
     });
@@ -108,9 +111,9 @@ __global__ __launch_bounds__(kBlock) void density_kernel(SlotRec* __restrict__ r
 
 template <bool BATCH, bool SOLIDS, bool BALS, bool INTD>
 __global__ __launch_bounds__(kBlock) void strain_kernel(
-    SlotRec* __restrict__ rec, const float* __restrict__ dstress, const int* __restrict__ perm, const Lists Ls,
-    const size_t nstride, const int kmax, float* __restrict__ balsara, float* __restrict__ density_dt,
-    float* __restrict__ dstress_dt, const BlockTables T, const DevParams P0)
+    const float4* __restrict__ xs, SlotRec* __restrict__ rec, const float* __restrict__ dstress,
+    const int* __restrict__ perm, const Lists Ls, const size_t nstride, const int kmax, float* __restrict__ balsara,
+    float* __restrict__ density_dt, float* __restrict__ dstress_dt, const BlockTables T, const DevParams P0)
 {
     const SlotMap m = slot_map<BATCH>(T, (int)nstride);
     const int t = m.slot0 + threadIdx.x;
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 
-    for_each_neighbour(rec, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
+    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const float4 pj = rec[j].pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
@@ -136,8 +139,9 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
         const float4 vj = rec[j].vr;
         const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
         const float mj = pj.w, rhoj = vj.w;
+        const float vol = mj / rhoj;  // one division per pair, shared by the terms below
         if (SOLIDS) {  // models.h:225-248
-            const float tmp = -0.5f * mj / rhoj;
+            const float tmp = -0.5f * vol;
             exx += tmp * (vx * gx + vx * gx);
             exy += tmp * (vx * gy + vy * gx);
             exz += tmp * (vx * gz + vz * gx);
@@ -148,12 +152,12 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
             rxz += tmp * (vx * gz - vz * gx);
             ryz += tmp * (vy * gz - vz * gy);
         } else if (BALS) {  // curl v, computeDerivatives.h:134
-            const float f = mj / rhoj;
+            const float f = vol;
             cx += f * (vy * gz - vz * gy);
             cy += f * (vz * gx - vx * gz);
             cz += f * (vx * gy - vy * gx);
         }
-        if (BALS || INTD) divv -= (mj / rhoj) * (vx * gx + vy * gy + vz * gz);  // :137
+        if (BALS || INTD) divv -= vol * (vx * gx + vy * gy + vz * gz);  // :137
     });
 
     // do_after, computeDerivatives.h:144-161
@@ -231,7 +235,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SlotRec* __restrict__ rec,
 
 template <bool BATCH, bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
 __global__ __launch_bounds__(kBlock) void accel_kernel(
-    const SlotRec* __restrict__ rec, const int* __restrict__ perm, const Lists Ls, const size_t nstride,
+    const float4* __restrict__ xs, const SlotRec* __restrict__ rec, const int* __restrict__ perm, const Lists Ls,
+    const size_t nstride,
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
     const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    for_each_neighbour(rec, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
+    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
         const SlotRec q = rec[j];
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
@@ -352,8 +357,8 @@ hipError_t strain_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
     const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
                in = P.flags & GRASPH_INTEGRATE_DENSITY;
 #define GRASPH_STRAIN(A, B, C)                                                                                     \
-    hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.perm,  \
-                       lists_of(L), (size_t)L.n, L.kmax, L.balsara, L.density_dt, L.dstress_dt, L.T, P)
+    hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.dstress,    \
+                       L.perm, lists_of(L), (size_t)L.n, L.kmax, L.balsara, L.density_dt, L.dstress_dt, L.T, P)
     const int key = (so ? 4 : 0) | (ba ? 2 : 0) | (in ? 1 : 0);
     switch (key) {
         case 0: GRASPH_STRAIN(false, false, false); break;
@@ -376,7 +381,7 @@ hipError_t accel_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
                av = P.flags & GRASPH_ARTIFICIAL_VISCOSITY, xs = P.flags & GRASPH_XSPH,
                ar = P.flags & GRASPH_ARTIFICIAL_STRESS;
 #define GRASPH_ACCEL(A, B, C, D, E)                                                                                  \
-    hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.rec, L.perm,         \
+    hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm,   \
                        lists_of(L), (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig, L.acc_has_gravity ? 1 : 0,  \
                        L.T, P)
     // SOLIDS + ARTIFICIAL_STRESS is rejected before launch (the reference does not compile it).
@@ -418,10 +423,10 @@ hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.perm, lists_of(L),
+        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm, lists_of(L),
                            (size_t)L.n, L.kmax, density, L.T, P);
     else
-        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.perm, lists_of(L),
+        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm, lists_of(L),
                            (size_t)L.n, L.kmax, density, L.T, P);
     return hipGetLastError();
 }
