@@ -22,6 +22,20 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// one axis of handleBoundaryConditions (integration.h:33-73): put a particle
+// past a wall on the wall and reflect that velocity component
+template <typename T>
+__device__ __forceinline__ void box_axis(T& x, float& v, const float lo, const float hi, const float vm)
+{
+    if (x > (T)hi) {
+        x = (T)hi;
+        v -= vm * v;
+    } else if (x < (T)lo) {
+        x = (T)lo;
+        v -= vm * v;
+    }
+}
+
 // dt of the second variable-step kernel, integration.h:170
 __device__ __forceinline__ float vts_dt(const float* t, const DevParams& P)
 {
@@ -108,15 +122,17 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
         }
     }
 
-    if (P.flags & GRASPH_SIMPLE_BOX_BOUNDARY) {  // :33-73
+    if (P.flags & GRASPH_SIMPLE_BOX_BOUNDARY) {  // :33-73, on the FP64 masters when present
         const float vm = 1.0f + P.refl;
-        if (p.x > P.bmaxx) { p.x = P.bmaxx; v.x -= vm * v.x; X = p.x; }
-        else if (p.x < P.bminx) { p.x = P.bminx; v.x -= vm * v.x; X = p.x; }
-        if (p.y > P.bmaxy) { p.y = P.bmaxy; v.y -= vm * v.y; Y = p.y; }
-        else if (p.y < P.bminy) { p.y = P.bminy; v.y -= vm * v.y; Y = p.y; }
-        if (P.dim == 3) {
-            if (p.z > P.bmaxz) { p.z = P.bmaxz; v.z -= vm * v.z; Z = p.z; }
-            else if (p.z < P.bminz) { p.z = P.bminz; v.z -= vm * v.z; Z = p.z; }
+        if (pos64) {
+            box_axis(X, v.x, P.bminx, P.bmaxx, vm);
+            box_axis(Y, v.y, P.bminy, P.bmaxy, vm);
+            if (P.dim == 3) box_axis(Z, v.z, P.bminz, P.bmaxz, vm);
+            p.x = (float)X; p.y = (float)Y; p.z = (float)Z;
+        } else {
+            box_axis(p.x, v.x, P.bminx, P.bmaxx, vm);
+            box_axis(p.y, v.y, P.bminy, P.bmaxy, vm);
+            if (P.dim == 3) box_axis(p.z, v.z, P.bminz, P.bmaxz, vm);
         }
     }
     if (pos64) {

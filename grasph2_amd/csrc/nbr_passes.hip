@@ -65,23 +65,42 @@ struct Lists {
 
 // Walk slot t's neighbours: the list, or when it overflowed, a scan of the
 // instance's slots with the exact test (same partners, same order).
-template <typename F>
+// load(j) fetches what body(j, data) needs; the walk is software-pipelined
+// (partner k+1's data and k+2's index are in flight while k is computed),
+// so a lane keeps two gathers outstanding instead of waiting on each.  The
+// look-ahead past the end reloads the last partner (always in range).
+template <typename L, typename F>
 __device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ xs, const Lists& Ls,
                                                    const size_t nstride, const int t, const int kmax,
-                                                   const float4 pi, const SlotMap& m, const float H2, F&& body)
+                                                   const float4 pi, const SlotMap& m, const float H2, L&& load,
+                                                   F&& body)
 {
     const int cnt = Ls.cnt[t];
     if (cnt <= kmax) {
-        // unrolled so the loads of the next partners issue before this one's arithmetic
-#pragma unroll 4
-        for (int k = 0; k < cnt; ++k) body(Ls.nbr[(size_t)k * nstride + t]);
+        if (cnt <= 0) return;
+        const int* list = Ls.nbr + t;
+        int j0 = list[0];
+        int j1 = (cnt > 1) ? list[nstride] : j0;
+        auto d0 = load(j0);
+        for (int k = 0; k < cnt; ++k) {
+            const auto d1 = load(j1);
+            const int j2 = (k + 2 < cnt) ? list[(size_t)(k + 2) * nstride] : j1;
+            body(j0, d0);
+            j0 = j1;
+            d0 = d1;
+            j1 = j2;
+        }
     } else {
         for (int j = m.off; j < m.off + m.n; ++j) {
             const float4 pj = xs[j];
-            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j);
+            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j, load(j));
         }
     }
 }
+
+struct PV {  // position/mass and velocity/density of a partner
+    float4 pm, vr;
+};
 
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ xs, SlotRec* __restrict__ rec,
@@ -98,8 +117,8 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     const int i = perm ? perm[t] : t;
     const float4 pi = xs[t];  // positions from the compact slot-ordered array (16 B per partner)
     CSum rho{0.0f, 0.0f};
-    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const float4 pj = xs[j];
+    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return xs[j]; },
+                       [&](int, const float4 pj) {
         const float r = sqrtf(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
         rho.add(pj.w * w_spline(r, P.H, P.Wpref));
     });
@@ -128,15 +147,16 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 
-    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const float4 pj = rec[j].pm;
+    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return PV{rec[j].pm, rec[j].vr}; },
+                       [&](int, const PV& q) {
+        const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
         const float r = sqrtf(r2);
         const float dw = dw_spline(r, P.H, P.dWpref);
         const float s = dw / r;
         const float gx = s * rx, gy = s * ry, gz = s * rz;
-        const float4 vj = rec[j].vr;
+        const float4 vj = q.vr;
         const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
         const float mj = pj.w, rhoj = vj.w;
         const float vol = mj / rhoj;  // one division per pair, shared by the terms below
@@ -261,8 +281,8 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) {
-        const SlotRec q = rec[j];
+    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return rec[j]; },
+                       [&](int, const SlotRec& q) {
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);

@@ -165,6 +165,67 @@ def test_liquid_extras(gpu_ctx):
     check_vec(out["xvel"], ref["xvel"], what="xvel")
 
 
+@pytest.mark.parametrize("case", ["solid", "gravity", "none"])
+def test_environment_forces(gpu_ctx, case):
+    """Clohessy-Wiltshire and constant acceleration on top of SPH + gravity
+    (added by the acceleration pass), on gravity alone (added where the
+    sweep stores its sums: j-split combine at this N) and with neither
+    (the sweep only finishes), against the FP64 oracle."""
+    if case == "solid":
+        h, prm = solid_case(2500, seed=23)
+    else:
+        h, prm = ics.plummer(5000, seed=24)
+        h["vel"][:, :3] = np.random.default_rng(24).normal(scale=0.3, size=(5000, 3))
+        if case == "none":
+            prm.flags &= ~g.SELF_GRAVITY
+    prm.flags |= g.CLOHESSY_WILTSHIRE | g.CONSTANT_ACCELERATION
+    prm.cw_n = 0.7
+    prm.constant_acceleration[:] = (0.1, -0.5, 0.25)
+    out = gpu_run(gpu_ctx, h, prm)
+    ref = O.compute_derivatives(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
+    if "xvel" in out:
+        assert np.all(out["xvel"][:, :3] == 0)
+
+
+@pytest.mark.parametrize("variable", [False, True])
+def test_leapfrog_box_boundary(gpu_ctx, variable):
+    """SIMPLE_BOX_BOUNDARY (integration.h:33-73): particles drifting out of
+    the box are put on the wall and their velocity reflected, for the fixed
+    and the variable step, against the FP64 oracle."""
+    h, prm = solid_case(2000, seed=53)
+    prm.flags |= g.SIMPLE_BOX_BOUNDARY
+    lo = np.quantile(h["posm"][:, :3], 0.1, axis=0)  # a box that cuts through the bodies
+    hi = np.quantile(h["posm"][:, :3], 0.9, axis=0)
+    prm.box_min[:] = lo
+    prm.box_max[:] = hi
+    prm.box_reflectiveness = 0.6
+    rng = np.random.default_rng(5)
+    n = h["posm"].shape[0]
+    h["vel"][:, :3] = rng.normal(scale=40.0, size=(n, 3))  # many cross a wall in one step
+    h["acc"][:, :3] = rng.normal(size=(n, 3))
+    if variable:
+        prm.flags |= g.VARIABLE_TIMESTEP
+        prm.initial_timestep = 2e-3
+        ctx = g.Context(0)
+        d = g.DeviceParticles.from_host(h)
+        ctx.integrate(d, prm, False)
+        g.hip.synchronize()
+        out = d.download()
+        d.free()
+        ctx.close()
+        st = O.TimestepState("f64")
+        ref = O.variable_leapfrog(prm, h, st, not_first_step=False)
+    else:
+        out = gpu_run(gpu_ctx, h, prm, "integrate", not_first_step=True)
+        ref = O.leapfrog(prm, h, not_first_step=True, prec="f64")
+    moved = np.any((ref["posm"][:, :3] <= lo) | (ref["posm"][:, :3] >= hi), axis=1)
+    assert moved.sum() > 100
+    # FP32 velocities (|v| ~ 40) times dt against the oracle's FP64 ones: ~1e-8
+    assert np.allclose(out["pos64"].T, ref["posm"][:, :3], rtol=0, atol=2e-8)
+    assert np.allclose(out["vel"][:, :3], ref["vel"][:, :3], rtol=1e-6, atol=1e-5)
+
+
 def test_standalone_passes(gpu_ctx):
     h, prm = solid_case(2500, seed=31)
     out = gpu_run(gpu_ctx, h, prm, "density")
