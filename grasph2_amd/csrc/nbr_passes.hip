@@ -26,6 +26,13 @@
 // batch (SlotMap and parameters of the block's instance from tables).
 #include "internal.hpp"
 
+// No multiply-add contraction in this file: the pair arithmetic is inlined
+// into the list walk and into the overflow scan, and contraction decided per
+// call site could round the two differently.  Every product and sum rounds
+// as written, as in the FP32 oracle; the passes are bound by their gathers,
+// not by arithmetic.
+#pragma clang fp contract(off)
+
 namespace grasph {
 namespace {
 
