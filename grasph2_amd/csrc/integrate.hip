@@ -167,12 +167,11 @@ __global__ __launch_bounds__(kBlock) void vts_kick_kernel(float4* __restrict__ v
         v.z = v.z + a.z * h;
         vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
         if (P.flags & GRASPH_ACCELERATION_CRITERION) {  // :149-151
-            const float a2 = __fadd_rn(__fadd_rn(__fmul_rn(a.x, a.x), __fmul_rn(a.y, a.y)), __fmul_rn(a.z, a.z));
-            const float la = __fsqrt_rn(a2);
-            mindt = fminf(mindt, __fsqrt_rn(__fdiv_rn(__fmul_rn(P.acc_acc, P.H), la)));
+            const float la = __fsqrt_rn(dist2_exact(a.x, a.y, a.z));  // length(acc), products and sums unfused
+            mindt = fminf(mindt, __fsqrt_rn(__fdiv_rn(mul_exact(P.acc_acc, P.H), la)));
         }
         if (P.flags & GRASPH_VELOCITY_CRITERION)  // :152-154
-            mindt = fminf(mindt, __fdiv_rn(__fmul_rn(P.vel_acc, P.H), maxvsig[i]));
+            mindt = fminf(mindt, __fdiv_rn(mul_exact(P.vel_acc, P.H), maxvsig[i]));
     }
     // block minimum, then one atomic minimum on the bit pattern (:158-168)
     for (int off = 32; off >= 1; off >>= 1) mindt = fminf(mindt, __shfl_xor(mindt, off, 64));

@@ -68,9 +68,25 @@ __device__ inline float eos_murnaghan(float rho, float rho0, float K0, float dK0
 // Exact FP32 squared distance in the reference's dot() order
 // (x*x + y*y) + z*z with every product and sum rounded (no contraction),
 // so the neighbour test r2 <= H2 && r2 > 0 decides exactly as the oracle.
+// The pragma marks these operations as not fusable wherever the function is
+// inlined (HIP's __fmul_rn / __fadd_rn are plain operators that a
+// contracting caller may fuse).
 __device__ inline float dist2_exact(float rx, float ry, float rz)
 {
-    return __fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz));
+#pragma clang fp contract(off)
+    return (rx * rx + ry * ry) + rz * rz;
+}
+
+// a * b and a + b rounded as written, never fused into a multiply-add
+__device__ inline float mul_exact(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ inline float add_exact(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
 }
 
 // Neighbour predicate of src/computeDerivatives.h:72,122,238:  0 < r2 <= H2.
