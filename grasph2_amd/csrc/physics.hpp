@@ -27,6 +27,14 @@ struct DevParams {
     float ts_init, ts_min, ts_max, acc_acc, vel_acc;
 };
 
+// Everything below rounds each product and sum as written, wherever it is
+// inlined: no multiply-add contraction (HIP contracts by default, and a
+// contraction decided per call site would let two walks over the same pairs
+// -- the neighbour list and its overflow scan -- round differently).  The
+// FP32 oracle is built the same way.  The default resumes at the end of
+// this header.
+#pragma clang fp contract(off)
+
 // src/sph/kernel.h:41-60
 __host__ __device__ inline float spline_prefactor(int dim, float h)
 {
@@ -68,26 +76,16 @@ __device__ inline float eos_murnaghan(float rho, float rho0, float K0, float dK0
 // Exact FP32 squared distance in the reference's dot() order
 // (x*x + y*y) + z*z with every product and sum rounded (no contraction),
 // so the neighbour test r2 <= H2 && r2 > 0 decides exactly as the oracle.
-// The pragma marks these operations as not fusable wherever the function is
-// inlined (HIP's __fmul_rn / __fadd_rn are plain operators that a
-// contracting caller may fuse).
+// (HIP's __fmul_rn / __fadd_rn are plain operators that a contracting caller
+// may fuse; the contract(off) region above keeps these unfused.)
 __device__ inline float dist2_exact(float rx, float ry, float rz)
 {
-#pragma clang fp contract(off)
     return (rx * rx + ry * ry) + rz * rz;
 }
 
 // a * b and a + b rounded as written, never fused into a multiply-add
-__device__ inline float mul_exact(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a * b;
-}
-__device__ inline float add_exact(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a + b;
-}
+__device__ inline float mul_exact(float a, float b) { return a * b; }
+__device__ inline float add_exact(float a, float b) { return a + b; }
 
 // Neighbour predicate of src/computeDerivatives.h:72,122,238:  0 < r2 <= H2.
 // As unsigned bit patterns of non-negative floats: r2 in (0, H2]  <=>
@@ -202,5 +200,7 @@ __device__ inline void environment_acceleration(const DevParams& P, float px, fl
     }
     (void)py;
 }
+
+#pragma clang fp contract(fast)
 
 }  // namespace grasph
