@@ -358,7 +358,15 @@ __global__ __launch_bounds__(kBlock) void merge_lists_kernel(int* __restrict__ n
         if (total + cs <= kmax) {
             const int* src = nbr + (size_t)s * kmax * ns;
             int* dst = nbr + (size_t)total * ns;
-            for (int k = 0; k < cs; ++k) dst[(size_t)k * ns + t] = src[(size_t)k * ns + t];
+            // 8 loads in flight, then their stores (the two regions never overlap)
+            for (int k = 0; k < cs; k += 8) {
+                int v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = (k + u < cs) ? src[(size_t)(k + u) * ns + t] : 0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (k + u < cs) dst[(size_t)(k + u) * ns + t] = v[u];
+            }
         }
         total += cs;  // once above kmax it stays above: the list is then ignored
     }
