@@ -114,16 +114,16 @@ int ensure_nbr(Ctx& c, int64_t n, int slices)
 
 // j-split of the single-instance sweep: enough slices for ~16 wavefronts
 // per SIMD (measured: the sweep keeps gaining up to there, as more resident
-// waves hide the scalar-load and rare-path latency), each slice at least
-// 1024 j; slice length a multiple of 8.
+// waves hide the scalar-load and rare-path latency), at most 32 slices of
+// at least 512 j each; slice length a multiple of 8.
 void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len)
 {
     int S = c.sweep_slices;
     if (S <= 0) {
         const int64_t waves = (n + 64 * ni - 1) / (64 * ni);
         const int64_t target = 16 * 4 * (int64_t)c.cus;
-        S = (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + waves - 1) / waves));
-        S = (int)std::max<int64_t>(1, std::min<int64_t>(S, n / 1024));
+        S = (int)std::min<int64_t>(32, std::max<int64_t>(1, (target + waves - 1) / waves));
+        S = (int)std::max<int64_t>(1, std::min<int64_t>(S, n / 512));
     }
     int64_t len = (n + S - 1) / S;
     len = (len + 7) / 8 * 8;
