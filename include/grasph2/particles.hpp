@@ -27,6 +27,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
@@ -49,17 +50,50 @@ inline f3_t operator-(f3_t a, f3_t b) { return {a.x - b.x, a.y - b.y, a.z - b.z}
 inline f3_t operator*(f3_t a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 inline f3_t operator*(float s, f3_t a) { return a * s; }
 inline bool operator==(f3_t a, f3_t b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+inline f3_t operator-(f3_t a) { return {-a.x, -a.y, -a.z}; }
+inline f3_t operator/(f3_t a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline f3_t& operator+=(f3_t& a, f3_t b) { return a = a + b; }
+inline f3_t& operator-=(f3_t& a, f3_t b) { return a = a - b; }
+inline f3_t& operator*=(f3_t& a, float s) { return a = a * s; }
+inline f3_t& operator/=(f3_t& a, float s) { return a = a / s; }
+inline float dot(f3_t a, f3_t b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline f3_t cross(f3_t a, f3_t b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float length(f3_t a) { return std::sqrt(dot(a, a)); }
+inline f3_t normalize(f3_t a) { return a / length(a); }
 
 // 3x3 matrix, row-major, Mat(v) = v * identity as mpu::Mat<float,3,3>
 struct m3_t {
     float m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     m3_t() = default;
     explicit m3_t(float v) { m[0][0] = m[1][1] = m[2][2] = v; }
+    // nine entries, row by row
+    m3_t(float a00, float a01, float a02, float a10, float a11, float a12, float a20, float a21, float a22)
+        : m{{a00, a01, a02}, {a10, a11, a12}, {a20, a21, a22}}
+    {
+    }
     float* operator[](int r) { return m[r]; }
     const float* operator[](int r) const { return m[r]; }
     float& operator()(int e) { return m[e / 3][e % 3]; }
     float operator()(int e) const { return m[e / 3][e % 3]; }
 };
+inline m3_t operator+(const m3_t& a, const m3_t& b)
+{
+    m3_t r;
+    for (int e = 0; e < 9; ++e) r(e) = a(e) + b(e);
+    return r;
+}
+inline m3_t operator*(float s, const m3_t& a)
+{
+    m3_t r;
+    for (int e = 0; e < 9; ++e) r(e) = s * a(e);
+    return r;
+}
+// matrix times column vector: row r of the result is row r of a dotted with v
+inline f3_t operator*(const m3_t& a, f3_t v)
+{
+    return {a[0][0] * v.x + a[0][1] * v.y + a[0][2] * v.z, a[1][0] * v.x + a[1][1] * v.y + a[1][2] * v.z,
+            a[2][0] * v.x + a[2][1] * v.y + a[2][2] * v.z};
+}
 
 // ------------------------------------------------------------------ attributes
 #define GRASPH_ATTRIBUTE(NAME, member, TYPE)                      \

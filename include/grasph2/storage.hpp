@@ -1,5 +1,6 @@
 // grasph2/storage.hpp -- GraSPH2's snapshot text files: the asynchronous
-// writer ResultStorageManager and the TextFile particle source.
+// writer ResultStorageManager, the TextFile particle source and main.cu's
+// loadParticlesFromFile.
 //
 // Format (src/ResultStorageManager.cu:101-186, printTextFile and
 // attributePrinter): one line per particle; every value of every attribute
@@ -39,6 +40,7 @@
 #include <utility>
 #include <vector>
 
+#include "initial_conditions.hpp"
 #include "particles.hpp"
 
 namespace grasph {
@@ -253,9 +255,10 @@ private:
 namespace ps {
 
 // TextFile<ParticleToRead> (src/initialConditions/particleSources/TextFile.h):
-// the particles of a snapshot file; generateParticle(id) parses line id.
+// a particle source over the lines of a snapshot file; particle id is line
+// id parsed into ParticleToRead, then the source's modifiers apply.
 template <typename ParticleToRead>
-class TextFile {
+class TextFile : public ParticleSource<ParticleToRead, TextFile<ParticleToRead>> {
 public:
     explicit TextFile(const std::string& filename, char seperator = '\t') : sep_(seperator)
     {
@@ -267,9 +270,11 @@ public:
             for (char c : line) blank = blank && std::isspace((unsigned char)c);
             if (!blank) lines_.push_back(line);
         }
+        this->m_numberOfParticles = lines_.size();
     }
     size_t size() const { return lines_.size(); }
-    ParticleToRead generateParticle(size_t id) const
+    // line id parsed, without the modifiers
+    ParticleToRead particle(size_t id) const
     {
         ParticleToRead p{};
         std::istringstream line(lines_[id]);
@@ -280,18 +285,35 @@ public:
         }
         return p;
     }
-    // all particles into a host buffer (what InitGenerator::generate() yields for this source)
-    template <typename... Tags> void load(HostParticleBuffer<Tags...>& out) const
+    // all particles into a host buffer (what InitGenerator::generate() yields for this source alone)
+    template <typename... Tags> void load(HostParticleBuffer<Tags...>& out)
     {
         out = HostParticleBuffer<Tags...>(size());
-        for (size_t i = 0; i < size(); ++i) out.storeParticle(i, generateParticle(i));
+        for (size_t i = 0; i < size(); ++i) (*this)(out, i, i);
     }
 
 private:
+    ParticleToRead generateParticle(size_t id) override { return particle(id); }
+
     std::vector<std::string> lines_;
     char sep_;
 };
 
 }  // namespace ps
+
+// Adds the particles of a snapshot file to a generator, by file ending
+// (src/main.cu:43-67): .tsv tab-separated, .csv comma-separated, anything
+// else with textfileSeperator.  HDF5 files (.h5, .hdf5) need HighFive,
+// which this build does not carry: they raise an error.
+template <typename LoadParticleType, typename GeneratorType>
+void loadParticlesFromFile(const std::string& filename, GeneratorType& generator, char textfileSeperator = '\t')
+{
+    const auto dot = filename.find_last_of('.');
+    const std::string ending = dot == std::string::npos ? std::string() : filename.substr(dot + 1);
+    if (ending == "h5" || ending == "hdf5")
+        throw std::runtime_error("HDF5 input is not supported by this build: " + filename);
+    const char sep = ending == "tsv" ? '\t' : ending == "csv" ? ',' : textfileSeperator;
+    generator.addParticles(ps::TextFile<LoadParticleType>(filename, sep));
+}
 
 #endif  // GRASPH2_STORAGE_HPP
