@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["capi.hip", "sweep.hip", "nbr_passes.hip", "integrate.hip", "reorder.hip", "batch.hip"]
+SOURCES = ["capi.hip", "sweep.hip", "sym_sweep.hip", "nbr_passes.hip", "integrate.hip", "reorder.hip", "batch.hip"]
 
 
 def _compile(src):

@@ -175,6 +175,66 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
     return L;
 }
 
+// The symmetric sweep keeps one partner-side partial per (group of 1024,
+// later particle): 6 n^2 / 1024 bytes.  Past this size the j-split sweep
+// runs instead.
+constexpr int64_t kSymMaxPartialBytes = int64_t(12) << 30;
+
+int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : 2 * c.cus; }
+
+bool sym_fits(const Ctx& c, int64_t n, bool want_nbr)
+{
+    const SymGeom G = sym_geometry(n, sym_wgs(c), want_nbr, 1);
+    return G.bpart_elems * (int64_t)sizeof(float) <= kSymMaxPartialBytes;
+}
+
+// shares of the symmetric sweep's near field: ~16 wavefronts per SIMD, at most 16
+int sym_near_shares(const Ctx& c, int64_t n)
+{
+    const int64_t blocks = (n + 63) / 64;
+    const int64_t target = 16 * 4 * (int64_t)c.cus;
+    return (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + blocks - 1) / blocks));
+}
+
+int run_sym_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s)
+{
+    const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.nslices);
+    int r;
+    if ((r = grow(c, c.sym_box, c.sym_box_cap, G.box_elems, "symmetric sweep boxes")) ||
+        (r = grow(c, c.sym_bits, c.sym_bits_cap, G.bits_elems, "symmetric sweep near bits")) ||
+        (r = grow(c, c.sym_near, c.sym_near_cap, G.near_elems, "symmetric sweep near sums")) ||
+        (r = grow(c, c.sym_apart, c.sym_apart_cap, G.apart_elems, "symmetric sweep own-side sums")) ||
+        (r = grow(c, c.sym_bpart, c.sym_bpart_cap, G.bpart_elems, "symmetric sweep partner-side sums")))
+        return r;
+    SymLaunch S;
+    std::memset(&S, 0, sizeof S);
+    S.xs = L.xs ? L.xs : L.posm;
+    S.perm = L.perm;
+    S.n = L.n;
+    S.grav = L.grav;
+    S.want_nbr = L.want_nbr;
+    S.finish = L.finish;
+    S.nshares = L.nslices;
+    S.acc = L.acc;
+    S.nbr = L.nbr;
+    S.nbr_cnt = L.nbr_cnt;
+    S.kmax = L.kmax;
+    S.vel = L.vel;
+    S.xvel = L.xvel;
+    S.maxvsig = L.maxvsig;
+    S.umass = L.umass;
+    S.wgs = G.wgs;
+    S.box = c.sym_box;
+    S.bits = c.sym_bits;
+    S.near_part = c.sym_near;
+    S.apart = c.sym_apart;
+    S.bpart = c.sym_bpart;
+    ProfScope ps(&c, s, KC_SWEEP);
+    hipError_t e = launch_sym_sweep(S, P, s);
+    if (e != hipSuccess) return hip_fail(c, e, "symmetric sweep launch");
+    return 0;
+}
+
 int run_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s)
 {
     ProfScope ps(&c, s, KC_SWEEP);
@@ -259,14 +319,16 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     // the order only matters for the neighbour branch: gravity alone runs in index order
     if (need_list && (r = ensure_order(c, p, s, true))) return r;
     const bool sweep_grav = accel_pass && grav;
+    const bool sym = c.sweep_symmetric && (sweep_grav || need_list) && sym_fits(c, n, need_list);
     int slices = 1, slice_len = (int)n;
-    if (sweep_grav || need_list) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
+    if ((sweep_grav || need_list) && !sym) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
+    if (sym) slices = sym_near_shares(c, n);  // the near field's shares take the slices' list planes
     if (need_list) {
         if ((r = ensure_nbr(c, n, slices))) return r;
         c.last_list_n = n;
         c.last_list_slices = slices;
     }
-    if (slices > 1 && sweep_grav) {
+    if (slices > 1 && sweep_grav && !sym) {
         if ((r = grow(c, c.part, c.part_cap, 2 * n * slices, "sweep partial sums"))) return r;
     }
     const DevParams P = make_dev_params(prm);
@@ -305,7 +367,11 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         if (e != hipSuccess) return hip_fail(c, e, "mass check launch");
         L.umass = c.umass;
     }
-    if ((r = run_sweep(c, L, P, s))) return r;
+    if (sym) {
+        if ((r = run_sym_sweep(c, L, P, s))) return r;
+    } else if ((r = run_sweep(c, L, P, s))) {
+        return r;
+    }
     if (need_list && slices > 1) {
         // one list per particle for the passes (walking per-slice lists costs them up to 2x)
         ProfScope ps(&c, s, KC_OTHER);
@@ -457,7 +523,7 @@ int grasph_destroy(grasph_ctx* ctx)
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
                     c.nbr,  c.nbr_cnt, c.rec,     c.pot,  c.red,      c.part, c.ts, c.umass,
-                    c.xs};
+                    c.xs,   c.sym_box, c.sym_bits, c.sym_near, c.sym_apart, c.sym_bpart};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {
@@ -491,6 +557,12 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sweep_slices") {
         if (value < 0 || value > 64) return grasph::fail(c, GRASPH_E_INVALID, "sweep_slices must be in [0, 64]");
         c.sweep_slices = (int)value;
+    } else if (k == "sweep_symmetric") {
+        if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "sweep_symmetric must be 0 or 1");
+        c.sweep_symmetric = (int)value;
+    } else if (k == "sym_workgroups") {
+        if (value < 0 || value > 65536) return grasph::fail(c, GRASPH_E_INVALID, "sym_workgroups must be in [0, 65536]");
+        c.sym_workgroups = (int)value;
     } else if (k == "batch_waves_per_block") {
         if (value != 1 && value != 2 && value != 4)
             return grasph::fail(c, GRASPH_E_INVALID, "batch_waves_per_block must be 1, 2 or 4");

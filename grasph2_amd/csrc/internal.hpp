@@ -31,6 +31,8 @@ struct Ctx {
     int reorder_interval = 1;
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
+    int sweep_symmetric = 0;  // single instance: far pairs once for both sides (sym_sweep.hip)
+    int sym_workgroups = 0;   // workgroups of its far kernel (0: two per compute unit)
     int cus = 256;         // compute units of the device
     bool profile = false;
     // batched sweep tuning: i per lane per size bucket, persistent launch shape
@@ -67,6 +69,17 @@ struct Ctx {
     int last_list_slices = 1;
     float4* part = nullptr;   // partial sums of a j-split sweep
     int64_t part_cap = 0;
+    // work arrays of the symmetric sweep
+    float4* sym_box = nullptr;
+    int64_t sym_box_cap = 0;
+    uint32_t* sym_bits = nullptr;
+    int64_t sym_bits_cap = 0;
+    float4* sym_near = nullptr;
+    int64_t sym_near_cap = 0;
+    float4* sym_apart = nullptr;
+    int64_t sym_apart_cap = 0;
+    float* sym_bpart = nullptr;
+    int64_t sym_bpart_cap = 0;
 
     // slot-ordered positions for the sweep, and the slot records of the passes
     float4* xs = nullptr;
@@ -127,6 +140,46 @@ struct SweepLaunch {
     const float* umass; // {uniform, mass} from launch_mass_check, or null
 };
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s);
+
+// The symmetric single-instance sweep (sym_sweep.hip): super-blocks of 256
+// slots, groups of 4 super-blocks; gravity inside a group through the
+// candidate sweep, between groups once per unordered pair; neighbour lists
+// from the blocks of 64 whose boxes come within H.  Sizes of its work
+// arrays, in elements.
+struct SymGeom {
+    int nb, groups, wgs, nwords;
+    int64_t nsb, units;   // super-blocks of 256 slots; far work units
+    int64_t box_elems;    // float4: bounding box per block of 64
+    int64_t bits_elems;   // uint32: near-block bit matrix (lists wanted)
+    int64_t near_elems;   // float4: near-field partial per slot and share
+    int64_t apart_elems;  // float4: far own-side fragments
+    int64_t bpart_elems;  // float: far partner-side partials
+};
+SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares);
+struct SymLaunch {
+    const float4* xs;   // slot-ordered positions + masses
+    const int* perm;    // slot -> particle, or null
+    int n;
+    bool grav, want_nbr;
+    bool finish;        // as SweepLaunch::finish
+    int nshares;        // near field cut into this many shares per block (own list planes, merged after)
+    float4* acc;
+    int* nbr;           // a list per slot and share
+    int* nbr_cnt;
+    int kmax;
+    const float4* vel;
+    float4* xvel;
+    float* maxvsig;
+    const float* umass; // {uniform, mass} (grav)
+    int wgs;            // workgroups of the far kernel
+    float4* box;
+    uint32_t* bits;
+    float4* near_part;
+    float4* apart;
+    float* bpart;
+};
+hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t s);
+
 // Fold the per-slice lists of a j-split sweep into one list per particle
 // (slice 0's arrays; a count above kmax means "scan the instance").
 hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s);

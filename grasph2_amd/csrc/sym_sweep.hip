@@ -1,0 +1,590 @@
+// sym_sweep.hip -- the single-instance all-pairs sweep evaluated once per
+// unordered pair (gfx950).
+//
+// Same results as the sweep of sweep.hip (softened gravity of
+// calcAcceleration, src/computeDerivatives.h:229-235, plus the neighbour
+// lists of :72,122,238), organised by super-blocks of 256 consecutive slots
+// and groups of 4 super-blocks (1024 slots):
+//  - near field, one wavefront per block of 64 slots, partner blocks in
+//    increasing slot order through the candidate sweep of sweep_core.hpp:
+//    the blocks of its own group with gravity, and (when lists are wanted)
+//    every other block whose bounding box comes within H of its own,
+//    distance test only -- every SPH neighbour of a slot lies in such a
+//    block (the box distance, rounded like the pair distance, never exceeds
+//    it), so the lists come out complete and in slot order, exactly as the
+//    one-range sweep writes them;
+//  - far field: every super-block pair of different groups (I < J) is
+//    computed once, for both sides (Newton's third law).  A workgroup of 4 wavefronts owns a
+//    group, a wavefront one super-block (4 slots per lane), and streams the
+//    later groups' particles through SGPRs, 4 at a time in ping-pong.  A
+//    lane sums its own slots' gravity in registers and the partners' side
+//    over its 4 slots; every 8 partners, the partner side (8 x 3 values) is
+//    reduced across the 64 lanes with v_permlane32_swap, v_permlane16_swap
+//    and DPP adds (~54 instructions), staged in LDS, summed over the 4
+//    wavefronts in wavefront order and stored as one partial per (group,
+//    partner);
+//  - a combine kernel adds, per slot and in a fixed order, the near sum,
+//    the far own-side fragments and the far partner-side partials
+//    (compensated), then finishes the acceleration as the one-range sweep.
+// Per far pair: 3 sub, 3 FMA for r^2 + H^2, one v_rsq_f32, 2 mul, 3 FMA for
+// the own side and 3 FMA for the partner side (2 more mul with unequal
+// masses), plus 54 / 32 for the reduction: ~16.7 VALU instructions per
+// wavefront and partner for two interactions per lane and slot, against
+// ~12 for one interaction in the one-sided sweep.
+// Every sum is in a fixed order, so results do not vary from run to run.
+#include <algorithm>
+
+#include "sweep_core.hpp"
+
+namespace grasph {
+namespace {
+
+constexpr int kSB = 256;          // slots per super-block
+constexpr int kGroupSB = 4;       // super-blocks per group (wavefronts per far workgroup)
+constexpr int kGroup = kSB * kGroupSB;  // slots per group (1024)
+constexpr int kNI = kSB / 64;     // own slots per lane in the far field (4)
+constexpr int kSegChunks = kSB / kChunk;  // chunks of 8 partners per far segment (one super-block, 32)
+constexpr int kStage = 24;        // partner values per chunk: 8 partners x 3 components
+
+// ------------------------------------------------------------------ geometry of the work
+
+// far units (partner super-blocks) of row g: super-blocks [4 (g + 1), nsb)
+__device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb)
+{
+    const int64_t u = nsb - (int64_t)kGroupSB * (g + 1);
+    return u > 0 ? u : 0;
+}
+// units before row g (g <= rows_with_units): sum over g' < g of (nsb - 4 (g' + 1))
+__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb) { return g * nsb - 2 * g * (g + 1); }
+// rows with far work: nsb - 4 (g + 1) >= 1
+__device__ __host__ inline int64_t rows_with_units(int64_t nsb) { return nsb >= 1 ? (nsb - 1) / kGroupSB : 0; }
+// the workgroup whose share [k U / W, (k + 1) U / W) holds unit u
+__device__ __host__ inline int64_t wg_of_unit(int64_t u, int64_t U, int64_t W) { return ((u + 1) * W - 1) / U; }
+// first element of the partner-side partials of group h (the h groups before it contribute)
+__device__ __host__ inline int64_t bpart_group_base(int64_t h) { return (int64_t)kGroup * (h * (h - 1) / 2); }
+
+// ------------------------------------------------------------------ cross-lane reduction
+
+__device__ __forceinline__ float swap32_sum(float p, float q)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(p), __float_as_uint(q), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap16_sum(float p, float q)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(p), __float_as_uint(q), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+// lanes with hi keep q, the others p, each added to its DPP partner's copy
+template <int CTRL>
+__device__ __forceinline__ float dpp_pair_sum(float p, float q, bool hi)
+{
+    const float keep = hi ? q : p, give = hi ? p : q;
+    return keep + dpp<CTRL>(give);
+}
+constexpr int kRowRor8 = 0x128;       // lane ^ 8 within a row of 16
+constexpr int kRowHalfMirror = 0x141; // lane i <-> 7 - i within 8
+constexpr int kQuadXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]
+
+// Sum over the 64 lanes of px[b], py[b], pz[b] for b < 8 (recursive halving:
+// value index b * 4 + component, its bits consumed by lane bits 5..1).
+// Afterwards an even lane L holds the sum for partner b = L >> 3 and
+// component (L >> 1) & 3 (3 is no component); lane L + 1 holds the same.
+// Needs all 64 lanes active.
+__device__ __forceinline__ float reduce_partners(const float (&px)[kChunk], const float (&py)[kChunk],
+                                                 const float (&pz)[kChunk], const int lane)
+{
+    float y[4][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        y[b][0] = swap32_sum(px[b], px[b + 4]);
+        y[b][1] = swap32_sum(py[b], py[b + 4]);
+        y[b][2] = swap32_sum(pz[b], pz[b + 4]);
+    }
+    float z[2][3];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) z[b][c] = swap16_sum(y[b][c], y[b + 2][c]);
+    const bool h3 = lane & 8, h2 = lane & 4, h1 = lane & 2;
+    float w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[c] = dpp_pair_sum<kRowRor8>(z[0][c], z[1][c], h3);
+    const float v0 = dpp_pair_sum<kRowHalfMirror>(w[0], w[2], h2);  // components 0 / 2
+    const float v1 = w[1] + dpp<kRowHalfMirror>(w[1]);              // component 1 (the lanes of 3 unused)
+    const float t = dpp_pair_sum<kQuadXor2>(v0, v1, h1);
+    return t + dpp<kQuadXor1>(t);
+}
+
+// ------------------------------------------------------------------ kernels
+
+// Bounding box of every block of 64 slots (one wavefront per block).
+__global__ __launch_bounds__(kBlock) void block_box_kernel(const float4* __restrict__ xs, const int n, const int nb,
+                                                           float4* __restrict__ box)
+{
+    const int blk = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (blk >= nb) return;  // whole wavefronts leave together
+    const int t = 64 * blk + lane;
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (t < n) {
+        const float4 p = xs[t];
+        v[0] = v[3] = p.x; v[1] = v[4] = p.y; v[2] = v[5] = p.z;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = fminf(v[c], __shfl_xor(v[c], o, 64));
+            v[c + 3] = fmaxf(v[c + 3], __shfl_xor(v[c + 3], o, 64));
+        }
+    }
+    if (lane == 0) {
+        box[2 * blk] = make_float4(v[0], v[1], v[2], 0.0f);
+        box[2 * blk + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+    }
+}
+
+// near[I][J] (bit J % 32 of word I * nwords + J / 32) for blocks of 64:
+// boxes within H.  The per-axis gaps and their squared sum round like the
+// exact pair distance, so a pair with r^2 <= H^2 never sits in two blocks
+// whose box distance exceeds H^2.  Symmetric by construction.
+__global__ __launch_bounds__(kBlock) void near_bits_kernel(const float4* __restrict__ box, const int nb,
+                                                           const int nwords, const float H2,
+                                                           uint32_t* __restrict__ bits)
+{
+    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= (int64_t)nb * nwords) return;
+    const int I = (int)(idx / nwords), w = (int)(idx - (int64_t)I * nwords);
+    const float4 lo = box[2 * I], hi = box[2 * I + 1];
+    uint32_t word = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int J = 32 * w + b;
+        if (J >= nb) break;
+        const float4 l2 = box[2 * J], h2 = box[2 * J + 1];
+        const float gx = fmaxf(0.0f, fmaxf(l2.x - hi.x, lo.x - h2.x));
+        const float gy = fmaxf(0.0f, fmaxf(l2.y - hi.y, lo.y - h2.y));
+        const float gz = fmaxf(0.0f, fmaxf(l2.z - hi.z, lo.z - h2.z));
+        if (dist2_exact(gx, gy, gz) <= H2) word |= 1u << b;
+    }
+    bits[idx] = word;
+}
+
+// Near field of the 64 slots of block I, share `sl` of `ns`: the partner
+// blocks are the blocks of its group (gravity, and the distance test when
+// lists are wanted) and, with bits, the other blocks near it (distance test
+// only); in increasing order they are cut into ns runs of equal count, and
+// this wavefront takes run sl.  Writes the slot's compensated gravity
+// partial (slot-indexed, two float4) and its list and count -- all in the
+// planes of share sl, which later fold in share order.
+template <bool GRAV, bool NBR, bool SOFT, bool UM>
+__device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const int n, const int nb,
+                                            const int nwords, const uint32_t* __restrict__ bits, const int I,
+                                            const int sl, const int ns, const int lane, int* __restrict__ nbr_cnt,
+                                            float4* __restrict__ near_part, const DevParams& P, const Rare& R,
+                                            const float m)
+{
+    constexpr int kGroupBlocks = kGroup / 64;  // 16
+    const int t = 64 * I + lane;
+    const bool act = t < n;
+    float xi[1], yi[1], zi[1];
+    xi[0] = yi[0] = zi[0] = __int_as_float(0x7fc00000);  // NaN: inactive slots never branch
+    if (act) {
+        const float4 p = xs[t];
+        xi[0] = p.x; yi[0] = p.y; zi[0] = p.z;
+    }
+    Acc<1> a;
+    a.ax[0] = a.ay[0] = a.az[0] = a.phi[0] = 0.0f;
+    a.sx[0] = a.sy[0] = a.sz[0] = a.sphi[0] = CSum{0.0f, 0.0f};
+    a.cnt[0] = 0;
+    const int G0 = (I / kGroupBlocks) * kGroupBlocks, G1 = min(nb, G0 + kGroupBlocks);  // own group's blocks
+    const bool use_bits = NBR && bits;
+    const uint32_t* row = use_bits ? bits + (size_t)I * nwords : nullptr;
+    // the own group's blocks lie in one word (groups of 16 blocks are 16-aligned)
+    const int gw = G0 >> 5;
+    const uint32_t gmask = (uint32_t)(((1ull << (G1 - G0)) - 1) << (G0 - 32 * gw));
+    auto word_of = [&](int w) -> uint32_t {
+        uint32_t word = use_bits ? __builtin_amdgcn_readfirstlane(row[w]) : 0u;
+        if ((GRAV || !use_bits) && w == gw) word |= gmask;
+        return word;
+    };
+    const int w0 = use_bits ? 0 : gw, w1 = use_bits ? nwords : gw + 1;
+    int total = 0;
+    for (int w = w0; w < w1; ++w) total += __builtin_popcount(word_of(w));
+    const int r0 = (int)((int64_t)sl * total / ns), r1 = (int)((int64_t)(sl + 1) * total / ns);
+    int rank = 0;
+    for (int w = w0; w < w1 && rank < r1; ++w) {
+        uint32_t word = word_of(w);
+        const int c = __builtin_popcount(word);
+        if (rank + c <= r0) {
+            rank += c;
+            continue;
+        }
+        while (word && rank < r1) {
+            const int J = 32 * w + __builtin_ctz(word);
+            word &= word - 1;
+            if (rank++ < r0) continue;
+            const int j0 = 64 * J, j1 = min(n, j0 + 64);
+            if (GRAV && J >= G0 && J < G1)
+                sweep_range<1, GRAV, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
+            else
+                sweep_range<1, false, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
+        }
+    }
+    if (!act) return;
+    if (GRAV) {
+        near_part[2 * (size_t)t] = make_float4(a.sx[0].s, a.sy[0].s, a.sz[0].s, 0.0f);
+        near_part[2 * (size_t)t + 1] = make_float4(a.sx[0].c, a.sy[0].c, a.sz[0].c, 0.0f);
+    }
+    if (NBR) nbr_cnt[t] = a.cnt[0];
+}
+
+template <bool GRAV, bool NBR, bool SOFT>
+__global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__ xs, const int n, const int nb,
+                                                      const int nwords, const uint32_t* __restrict__ bits,
+                                                      int* __restrict__ nbr, int* __restrict__ nbr_cnt,
+                                                      const int kmax, float4* __restrict__ near_part,
+                                                      const DevParams P, const float* __restrict__ umass)
+{
+    const int I = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
+    const int lane = threadIdx.x & 63;
+    if (I >= nb) return;
+    // share blockIdx.y of gridDim.y: its list, count and partial planes
+    const int sl = blockIdx.y, ns = gridDim.y;
+    const Rare R = make_rare(nbr + (size_t)sl * kmax * n, (size_t)n, kmax, P.H2);
+    int* const cnt = nbr_cnt + (size_t)sl * n;
+    float4* const part = near_part + 2 * (size_t)sl * n;
+    if constexpr (GRAV) {
+        if (umass && umass[0] != 0.0f) {
+            near_blocks<GRAV, NBR, SOFT, true>(xs, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, umass[1]);
+            return;
+        }
+    }
+    near_blocks<GRAV, NBR, SOFT, false>(xs, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, 1.0f);
+}
+
+// Own slots of a lane in the far field: position, mass, plain and
+// compensated gravity sums.
+struct FarOwn {
+    float x[kNI], y[kNI], z[kNI], m[kNI];
+    float ax[kNI], ay[kNI], az[kNI];
+    CSum sx[kNI], sy[kNI], sz[kNI];
+};
+
+// Four partners (half a chunk) against the lane's kNI slots: own-side sums,
+// and the partner side summed over the slots into px/py/pz[H .. H + 3]
+// (UM: without the common mass; else with the slots' masses).
+template <int H, bool UM, bool SOFT>
+__device__ __forceinline__ void far_half(const float4 (&pj)[4], FarOwn& o, const float H2, float (&px)[kChunk],
+                                         float (&py)[kChunk], float (&pz)[kChunk])
+{
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int q = 0; q < kNI; ++q) {
+            const float dx = pj[c].x - o.x[q], dy = pj[c].y - o.y[q], dz = pj[c].z - o.z[q];
+            const float d2 =
+                SOFT ? fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, H2))) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float inv = __builtin_amdgcn_rsqf(d2);
+            float s = inv * inv;
+            s = s * inv;
+            if (!SOFT) s = (d2 > 0.0f) ? s : 0.0f;  // coincident particles exert nothing (computeDerivatives.h:226)
+            float sa = s, sb = s;
+            if (!UM) {
+                sa = s * pj[c].w;
+                sb = s * o.m[q];
+            }
+            o.ax[q] = fmaf(dx, sa, o.ax[q]); o.ay[q] = fmaf(dy, sa, o.ay[q]); o.az[q] = fmaf(dz, sa, o.az[q]);
+            if (q == 0) {
+                px[H + c] = dx * sb; py[H + c] = dy * sb; pz[H + c] = dz * sb;
+            } else {
+                px[H + c] = fmaf(dx, sb, px[H + c]); py[H + c] = fmaf(dy, sb, py[H + c]);
+                pz[H + c] = fmaf(dz, sb, pz[H + c]);
+            }
+        }
+    }
+}
+
+// partners b .. b + 3; BOUNDED: slots at or past n are far away and massless
+template <bool BOUNDED>
+__device__ __forceinline__ void load_partners(const float4* __restrict__ xs, const int b, const int n, float4 (&pj)[4])
+{
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (BOUNDED) {
+            const float4 v = xs[min(b + c, n - 1)];
+            pj[c] = (b + c < n) ? v : make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
+        } else {
+            pj[c] = xs[b + c];
+        }
+    }
+}
+
+// stage slot of this lane's reduced value, or -1
+__device__ __forceinline__ int stage_index(const int lane)
+{
+    const int comp = (lane >> 1) & 3;
+    return ((lane & 1) == 0 && comp < 3) ? (lane >> 3) * 3 + comp : -1;
+}
+
+// One partner super-block [b0, b0 + 256) against the wavefront's slots;
+// the reduced partner side of chunk q goes to my_stage[q * 24 + stg].
+template <bool UM, bool SOFT, bool BOUNDED>
+__device__ __forceinline__ void far_segment(const float4* __restrict__ xs, const int n, const int b0, FarOwn& o,
+                                            const float H2, const int lane, const int stg, float* my_stage)
+{
+    float4 A[4], B[4];
+    load_partners<BOUNDED>(xs, b0, n, A);
+    for (int q = 0; q < kSegChunks; ++q) {
+        float px[kChunk], py[kChunk], pz[kChunk];
+        load_partners<BOUNDED>(xs, b0 + kChunk * q + 4, n, B);
+        far_half<0, UM, SOFT>(A, o, H2, px, py, pz);
+        if (q + 1 < kSegChunks) load_partners<BOUNDED>(xs, b0 + kChunk * (q + 1), n, A);
+        far_half<4, UM, SOFT>(B, o, H2, px, py, pz);
+        const float r = reduce_partners(px, py, pz, lane);
+        if (stg >= 0) my_stage[q * kStage + stg] = r;
+    }
+}
+
+// Far field.  Workgroup k of W takes the units [k U / W, (k + 1) U / W) of
+// the rows g (group g against the later super-blocks) laid end to end;
+// wavefront w holds slots 1024 g + 256 w + 64 q + lane, q < 4.
+template <bool UM, bool SOFT>
+__device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
+                                         const int64_t U, const int W, float4* __restrict__ apart,
+                                         float* __restrict__ bpart, const float H2, const float m, float* stage)
+{
+    const int k = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
+    const int64_t u0 = (int64_t)k * U / W, u1 = (int64_t)(k + 1) * U / W;
+    if (u0 >= u1) return;  // the whole workgroup leaves: no barrier is reached
+    const int64_t npos = rows_with_units(nsb);
+    // row of u0: the largest g < npos with row_prefix(g) <= u0
+    int64_t lo = 0, hi = npos - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (row_prefix(mid, nsb) <= u0) lo = mid; else hi = mid - 1;
+    }
+    int64_t g = lo, s = u0 - row_prefix(g, nsb);
+    const int stg = stage_index(lane);
+    float* const my_stage = stage + (size_t)w * kSegChunks * kStage;
+    const bool last_partial = (n % kSB) != 0;
+
+    FarOwn o;
+    int sbA = 0;
+    bool wave_on = false;
+    auto load_row = [&]() {
+        sbA = (int)(g * kGroupSB) + w;
+        wave_on = sbA < nsb;
+#pragma unroll
+        for (int q = 0; q < kNI; ++q) {
+            const int t = kSB * sbA + 64 * q + lane;
+            o.x[q] = o.y[q] = o.z[q] = -3.0e18f;
+            o.m[q] = 0.0f;
+            if (t < n) {
+                const float4 p = xs[t];
+                o.x[q] = p.x; o.y[q] = p.y; o.z[q] = p.z; o.m[q] = p.w;
+            }
+            o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
+            o.sx[q] = o.sy[q] = o.sz[q] = CSum{0.0f, 0.0f};
+        }
+    };
+    auto flush_row = [&]() {
+        const size_t f = (size_t)(g + k);
+#pragma unroll
+        for (int q = 0; q < kNI; ++q) {
+            const size_t e = 2 * (f * kGroup + kSB * w + 64 * q + lane);
+            apart[e] = make_float4(o.sx[q].s, o.sy[q].s, o.sz[q].s, 0.0f);
+            apart[e + 1] = make_float4(o.sx[q].c, o.sy[q].c, o.sz[q].c, 0.0f);
+        }
+    };
+    load_row();
+    for (int64_t u = u0; u < u1; ++u) {
+        if (s == row_units(g, nsb)) {
+            flush_row();
+            ++g;
+            s = 0;
+            load_row();
+        }
+        const int J = (int)(kGroupSB * (g + 1) + s);  // partner super-block of this unit
+        if (!wave_on) {
+            if (stg >= 0)
+                for (int q = 0; q < kSegChunks; ++q) my_stage[q * kStage + stg] = 0.0f;
+        } else if (last_partial && J == nsb - 1) {
+            far_segment<UM, SOFT, true>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+        } else {
+            far_segment<UM, SOFT, false>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+        }
+        // own side: plain sums over the segment, then compensated
+#pragma unroll
+        for (int q = 0; q < kNI; ++q) {
+            if (UM) {
+                o.sx[q].add(o.ax[q] * m); o.sy[q].add(o.ay[q] * m); o.sz[q].add(o.az[q] * m);
+            } else {
+                o.sx[q].add(o.ax[q]); o.sy[q].add(o.ay[q]); o.sz[q].add(o.az[q]);
+            }
+            o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
+        }
+        __syncthreads();
+        // partner side: the 4 wavefronts' values in wavefront order
+#pragma unroll
+        for (int r = 0; r < kSegChunks * kStage / kSB; ++r) {
+            const int idx = tid + kSB * r;
+            float v = 0.0f;
+#pragma unroll
+            for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kSegChunks * kStage + idx];
+            const int q = idx / kStage, e = idx - q * kStage;
+            const int b = e / 3, comp = e - 3 * b;
+            const int B = kSB * J + kChunk * q + b;
+            if (B < n) {
+                const int64_t h = B / kGroup;
+                const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
+                bpart[3 * x + comp] = UM ? v * m : v;
+            }
+        }
+        __syncthreads();
+        ++s;
+    }
+    flush_row();
+}
+
+template <bool SOFT>
+__global__ __launch_bounds__(kGroupSB * 64) void far_kernel(const float4* __restrict__ xs, const int n,
+                                                            const int64_t nsb, const int64_t U, const int W,
+                                                            float4* __restrict__ apart, float* __restrict__ bpart,
+                                                            const float H2, const float* __restrict__ umass)
+{
+    __shared__ float stage[kGroupSB * kSegChunks * kStage];
+    if (umass[0] != 0.0f)
+        far_work<true, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, umass[1], stage);
+    else
+        far_work<false, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, 1.0f, stage);
+}
+
+// Per slot, in a fixed order: the near partials (share order), the far own-side fragments
+// (workgroup order), the far partner-side partials (group order, negated:
+// they were summed as seen from the other side), compensated; then the
+// acceleration is finished as the one-range sweep does.
+__global__ __launch_bounds__(kBlock) void sym_combine_kernel(
+    const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
+    const int n, const int nshares, const int64_t nsb, const int64_t U, const int W, const int grav,
+    const int* __restrict__ perm,
+    const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const float4* __restrict__ vel,
+    float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams P)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    CSum x{0.0f, 0.0f}, y{0.0f, 0.0f}, z{0.0f, 0.0f};
+    if (grav) {
+        for (int sl = 0; sl < nshares; ++sl) {
+            const size_t e = 2 * ((size_t)sl * n + t);
+            const float4 a = near_part[e], c = near_part[e + 1];
+            x.add(a.x); y.add(a.y); z.add(a.z);
+            x.c += c.x; y.c += c.y; z.c += c.z;
+        }
+        const int64_t g = t / kGroup, local = t - g * kGroup;
+        if (U > 0 && g < rows_with_units(nsb)) {
+            const int64_t k0 = wg_of_unit(row_prefix(g, nsb), U, W);
+            const int64_t k1 = wg_of_unit(row_prefix(g + 1, nsb) - 1, U, W);
+            for (int64_t k = k0; k <= k1; ++k) {
+                if (k * U / W == (k + 1) * U / W) continue;  // no share (never with W <= U)
+                const size_t e = 2 * ((size_t)(g + k) * kGroup + local);
+                const float4 f = apart[e], fc = apart[e + 1];
+                x.add(f.x); y.add(f.y); z.add(f.z);
+                x.c += fc.x; y.c += fc.y; z.c += fc.z;
+            }
+        }
+        const int64_t base = bpart_group_base(g) + local;
+        for (int64_t gp = 0; gp < g; ++gp) {
+            const size_t e = 3 * (size_t)(base + gp * kGroup);
+            x.add(-bpart[e]); y.add(-bpart[e + 1]); z.add(-bpart[e + 2]);
+        }
+    }
+    float ax = x.value(), ay = y.value(), az = z.value();
+    const int i = perm ? perm[t] : t;
+    if (finish) {
+        const float4 p = xs[t];
+        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        environment_acceleration(P, p.x, p.y, p.z, v.x, v.y, v.z, ax, ay, az);
+        if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (maxvsig) maxvsig[i] = 0.0f;
+    }
+    acc[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+template <bool GRAV, bool NBR, bool SOFT>
+void launch_near(const SymLaunch& L, const SymGeom& G, const DevParams& P, hipStream_t s)
+{
+    const int blocks = (G.nb + kBlock / 64 - 1) / (kBlock / 64);
+    hipLaunchKernelGGL((near_kernel<GRAV, NBR, SOFT>), dim3(blocks, L.nshares), dim3(kBlock), 0, s, L.xs, L.n, G.nb,
+                       G.nwords,
+                       L.want_nbr ? L.bits : nullptr, L.nbr, L.nbr_cnt, L.kmax, L.near_part, P, L.umass);
+}
+
+}  // namespace
+
+SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
+{
+    SymGeom G;
+    G.nb = (int)((n + 63) / 64);
+    G.nsb = (n + kSB - 1) / kSB;
+    G.groups = (int)((G.nsb + kGroupSB - 1) / kGroupSB);
+    G.units = row_prefix(rows_with_units(G.nsb), G.nsb);
+    // every workgroup gets at least one unit, so a row's fragments come from consecutive workgroups
+    G.wgs = (int)std::max<int64_t>(1, std::min<int64_t>(wgs, G.units));
+    G.nwords = (G.nb + 31) / 32;
+    G.box_elems = 2 * (int64_t)G.nb;
+    G.bits_elems = want_nbr ? (int64_t)G.nb * G.nwords : 0;
+    G.near_elems = 2 * n * nshares;
+    G.apart_elems = 2 * (int64_t)kGroup * (G.groups + wgs);
+    G.bpart_elems = 3 * bpart_group_base(G.groups);
+    return G;
+}
+
+hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0) return hipSuccess;
+    if (L.nshares < 1) return hipErrorInvalidValue;
+    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
+    const bool soft = P.H2 > 0.0f;
+    if (L.want_nbr) {
+        hipLaunchKernelGGL(block_box_kernel, dim3((G.nb + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, L.xs,
+                           L.n, G.nb, L.box);
+        const int64_t words = (int64_t)G.nb * G.nwords;
+        hipLaunchKernelGGL(near_bits_kernel, dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           L.box, G.nb, G.nwords, P.H2, L.bits);
+    }
+    if (L.grav && L.want_nbr) {
+        if (soft) launch_near<true, true, true>(L, G, P, s); else launch_near<true, true, false>(L, G, P, s);
+    } else if (L.grav) {
+        if (soft) launch_near<true, false, true>(L, G, P, s); else launch_near<true, false, false>(L, G, P, s);
+    } else if (L.want_nbr) {
+        if (soft) launch_near<false, true, true>(L, G, P, s); else launch_near<false, true, false>(L, G, P, s);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (L.grav && G.units > 0) {
+        if (soft)
+            hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kGroupSB * 64), 0, s, L.xs, L.n, G.nsb, G.units,
+                               L.wgs, L.apart, L.bpart, P.H2, L.umass);
+        else
+            hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kGroupSB * 64), 0, s, L.xs, L.n, G.nsb, G.units,
+                               L.wgs, L.apart, L.bpart, P.H2, L.umass);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (L.grav || L.finish) {
+        hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part,
+                           L.apart, L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
+                           L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+}  // namespace grasph
