@@ -179,6 +179,9 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
 // later particle): 6 n^2 / 1024 bytes.  Past this size the j-split sweep
 // runs instead.
 constexpr int64_t kSymMaxPartialBytes = int64_t(12) << 30;
+// From this size on the symmetric sweep is the faster one (measured: 9%
+// slower at 16k particles, 3% faster at 65k, 10-16% at 131k).
+constexpr int64_t kSymAutoMin = 49152;
 
 int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : 2 * c.cus; }
 
@@ -319,7 +322,8 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     // the order only matters for the neighbour branch: gravity alone runs in index order
     if (need_list && (r = ensure_order(c, p, s, true))) return r;
     const bool sweep_grav = accel_pass && grav;
-    const bool sym = c.sweep_symmetric && (sweep_grav || need_list) && sym_fits(c, n, need_list);
+    const bool sym_on = c.sweep_symmetric == 1 || (c.sweep_symmetric == 2 && n >= kSymAutoMin);
+    const bool sym = sym_on && (sweep_grav || need_list) && sym_fits(c, n, need_list);
     int slices = 1, slice_len = (int)n;
     if ((sweep_grav || need_list) && !sym) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
     if (sym) slices = sym_near_shares(c, n);  // the near field's shares take the slices' list planes
@@ -558,7 +562,7 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
         if (value < 0 || value > 64) return grasph::fail(c, GRASPH_E_INVALID, "sweep_slices must be in [0, 64]");
         c.sweep_slices = (int)value;
     } else if (k == "sweep_symmetric") {
-        if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "sweep_symmetric must be 0 or 1");
+        if (value < 0 || value > 2) return grasph::fail(c, GRASPH_E_INVALID, "sweep_symmetric must be 0, 1 or 2");
         c.sweep_symmetric = (int)value;
     } else if (k == "sym_workgroups") {
         if (value < 0 || value > 65536) return grasph::fail(c, GRASPH_E_INVALID, "sym_workgroups must be in [0, 65536]");

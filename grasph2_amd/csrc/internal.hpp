@@ -31,7 +31,7 @@ struct Ctx {
     int reorder_interval = 1;
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
-    int sweep_symmetric = 0;  // single instance: far pairs once for both sides (sym_sweep.hip)
+    int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 49152 particles
     int sym_workgroups = 0;   // workgroups of its far kernel (0: two per compute unit)
     int cus = 256;         // compute units of the device
     bool profile = false;

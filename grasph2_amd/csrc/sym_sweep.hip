@@ -218,7 +218,18 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
     int total = 0;
     for (int w = w0; w < w1; ++w) total += __builtin_popcount(word_of(w));
     const int r0 = (int)((int64_t)sl * total / ns), r1 = (int)((int64_t)(sl + 1) * total / ns);
-    int rank = 0;
+    // blocks of this share, consecutive ones of one kind (own group or not) swept as one range
+    int rank = 0, run0 = -1, run1 = -1;
+    bool run_grav = false;
+    auto flush = [&]() {
+        if (run0 < 0) return;
+        const int j0 = 64 * run0, j1 = min(n, 64 * run1);
+        if (GRAV && run_grav)
+            sweep_range<1, GRAV, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
+        else
+            sweep_range<1, false, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
+        run0 = -1;
+    };
     for (int w = w0; w < w1 && rank < r1; ++w) {
         uint32_t word = word_of(w);
         const int c = __builtin_popcount(word);
@@ -230,13 +241,16 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
             const int J = 32 * w + __builtin_ctz(word);
             word &= word - 1;
             if (rank++ < r0) continue;
-            const int j0 = 64 * J, j1 = min(n, j0 + 64);
-            if (GRAV && J >= G0 && J < G1)
-                sweep_range<1, GRAV, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
-            else
-                sweep_range<1, false, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
+            const bool g = J >= G0 && J < G1;
+            if (run0 >= 0 && (J != run1 || g != run_grav)) flush();
+            if (run0 < 0) {
+                run0 = J;
+                run_grav = g;
+            }
+            run1 = J + 1;
         }
     }
+    flush();
     if (!act) return;
     if (GRAV) {
         near_part[2 * (size_t)t] = make_float4(a.sx[0].s, a.sy[0].s, a.sz[0].s, 0.0f);
@@ -311,20 +325,30 @@ __device__ __forceinline__ void far_half(const float4 (&pj)[4], FarOwn& o, const
     }
 }
 
-// partners b .. b + 3; BOUNDED: slots at or past n are far away and massless
+// partners b .. b + 3; BOUNDED: slots at or past n are far away and
+// massless; else b is clamped to n - 4 (a look-ahead past the end loads
+// values nobody uses)
 template <bool BOUNDED>
 __device__ __forceinline__ void load_partners(const float4* __restrict__ xs, const int b, const int n, float4 (&pj)[4])
 {
+    if (BOUNDED) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        if (BOUNDED) {
+        for (int c = 0; c < 4; ++c) {
             const float4 v = xs[min(b + c, n - 1)];
             pj[c] = (b + c < n) ? v : make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
-        } else {
-            pj[c] = xs[b + c];
         }
+    } else {
+        const int bb = min(b, n - 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pj[c] = xs[bb + c];
     }
 }
+
+// s_waitcnt lgkmcnt(0), vmcnt and expcnt left free: scalar loads may
+// return out of order, so waiting for one waits for all; doing it at the
+// end of each half keeps the next half's load in flight through a whole
+// half of compute.
+__device__ __forceinline__ void wait_scalar_loads() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 // stage slot of this lane's reduced value, or -1
 __device__ __forceinline__ int stage_index(const int lane)
@@ -341,14 +365,19 @@ __device__ __forceinline__ void far_segment(const float4* __restrict__ xs, const
 {
     float4 A[4], B[4];
     load_partners<BOUNDED>(xs, b0, n, A);
+    wait_scalar_loads();
     for (int q = 0; q < kSegChunks; ++q) {
         float px[kChunk], py[kChunk], pz[kChunk];
         load_partners<BOUNDED>(xs, b0 + kChunk * q + 4, n, B);
+        __builtin_amdgcn_sched_barrier(0);  // issue the load before the half it overlaps
         far_half<0, UM, SOFT>(A, o, H2, px, py, pz);
-        if (q + 1 < kSegChunks) load_partners<BOUNDED>(xs, b0 + kChunk * (q + 1), n, A);
+        wait_scalar_loads();
+        load_partners<BOUNDED>(xs, b0 + kChunk * (q + 1), n, A);  // past the segment on the last chunk: unused
+        __builtin_amdgcn_sched_barrier(0);
         far_half<4, UM, SOFT>(B, o, H2, px, py, pz);
         const float r = reduce_partners(px, py, pz, lane);
         if (stg >= 0) my_stage[q * kStage + stg] = r;
+        wait_scalar_loads();
     }
 }
 

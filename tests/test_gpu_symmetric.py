@@ -94,6 +94,21 @@ def test_standalone_density_pass(gpu_ctx, sym_ctx):
     assert np.array_equal(out["density"], one["density"])
 
 
+def test_list_overflow_scan_is_bitwise_equal(sym_ctx):
+    """Lists that overflow fall back to the exact scan of the whole instance,
+    which walks the same partners in the same order."""
+    print("symmetric list overflow", flush=True)
+    h, prm = solid_case(n=9000, seed=14)
+    base = gpu_run(sym_ctx, h, prm)
+    try:
+        sym_ctx.set_option("max_neighbours", 8)
+        small = gpu_run(sym_ctx, h, prm)
+    finally:
+        sym_ctx.set_option("max_neighbours", 160)
+    for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
+        assert np.array_equal(base[f], small[f]), f
+
+
 def test_deterministic(sym_ctx):
     print("symmetric determinism", flush=True)
     h, prm = solid_case(n=9000, seed=13)
