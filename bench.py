@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (AMD published)
 PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (AMD published)
+# The library's default for the single-instance sweep (option "sweep_symmetric" = 2, include/grasph2.h):
+# from this many particles on, gravity between groups of 1024 is evaluated once per pair for both sides.
+# The roofline stays algorithmic either way: 20 FLOP per ordered pair.
+SYM_AUTO_MIN = 49152
 FLOP_GRAVITY_PAIR = 20
 FLOP_DENSITY_NBR = 9
 FLOP_STRAIN_NBR = 63
@@ -191,7 +195,8 @@ def main():
                    "dt": prm.fixed_timestep, "fp64_drift": True},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "sweep", "kernel_ms": sweep_ms, "flops_per_launch": sweep_flops},
+                     "kernel": "sweep", "kernel_ms": sweep_ms, "flops_per_launch": sweep_flops,
+                     "sweep": "symmetric" if n >= SYM_AUTO_MIN else "one-sided"},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1]},
         "step_tflops": step_flops / (ms_step * 1e-3) / 1e12,
         "neighbour_pairs": nb_pairs,

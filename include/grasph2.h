@@ -148,7 +148,12 @@ int grasph_destroy(grasph_ctx* ctx);
  * "batch_waves_per_block", "batch_blocks_per_cu" (batched sweep launch
  * shape), "timestep_reset" (1 = the variable-step leapfrog starts each
  * step's dt minimum from max_timestep, so dt can grow again; 0 = the
- * reference's running minimum, the default). */
+ * reference's running minimum, the default), "sweep_symmetric" (single
+ * instance: 1 = gravity between groups of 1024 particles evaluated once per
+ * pair for both sides, 0 = the one-sided sweep, 2 = the symmetric sweep from
+ * 49152 particles on, the default; results agree to FP32 rounding, the
+ * neighbour lists are identical), "sym_workgroups" (workgroups of the
+ * symmetric sweep's far-field kernel, 0 = two per compute unit). */
 int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
 
 /* ---- the hot path ------------------------------------------------------- */
