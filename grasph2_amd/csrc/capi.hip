@@ -183,7 +183,7 @@ constexpr int64_t kSymMaxPartialBytes = int64_t(12) << 30;
 // slower at 16k particles, 3% faster at 65k, 10-16% at 131k).
 constexpr int64_t kSymAutoMin = 49152;
 
-int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : 2 * c.cus; }
+int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : sym_default_workgroups(c.cus); }
 
 bool sym_fits(const Ctx& c, int64_t n, bool want_nbr)
 {
@@ -194,6 +194,7 @@ bool sym_fits(const Ctx& c, int64_t n, bool want_nbr)
 // shares of the symmetric sweep's near field: ~16 wavefronts per SIMD, at most 16
 int sym_near_shares(const Ctx& c, int64_t n)
 {
+    if (c.sym_near_shares > 0) return c.sym_near_shares;
     const int64_t blocks = (n + 63) / 64;
     const int64_t target = 16 * 4 * (int64_t)c.cus;
     return (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + blocks - 1) / blocks));
@@ -567,6 +568,9 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sym_workgroups") {
         if (value < 0 || value > 65536) return grasph::fail(c, GRASPH_E_INVALID, "sym_workgroups must be in [0, 65536]");
         c.sym_workgroups = (int)value;
+    } else if (k == "sym_near_shares") {
+        if (value < 0 || value > 16) return grasph::fail(c, GRASPH_E_INVALID, "sym_near_shares must be in [0, 16]");
+        c.sym_near_shares = (int)value;
     } else if (k == "batch_waves_per_block") {
         if (value != 1 && value != 2 && value != 4)
             return grasph::fail(c, GRASPH_E_INVALID, "batch_waves_per_block must be 1, 2 or 4");

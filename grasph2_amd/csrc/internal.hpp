@@ -32,7 +32,8 @@ struct Ctx {
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
     int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 49152 particles
-    int sym_workgroups = 0;   // workgroups of its far kernel (0: two per compute unit)
+    int sym_workgroups = 0;   // workgroups of its far kernel (0: 2 wavefronts per SIMD)
+    int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
     int cus = 256;         // compute units of the device
     bool profile = false;
     // batched sweep tuning: i per lane per size bucket, persistent launch shape
@@ -156,6 +157,7 @@ struct SymGeom {
     int64_t bpart_elems;  // float: far partner-side partials
 };
 SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares);
+int sym_default_workgroups(int cus);  // far-field workgroups for 2 wavefronts per SIMD
 struct SymLaunch {
     const float4* xs;   // slot-ordered positions + masses
     const int* perm;    // slot -> particle, or null

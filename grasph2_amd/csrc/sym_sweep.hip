@@ -39,24 +39,31 @@
 namespace grasph {
 namespace {
 
-constexpr int kSB = 256;          // slots per super-block
-constexpr int kGroupSB = 4;       // super-blocks per group (wavefronts per far workgroup)
-constexpr int kGroup = kSB * kGroupSB;  // slots per group (1024)
-constexpr int kNI = kSB / 64;     // own slots per lane in the far field (4)
-constexpr int kSegChunks = kSB / kChunk;  // chunks of 8 partners per far segment (one super-block, 32)
+// own slots per lane in the far field (8 was measured 12% slower: the
+// registers cap it at 2 wavefronts per SIMD with half as many workgroups'
+// worth of slots in flight)
+constexpr int kNI = 4;
+constexpr int kSB = 64 * kNI;     // slots per super-block (a far wavefront's slots)
+constexpr int kGroup = 1024;      // slots per group
+constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)
+constexpr int kFarThreads = 64 * kGroupSB;
+constexpr int kSegChunks = kSB / kChunk;  // chunks of 8 partners per far segment (one super-block)
 constexpr int kStage = 24;        // partner values per chunk: 8 partners x 3 components
 
 // ------------------------------------------------------------------ geometry of the work
 
-// far units (partner super-blocks) of row g: super-blocks [4 (g + 1), nsb)
+// far units (partner super-blocks) of row g: super-blocks [kGroupSB (g + 1), nsb)
 __device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb)
 {
     const int64_t u = nsb - (int64_t)kGroupSB * (g + 1);
     return u > 0 ? u : 0;
 }
-// units before row g (g <= rows_with_units): sum over g' < g of (nsb - 4 (g' + 1))
-__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb) { return g * nsb - 2 * g * (g + 1); }
-// rows with far work: nsb - 4 (g + 1) >= 1
+// units before row g (g <= rows_with_units): sum over g' < g of (nsb - kGroupSB (g' + 1))
+__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb)
+{
+    return g * nsb - (int64_t)kGroupSB * (g * (g + 1) / 2);
+}
+// rows with far work: nsb - kGroupSB (g + 1) >= 1
 __device__ __host__ inline int64_t rows_with_units(int64_t nsb) { return nsb >= 1 ? (nsb - 1) / kGroupSB : 0; }
 // the workgroup whose share [k U / W, (k + 1) U / W) holds unit u
 __device__ __host__ inline int64_t wg_of_unit(int64_t u, int64_t U, int64_t W) { return ((u + 1) * W - 1) / U; }
@@ -461,10 +468,10 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
             o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
         }
         __syncthreads();
-        // partner side: the 4 wavefronts' values in wavefront order
+        // partner side: the wavefronts' values in wavefront order
 #pragma unroll
-        for (int r = 0; r < kSegChunks * kStage / kSB; ++r) {
-            const int idx = tid + kSB * r;
+        for (int r = 0; r < kSegChunks * kStage / kFarThreads; ++r) {
+            const int idx = tid + kFarThreads * r;
             float v = 0.0f;
 #pragma unroll
             for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kSegChunks * kStage + idx];
@@ -484,7 +491,7 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
 }
 
 template <bool SOFT>
-__global__ __launch_bounds__(kGroupSB * 64) void far_kernel(const float4* __restrict__ xs, const int n,
+__global__ __launch_bounds__(kFarThreads) void far_kernel(const float4* __restrict__ xs, const int n,
                                                             const int64_t nsb, const int64_t U, const int W,
                                                             float4* __restrict__ apart, float* __restrict__ bpart,
                                                             const float H2, const float* __restrict__ umass)
@@ -558,6 +565,11 @@ void launch_near(const SymLaunch& L, const SymGeom& G, const DevParams& P, hipSt
 
 }  // namespace
 
+int sym_default_workgroups(int cus)
+{
+    return std::max(1, 8 * cus / kGroupSB);  // 2 wavefronts per SIMD
+}
+
 SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
 {
     SymGeom G;
@@ -600,10 +612,10 @@ hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t 
     if (e != hipSuccess) return e;
     if (L.grav && G.units > 0) {
         if (soft)
-            hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kGroupSB * 64), 0, s, L.xs, L.n, G.nsb, G.units,
+            hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units,
                                L.wgs, L.apart, L.bpart, P.H2, L.umass);
         else
-            hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kGroupSB * 64), 0, s, L.xs, L.n, G.nsb, G.units,
+            hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units,
                                L.wgs, L.apart, L.bpart, P.H2, L.umass);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

@@ -153,7 +153,9 @@ int grasph_destroy(grasph_ctx* ctx);
  * pair for both sides, 0 = the one-sided sweep, 2 = the symmetric sweep from
  * 49152 particles on, the default; results agree to FP32 rounding, the
  * neighbour lists are identical), "sym_workgroups" (workgroups of the
- * symmetric sweep's far-field kernel, 0 = two per compute unit). */
+ * symmetric sweep's far-field kernel, 0 = two wavefronts per SIMD),
+ * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
+ * 0 = chosen from the occupancy). */
 int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
 
 /* ---- the hot path ------------------------------------------------------- */
