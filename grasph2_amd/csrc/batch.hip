@@ -35,6 +35,7 @@ struct BatchState {
     uint32_t flags = 0;
     int tuned_ni[4] = {0, 0, 0, 0};  // i per lane the items table was built with
     int items_ni_max = 1;            // largest ni among the items
+    int64_t items_sym_min = -1;      // instances from this size on were left out of the items (-1: none)
 
     // device tables
     BatchInst* d_insts = nullptr;
@@ -49,6 +50,19 @@ struct BatchState {
     float* d_um = nullptr;  // {uniform, mass} per instance, written before each gravity sweep
     // slot-ordered positions for the sweep (SPH with a Morton order)
     float4* d_xs = nullptr;
+    // symmetric sweep (option batch_symmetric): tables and work arrays
+    SymInst* d_sinst = nullptr;
+    SymRow* d_srows = nullptr;
+    int n_srows = 0;
+    int4* d_sblk = nullptr;  // 256-slot blocks of the instances in the symmetric sweep
+    int n_sblk = 0;
+    int64_t sym_min = -1;    // the size threshold the tables were built with (-1: none)
+    int sym_max_words = 0;
+    float4* d_sbox = nullptr;
+    uint32_t* d_sbits = nullptr;
+    float4* d_snear = nullptr;
+    float4* d_sapart = nullptr;
+    float* d_sbpart = nullptr;
 
     // Morton order
     float* d_bbox = nullptr;
@@ -103,8 +117,15 @@ void drop_graph(BatchState& b)
 void free_tables(BatchState& b)
 {
     void* ps[] = {b.d_insts, b.d_prm, b.d_inst_of, b.d_items, b.d_blk, b.d_queue, b.d_bbox, b.d_keys, b.d_keys_alt,
-                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um, b.d_xs};
+                  b.d_vals, b.d_perm, b.d_sort_tmp, b.d_ts, b.d_um, b.d_xs, b.d_sinst, b.d_srows, b.d_sbox,
+                  b.d_sbits, b.d_snear, b.d_sapart, b.d_sbpart, b.d_sblk};
     for (void* p : ps) free_ptr(p);
+    b.d_sinst = nullptr; b.d_srows = nullptr; b.d_sbox = nullptr; b.d_sbits = nullptr; b.d_snear = nullptr;
+    b.d_sapart = nullptr; b.d_sbpart = nullptr; b.d_sblk = nullptr;
+    b.n_srows = 0;
+    b.n_sblk = 0;
+    b.sym_min = -1;
+    b.sym_max_words = 0;
     b.d_insts = nullptr; b.d_prm = nullptr; b.d_inst_of = nullptr; b.d_items = nullptr; b.d_blk = nullptr;
     b.d_queue = nullptr; b.d_bbox = nullptr; b.d_keys = nullptr; b.d_keys_alt = nullptr; b.d_vals = nullptr;
     b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr; b.d_um = nullptr;
@@ -141,13 +162,16 @@ int build_items(Ctx& c, BatchState& b)
     });
     std::vector<int4> items;
     b.items_ni_max = 1;
+    const int64_t sym_min = c.batch_symmetric ? c.batch_symmetric_min : -1;
     for (int k : order) {
         const int64_t off = b.offsets[k], n = b.offsets[k + 1] - off;
+        if (sym_min >= 0 && n >= sym_min) continue;  // swept by the symmetric sweep
         const int ni = c.batch_ni[bucket_of(n)];
         if (n > 0) b.items_ni_max = std::max(b.items_ni_max, ni);
         for (int64_t s = off; s < off + n; s += 64 * ni) items.push_back(make_int4(k, (int)s, ni, 0));
     }
     b.n_items = (int)items.size();
+    b.items_sym_min = sym_min;
     for (int q = 0; q < 4; ++q) b.tuned_ni[q] = c.batch_ni[q];
     drop_graph(b);
     return upload(c, b.d_items, items, "batch work items");
@@ -171,6 +195,8 @@ int validate(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
     }
     return 0;
 }
+
+int ensure_sym(Ctx& c, BatchState& b);
 
 // (Re)build the tables when the batch layout or parameters changed.
 int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
@@ -233,7 +259,8 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
         }
         b.tuned_ni[0] = 0;  // force the items table
     }
-    if (std::memcmp(b.tuned_ni, c.batch_ni, sizeof b.tuned_ni) != 0 || !b.d_items)
+    const int64_t want_sym_min = c.batch_symmetric ? c.batch_symmetric_min : -1;
+    if (std::memcmp(b.tuned_ni, c.batch_ni, sizeof b.tuned_ni) != 0 || !b.d_items || b.items_sym_min != want_sym_min)
         if ((r = build_items(c, b))) return r;
     // lists and records
     const int64_t need = (int64_t)c.max_neighbours * b.ntotal;
@@ -258,6 +285,8 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
         if ((r = grow(c, b.d_rec, cap, b.ntotal, "batch slot records"))) return r;
         b.rec_cap = cap;
     }
+    // the symmetric sweep's tables, here and not at first use: a step may be under graph capture then
+    if (c.batch_symmetric && (b.flags & (GRASPH_SPH | GRASPH_SELF_GRAVITY)) && (r = ensure_sym(c, b))) return r;
     if (!b.gs) {
         if (hipStreamCreateWithFlags(&b.gs, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b.ev_in, hipEventDisableTiming) != hipSuccess ||
@@ -378,6 +407,97 @@ int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStrea
     return 0;
 }
 
+// tables and work arrays of the symmetric sweep, built once per batch layout
+int ensure_sym(Ctx& c, BatchState& b)
+{
+    if (b.d_sinst && b.sym_min == c.batch_symmetric_min) return 0;
+    const SymBatchTables T = sym_batch_tables(b.offsets, (b.flags & GRASPH_SPH) != 0, c.batch_symmetric_min);
+    int r;
+    free_ptr(b.d_sinst); free_ptr(b.d_srows); free_ptr(b.d_sblk);
+    b.d_sinst = nullptr; b.d_srows = nullptr; b.d_sblk = nullptr;
+    if ((r = upload(c, b.d_sinst, T.insts, "batch symmetric-sweep instances")) ||
+        (r = upload(c, b.d_srows, T.rows, "batch symmetric-sweep rows")) ||
+        (r = upload(c, b.d_sblk, T.blk, "batch symmetric-sweep blocks")))
+        return r;
+    b.n_srows = (int)T.rows.size();
+    b.n_sblk = (int)T.blk.size();
+    b.sym_min = c.batch_symmetric_min;
+    b.sym_max_words = T.max_words;
+    // fresh arrays for the new tables (upload()-style: free, then allocate)
+    void* old[] = {b.d_sbox, b.d_sbits, b.d_snear, b.d_sapart, b.d_sbpart};
+    for (void* q : old) free_ptr(q);
+    b.d_sbox = nullptr; b.d_sbits = nullptr; b.d_snear = nullptr; b.d_sapart = nullptr; b.d_sbpart = nullptr;
+    int64_t cap = 0;
+    if ((r = grow(c, b.d_sbox, cap, std::max<int64_t>(T.box_elems, 1), "batch symmetric-sweep boxes"))) return r;
+    cap = 0;
+    if ((r = grow(c, b.d_sbits, cap, std::max<int64_t>(T.bits_elems, 1), "batch symmetric-sweep near bits"))) return r;
+    cap = 0;
+    if ((r = grow(c, b.d_snear, cap, std::max<int64_t>(2 * b.ntotal, 1), "batch symmetric-sweep near sums"))) return r;
+    cap = 0;
+    if ((r = grow(c, b.d_sapart, cap, std::max<int64_t>(T.apart_elems, 1), "batch symmetric-sweep own sums"))) return r;
+    cap = 0;
+    if ((r = grow(c, b.d_sbpart, cap, std::max<int64_t>(T.bpart_elems, 1), "batch symmetric-sweep partner sums")))
+        return r;
+    drop_graph(b);
+    return 0;
+}
+
+// the symmetric sweep over the batch's larger instances (after
+// launch_sweep_only, which has then run the mass check: mass_done)
+int launch_sym_sweep_batch(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, bool mass_done)
+{
+    int r;
+    if ((r = ensure_sym(c, b))) return r;
+    const uint32_t f = b.flags;
+    SymBatchLaunch L;
+    std::memset(&L, 0, sizeof L);
+    L.perm = slot_order(c, b);
+    L.xs = L.perm ? b.d_xs : (const float4*)p->posm;
+    L.ntotal = (int)b.ntotal;
+    L.n_inst = b.n_inst;
+    L.grav = f & GRASPH_SELF_GRAVITY;
+    L.want_nbr = f & GRASPH_SPH;
+    L.finish = false;  // the finishing launch of the batch follows
+    bool soft = true;
+    for (const auto& q : b.params) soft = soft && q.H > 0.0f;
+    L.soft = soft;
+    L.sinst = b.d_sinst;
+    L.insts = b.d_insts;
+    L.prm = b.d_prm;
+    L.blk = b.d_sblk;
+    L.n_blk = b.n_sblk;
+    L.inst_of = b.d_inst_of;
+    L.rows = b.d_srows;
+    L.nrows = b.n_srows;
+    L.max_words = b.sym_max_words;
+    L.queue = b.d_queue + 1;
+    L.wgs = sym_default_workgroups(device_cus(c));
+    L.acc = (float4*)p->acc;
+    L.nbr = b.d_nbr;
+    L.nbr_cnt = b.d_cnt;
+    L.kmax = b.kmax;
+    L.vel = (const float4*)p->vel;
+    L.xvel = (float4*)p->xvel;
+    L.maxvsig = p->maxvsig;
+    L.box = b.d_sbox;
+    L.bits = b.d_sbits;
+    L.near_part = b.d_snear;
+    L.apart = b.d_sapart;
+    L.bpart = b.d_sbpart;
+    hipError_t e;
+    if (L.grav) {
+        if (!mass_done) {
+            e = launch_mass_check((const float4*)p->posm, b.d_insts, b.n_inst, 0, b.d_um, s);
+            if (e != hipSuccess) return hip_fail(c, e, "batch mass check launch");
+        }
+        L.umass = b.d_um;
+    }
+    ProfScope ps(&c, s, KC_SWEEP);
+    e = launch_sym_batch(L, s);
+    if (e != hipSuccess) return hip_fail(c, e, "batched symmetric sweep launch");
+    return 0;
+}
+
 // derivative passes of all instances (the order must be current)
 int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
 {
@@ -388,7 +508,12 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     const DevParams P = make_dev_params(&b.params[0]);  // flags; values come from the per-instance table
     int r;
     if ((r = batch_gather(c, b, p, s))) return r;
-    if ((sph || grav) && (r = launch_sweep_only(c, b, p, s))) return r;
+    if (sph || grav) {
+        // the one-sided persistent sweep takes the items (with the symmetric
+        // sweep on: the smaller instances), the symmetric sweep the rest
+        if (b.n_items > 0 && (r = launch_sweep_only(c, b, p, s))) return r;
+        if (c.batch_symmetric && (r = launch_sym_sweep_batch(c, b, p, s, b.n_items > 0))) return r;
+    }
 
     const int* perm = slot_order(c, b);
     NbrLaunch N;
@@ -490,7 +615,8 @@ std::string graph_key(const Ctx& c, const BatchState& b, const grasph_particles*
       << p->density_dt << ' ' << p->balsara << ' ' << p->maxvsig << ' ' << p->dstress << ' ' << p->dstress_dt << ' '
       << (c.reorder_interval > 0) << ' ' << c.batch_wpb << ' ' << c.batch_bpc << ' ' << (const void*)b.d_nbr << ' '
       << (const void*)b.d_rec << ' ' << (const void*)b.d_items << ' ' << (const void*)b.d_xs << ' '
-      << c.timestep_reset;
+      << c.timestep_reset << ' ' << c.batch_symmetric << ' ' << c.batch_symmetric_min << ' '
+      << (const void*)b.d_sinst << ' ' << (const void*)b.d_sbpart << ' ' << (const void*)b.d_sblk;
     return k.str();
 }
 

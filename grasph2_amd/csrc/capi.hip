@@ -568,6 +568,13 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sym_workgroups") {
         if (value < 0 || value > 65536) return grasph::fail(c, GRASPH_E_INVALID, "sym_workgroups must be in [0, 65536]");
         c.sym_workgroups = (int)value;
+    } else if (k == "batch_symmetric") {
+        if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "batch_symmetric must be 0 or 1");
+        c.batch_symmetric = (int)value;
+    } else if (k == "batch_symmetric_min") {
+        if (value < 0 || value > (int64_t)1 << 30)
+            return grasph::fail(c, GRASPH_E_INVALID, "batch_symmetric_min must be in [0, 2^30]");
+        c.batch_symmetric_min = (int)value;
     } else if (k == "sym_near_shares") {
         if (value < 0 || value > 16) return grasph::fail(c, GRASPH_E_INVALID, "sym_near_shares must be in [0, 16]");
         c.sym_near_shares = (int)value;

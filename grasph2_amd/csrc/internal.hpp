@@ -34,6 +34,8 @@ struct Ctx {
     int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 49152 particles
     int sym_workgroups = 0;   // workgroups of its far kernel (0: 2 wavefronts per SIMD)
     int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
+    int batch_symmetric = 0;  // batches: the symmetric sweep for instances of batch_symmetric_min particles or more
+    int batch_symmetric_min = 0;
     int cus = 256;         // compute units of the device
     bool profile = false;
     // batched sweep tuning: i per lane per size bucket, persistent launch shape
@@ -181,6 +183,60 @@ struct SymLaunch {
     float* bpart;
 };
 hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t s);
+
+// The symmetric sweep over a batch (slot indices are the batch's).
+struct BatchInst;
+struct SymInst {
+    int off, n;         // slots [off, off + n)
+    int nb, nwords;     // blocks of 64; words per near-bit row
+    int nsb, nrows;     // super-blocks; far rows (groups with later super-blocks)
+    int box0, row0;     // its first block box and first far row in the batch's arrays
+    int64_t bits0;      // first word of its near-bit matrix
+    int64_t bpart0;     // first float of its partner-side partials
+};
+struct SymRow {
+    int inst, g, units, pad;  // far row g of instance inst: units partner super-blocks
+};
+struct SymBatchTables {
+    std::vector<SymInst> insts;  // every instance; those below min_n have n = 0
+    std::vector<SymRow> rows;    // longest first
+    std::vector<int4> blk;       // 256-slot blocks (instance, first, end) of the instances in the sweep
+    int max_words = 0;
+    int64_t box_elems = 0, bits_elems = 0, apart_elems = 0, bpart_elems = 0;
+};
+// instances of at least min_n particles take the symmetric sweep
+SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_nbr, int64_t min_n);
+struct SymBatchLaunch {
+    const float4* xs;     // slot-ordered positions + masses of the batch
+    const int* perm;      // slot -> particle, or null
+    int ntotal, n_inst;
+    bool grav, want_nbr, finish, soft;
+    const SymInst* sinst;
+    const BatchInst* insts;
+    const DevParams* prm;
+    const int4* blk;      // 256-slot blocks of the instances in the sweep
+    int n_blk;
+    const int* inst_of;
+    const SymRow* rows;
+    int nrows;
+    int max_words;
+    int* queue;           // one work counter
+    int wgs;
+    const float* umass;   // {uniform, mass} per instance
+    float4* acc;
+    int* nbr;
+    int* nbr_cnt;
+    int kmax;
+    const float4* vel;
+    float4* xvel;
+    float* maxvsig;
+    float4* box;
+    uint32_t* bits;
+    float4* near_part;
+    float4* apart;
+    float* bpart;
+};
+hipError_t launch_sym_batch(const SymBatchLaunch& L, hipStream_t s);
 
 // Fold the per-slice lists of a j-split sweep into one list per particle
 // (slice 0's arrays; a count above kmax means "scan the instance").

@@ -33,6 +33,7 @@
 // ~12 for one interaction in the one-sided sweep.
 // Every sum is in a fixed order, so results do not vary from run to run.
 #include <algorithm>
+#include <vector>
 
 #include "sweep_core.hpp"
 
@@ -183,7 +184,8 @@ __global__ __launch_bounds__(kBlock) void near_bits_kernel(const float4* __restr
     bits[idx] = word;
 }
 
-// Near field of the 64 slots of block I, share `sl` of `ns`: the partner
+// Near field of the 64 slots of block I of the instance whose slots are
+// [base, base + n), share `sl` of `ns`: the partner
 // blocks are the blocks of its group (gravity, and the distance test when
 // lists are wanted) and, with bits, the other blocks near it (distance test
 // only); in increasing order they are cut into ns runs of equal count, and
@@ -191,15 +193,15 @@ __global__ __launch_bounds__(kBlock) void near_bits_kernel(const float4* __restr
 // partial (slot-indexed, two float4) and its list and count -- all in the
 // planes of share sl, which later fold in share order.
 template <bool GRAV, bool NBR, bool SOFT, bool UM>
-__device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const int n, const int nb,
+__device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const int base, const int n, const int nb,
                                             const int nwords, const uint32_t* __restrict__ bits, const int I,
                                             const int sl, const int ns, const int lane, int* __restrict__ nbr_cnt,
                                             float4* __restrict__ near_part, const DevParams& P, const Rare& R,
                                             const float m)
 {
     constexpr int kGroupBlocks = kGroup / 64;  // 16
-    const int t = 64 * I + lane;
-    const bool act = t < n;
+    const int t = base + 64 * I + lane;  // slots of the instance: [base, base + n)
+    const bool act = 64 * I + lane < n;
     float xi[1], yi[1], zi[1];
     xi[0] = yi[0] = zi[0] = __int_as_float(0x7fc00000);  // NaN: inactive slots never branch
     if (act) {
@@ -230,7 +232,7 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
     bool run_grav = false;
     auto flush = [&]() {
         if (run0 < 0) return;
-        const int j0 = 64 * run0, j1 = min(n, 64 * run1);
+        const int j0 = base + 64 * run0, j1 = base + min(n, 64 * run1);
         if (GRAV && run_grav)
             sweep_range<1, GRAV, NBR, false, SOFT, UM>(xs, j0, j1, xi, yi, zi, a, t, P.H2, P.cand, R, m);
         else
@@ -283,11 +285,11 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
     float4* const part = near_part + 2 * (size_t)sl * n;
     if constexpr (GRAV) {
         if (umass && umass[0] != 0.0f) {
-            near_blocks<GRAV, NBR, SOFT, true>(xs, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, umass[1]);
+            near_blocks<GRAV, NBR, SOFT, true>(xs, 0, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, umass[1]);
             return;
         }
     }
-    near_blocks<GRAV, NBR, SOFT, false>(xs, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, 1.0f);
+    near_blocks<GRAV, NBR, SOFT, false>(xs, 0, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, 1.0f);
 }
 
 // Own slots of a lane in the far field: position, mass, plain and
@@ -388,9 +390,93 @@ __device__ __forceinline__ void far_segment(const float4* __restrict__ xs, const
     }
 }
 
-// Far field.  Workgroup k of W takes the units [k U / W, (k + 1) U / W) of
-// the rows g (group g against the later super-blocks) laid end to end;
-// wavefront w holds slots 1024 g + 256 w + 64 q + lane, q < 4.
+// The own slots of wavefront w in row g (slots 1024 g + 256 w + 64 q + lane,
+// q < 4) of an instance of n particles at xs; slots past n sit far away and
+// massless.  Returns whether the wavefront has any slot.
+__device__ __forceinline__ bool far_load_row(const float4* __restrict__ xs, const int n, const int64_t nsb,
+                                             const int64_t g, const int w, const int lane, FarOwn& o)
+{
+    const int sbA = (int)(g * kGroupSB) + w;
+#pragma unroll
+    for (int q = 0; q < kNI; ++q) {
+        const int t = kSB * sbA + 64 * q + lane;
+        o.x[q] = o.y[q] = o.z[q] = -3.0e18f;
+        o.m[q] = 0.0f;
+        if (t < n) {
+            const float4 p = xs[t];
+            o.x[q] = p.x; o.y[q] = p.y; o.z[q] = p.z; o.m[q] = p.w;
+        }
+        o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
+        o.sx[q] = o.sy[q] = o.sz[q] = CSum{0.0f, 0.0f};
+    }
+    return sbA < nsb;
+}
+
+// own-side compensated sums of the row into fragment f (1024 slots, two float4 each)
+__device__ __forceinline__ void far_flush(float4* __restrict__ apart, const size_t f, const int w, const int lane,
+                                          const FarOwn& o)
+{
+#pragma unroll
+    for (int q = 0; q < kNI; ++q) {
+        const size_t e = 2 * (f * kGroup + kSB * w + 64 * q + lane);
+        apart[e] = make_float4(o.sx[q].s, o.sy[q].s, o.sz[q].s, 0.0f);
+        apart[e + 1] = make_float4(o.sx[q].c, o.sy[q].c, o.sz[q].c, 0.0f);
+    }
+}
+
+// One far unit of row g: partner super-block J against the workgroup's
+// 1024 slots.  The own side is folded into the compensated sums; the
+// partner side, summed over the wavefronts in wavefront order, is stored
+// as row g's partial of each partner (bpart: the instance's partials).
+// Called by the whole workgroup (two barriers).
+template <bool UM, bool SOFT>
+__device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t nsb,
+                                         const int64_t g, const int J, FarOwn& o, const bool wave_on,
+                                         float* __restrict__ bpart, const float H2, const float m, const float* stage,
+                                         float* my_stage, const int lane, const int tid, const int stg)
+{
+    if (!wave_on) {
+        if (stg >= 0)
+            for (int q = 0; q < kSegChunks; ++q) my_stage[q * kStage + stg] = 0.0f;
+    } else if ((n % kSB) != 0 && J == nsb - 1) {
+        far_segment<UM, SOFT, true>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+    } else {
+        far_segment<UM, SOFT, false>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+    }
+    // own side: plain sums over the segment, then compensated
+#pragma unroll
+    for (int q = 0; q < kNI; ++q) {
+        if (UM) {
+            o.sx[q].add(o.ax[q] * m); o.sy[q].add(o.ay[q] * m); o.sz[q].add(o.az[q] * m);
+        } else {
+            o.sx[q].add(o.ax[q]); o.sy[q].add(o.ay[q]); o.sz[q].add(o.az[q]);
+        }
+        o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
+    }
+    __syncthreads();
+    // partner side: the wavefronts' values in wavefront order
+#pragma unroll
+    for (int r = 0; r < kSegChunks * kStage / kFarThreads; ++r) {
+        const int idx = tid + kFarThreads * r;
+        float v = 0.0f;
+#pragma unroll
+        for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kSegChunks * kStage + idx];
+        const int q = idx / kStage, e = idx - q * kStage;
+        const int b = e / 3, comp = e - 3 * b;
+        const int B = kSB * J + kChunk * q + b;
+        if (B < n) {
+            const int64_t h = B / kGroup;
+            const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
+            bpart[3 * x + comp] = UM ? v * m : v;
+        }
+    }
+    __syncthreads();
+}
+
+// Far field of one instance.  Workgroup k of W takes the units
+// [k U / W, (k + 1) U / W) of the rows g (group g against the later
+// super-blocks) laid end to end; a row's own-side sums leave as fragment
+// g + k.
 template <bool UM, bool SOFT>
 __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
                                          const int64_t U, const int W, float4* __restrict__ apart,
@@ -410,84 +496,20 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
     int64_t g = lo, s = u0 - row_prefix(g, nsb);
     const int stg = stage_index(lane);
     float* const my_stage = stage + (size_t)w * kSegChunks * kStage;
-    const bool last_partial = (n % kSB) != 0;
-
     FarOwn o;
-    int sbA = 0;
-    bool wave_on = false;
-    auto load_row = [&]() {
-        sbA = (int)(g * kGroupSB) + w;
-        wave_on = sbA < nsb;
-#pragma unroll
-        for (int q = 0; q < kNI; ++q) {
-            const int t = kSB * sbA + 64 * q + lane;
-            o.x[q] = o.y[q] = o.z[q] = -3.0e18f;
-            o.m[q] = 0.0f;
-            if (t < n) {
-                const float4 p = xs[t];
-                o.x[q] = p.x; o.y[q] = p.y; o.z[q] = p.z; o.m[q] = p.w;
-            }
-            o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
-            o.sx[q] = o.sy[q] = o.sz[q] = CSum{0.0f, 0.0f};
-        }
-    };
-    auto flush_row = [&]() {
-        const size_t f = (size_t)(g + k);
-#pragma unroll
-        for (int q = 0; q < kNI; ++q) {
-            const size_t e = 2 * (f * kGroup + kSB * w + 64 * q + lane);
-            apart[e] = make_float4(o.sx[q].s, o.sy[q].s, o.sz[q].s, 0.0f);
-            apart[e + 1] = make_float4(o.sx[q].c, o.sy[q].c, o.sz[q].c, 0.0f);
-        }
-    };
-    load_row();
+    bool wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
     for (int64_t u = u0; u < u1; ++u) {
         if (s == row_units(g, nsb)) {
-            flush_row();
+            far_flush(apart, (size_t)(g + k), w, lane, o);
             ++g;
             s = 0;
-            load_row();
+            wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
         }
         const int J = (int)(kGroupSB * (g + 1) + s);  // partner super-block of this unit
-        if (!wave_on) {
-            if (stg >= 0)
-                for (int q = 0; q < kSegChunks; ++q) my_stage[q * kStage + stg] = 0.0f;
-        } else if (last_partial && J == nsb - 1) {
-            far_segment<UM, SOFT, true>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
-        } else {
-            far_segment<UM, SOFT, false>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
-        }
-        // own side: plain sums over the segment, then compensated
-#pragma unroll
-        for (int q = 0; q < kNI; ++q) {
-            if (UM) {
-                o.sx[q].add(o.ax[q] * m); o.sy[q].add(o.ay[q] * m); o.sz[q].add(o.az[q] * m);
-            } else {
-                o.sx[q].add(o.ax[q]); o.sy[q].add(o.ay[q]); o.sz[q].add(o.az[q]);
-            }
-            o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
-        }
-        __syncthreads();
-        // partner side: the wavefronts' values in wavefront order
-#pragma unroll
-        for (int r = 0; r < kSegChunks * kStage / kFarThreads; ++r) {
-            const int idx = tid + kFarThreads * r;
-            float v = 0.0f;
-#pragma unroll
-            for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kSegChunks * kStage + idx];
-            const int q = idx / kStage, e = idx - q * kStage;
-            const int b = e / 3, comp = e - 3 * b;
-            const int B = kSB * J + kChunk * q + b;
-            if (B < n) {
-                const int64_t h = B / kGroup;
-                const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
-                bpart[3 * x + comp] = UM ? v * m : v;
-            }
-        }
-        __syncthreads();
+        far_unit<UM, SOFT>(xs, n, nsb, g, J, o, wave_on, bpart, H2, m, stage, my_stage, lane, tid, stg);
         ++s;
     }
-    flush_row();
+    far_flush(apart, (size_t)(g + k), w, lane, o);
 }
 
 template <bool SOFT>
@@ -552,6 +574,192 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
         if (maxvsig) maxvsig[i] = 0.0f;
     }
     acc[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// ------------------------------------------------------------------ batches
+// The same sweep over the larger instances of a batch (slot indices are
+// the batch's; the smaller instances stay with the one-sided persistent
+// sweep).  Differences: no shares in the near field (the batch has
+// wavefronts to spare); the far kernel is persistent and takes whole rows
+// (instance, group) from a counter, longest first, so each row's own-side
+// sums leave as one fragment, numbered by the instance's first row + g.
+
+// Bounding boxes of the 64-slot blocks of every instance (the batch's
+// 256-slot block table, one wavefront per 64 slots).
+__global__ __launch_bounds__(kBlock) void sym_batch_box_kernel(const float4* __restrict__ xs,
+                                                               const int4* __restrict__ blk,
+                                                               const SymInst* __restrict__ sinst,
+                                                               float4* __restrict__ box)
+{
+    const int4 bl = blk[blockIdx.x];
+    const int lane = threadIdx.x & 63, first = bl.y + 64 * (threadIdx.x >> 6);
+    if (first >= bl.z) return;  // whole wavefronts leave together
+    const SymInst S = sinst[bl.x];
+    const int I = (first - S.off) >> 6, t = first + lane;
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (t < bl.z) {
+        const float4 p = xs[t];
+        v[0] = v[3] = p.x; v[1] = v[4] = p.y; v[2] = v[5] = p.z;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = fminf(v[c], __shfl_xor(v[c], o, 64));
+            v[c + 3] = fmaxf(v[c + 3], __shfl_xor(v[c + 3], o, 64));
+        }
+    }
+    if (lane == 0) {
+        box[2 * (size_t)(S.box0 + I)] = make_float4(v[0], v[1], v[2], 0.0f);
+        box[2 * (size_t)(S.box0 + I) + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+    }
+}
+
+// near-block bits of instance blockIdx.y (as near_bits_kernel, per instance)
+__global__ __launch_bounds__(kBlock) void sym_batch_bits_kernel(const float4* __restrict__ box,
+                                                                const SymInst* __restrict__ sinst,
+                                                                const BatchInst* __restrict__ insts,
+                                                                uint32_t* __restrict__ bits)
+{
+    const SymInst S = sinst[blockIdx.y];
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= S.nb * S.nwords) return;
+    const float H2 = insts[blockIdx.y].H2;
+    const int I = idx / S.nwords, w = idx - I * S.nwords;
+    const float4* bx = box + 2 * (size_t)S.box0;
+    const float4 lo = bx[2 * I], hi = bx[2 * I + 1];
+    uint32_t word = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int J = 32 * w + b;
+        if (J >= S.nb) break;
+        const float4 l2 = bx[2 * J], h2 = bx[2 * J + 1];
+        const float gx = fmaxf(0.0f, fmaxf(l2.x - hi.x, lo.x - h2.x));
+        const float gy = fmaxf(0.0f, fmaxf(l2.y - hi.y, lo.y - h2.y));
+        const float gz = fmaxf(0.0f, fmaxf(l2.z - hi.z, lo.z - h2.z));
+        if (dist2_exact(gx, gy, gz) <= H2) word |= 1u << b;
+    }
+    bits[S.bits0 + idx] = word;
+}
+
+template <bool GRAV, bool NBR, bool SOFT>
+__global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
+    const float4* __restrict__ xs, const int4* __restrict__ blk, const SymInst* __restrict__ sinst,
+    const uint32_t* __restrict__ bits, int* __restrict__ nbr, int* __restrict__ nbr_cnt, const int kmax,
+    const int ntotal, float4* __restrict__ near_part, const DevParams* __restrict__ prm,
+    const float* __restrict__ umass)
+{
+    const int4 bl = blk[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int first = __builtin_amdgcn_readfirstlane(bl.y + 64 * (int)(threadIdx.x >> 6));
+    if (first >= bl.z) return;
+    const int k = bl.x;
+    const SymInst S = sinst[k];
+    const int I = (first - S.off) >> 6;
+    const DevParams P = prm[k];
+    const Rare R = make_rare(nbr, (size_t)ntotal, kmax, P.H2);
+    const uint32_t* b = NBR ? bits + S.bits0 : nullptr;
+    if constexpr (GRAV) {
+        if (umass[2 * k] != 0.0f) {
+            near_blocks<GRAV, NBR, SOFT, true>(xs, S.off, S.n, S.nb, S.nwords, b, I, 0, 1, lane, nbr_cnt, near_part, P,
+                                               R, umass[2 * k + 1]);
+            return;
+        }
+    }
+    near_blocks<GRAV, NBR, SOFT, false>(xs, S.off, S.n, S.nb, S.nwords, b, I, 0, 1, lane, nbr_cnt, near_part, P, R,
+                                        1.0f);
+}
+
+// Persistent far field over the batch's rows; every wavefront leaves once
+// the counter passes nrows.
+template <bool SOFT>
+__global__ __launch_bounds__(kFarThreads) void sym_batch_far_kernel(
+    const float4* __restrict__ xs, const SymInst* __restrict__ sinst, const BatchInst* __restrict__ insts,
+    const SymRow* __restrict__ rows, const int nrows, int* __restrict__ queue, float4* __restrict__ apart,
+    float* __restrict__ bpart, const float* __restrict__ umass)
+{
+    __shared__ float stage[kGroupSB * kSegChunks * kStage];
+    __shared__ int row_sh;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
+    const int stg = stage_index(lane);
+    float* const my_stage = stage + (size_t)w * kSegChunks * kStage;
+    for (;;) {
+        if (tid == 0) row_sh = atomicAdd(queue, 1);
+        __syncthreads();
+        const int r = row_sh;
+        __syncthreads();  // all have read it before it is written again
+        if (r >= nrows) break;
+        const SymRow R = rows[r];
+        const SymInst S = sinst[R.inst];
+        const float H2 = insts[R.inst].H2;
+        const float4* xk = xs + S.off;
+        float* const bk = bpart + S.bpart0;
+        const bool um = umass[2 * R.inst] != 0.0f;
+        const float m = umass[2 * R.inst + 1];
+        FarOwn o;
+        const bool wave_on = far_load_row(xk, S.n, S.nsb, R.g, w, lane, o);
+        for (int s = 0; s < R.units; ++s) {
+            const int J = kGroupSB * (R.g + 1) + s;
+            if (um)
+                far_unit<true, SOFT>(xk, S.n, S.nsb, R.g, J, o, wave_on, bk, H2, m, stage, my_stage, lane, tid, stg);
+            else
+                far_unit<false, SOFT>(xk, S.n, S.nsb, R.g, J, o, wave_on, bk, H2, 1.0f, stage, my_stage, lane, tid,
+                                      stg);
+        }
+        far_flush(apart, (size_t)(S.row0 + R.g), w, lane, o);
+    }
+}
+
+// Per slot of the instances in the sweep (256-slot block table), as
+// sym_combine_kernel: near partial, the row's one own-side fragment, the
+// partner-side partials (group order, negated).
+__global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
+    const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
+    const SymInst* __restrict__ sinst, const int4* __restrict__ blk, const int grav, const int* __restrict__ perm,
+    const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const float4* __restrict__ vel,
+    float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams* __restrict__ prm)
+{
+    const int4 bl = blk[blockIdx.x];
+    const int t = bl.y + threadIdx.x;
+    if (t >= bl.z) return;
+    const int k = bl.x;
+    const SymInst S = sinst[k];
+    CSum x{0.0f, 0.0f}, y{0.0f, 0.0f}, z{0.0f, 0.0f};
+    if (grav) {
+        const float4 a = near_part[2 * (size_t)t], c = near_part[2 * (size_t)t + 1];
+        x.add(a.x); y.add(a.y); z.add(a.z);
+        x.c += c.x; y.c += c.y; z.c += c.z;
+        const int local = t - S.off;
+        const int g = local / kGroup, lg = local - g * kGroup;
+        if (g < S.nrows) {
+            const size_t e = 2 * ((size_t)(S.row0 + g) * kGroup + lg);
+            const float4 f = apart[e], fc = apart[e + 1];
+            x.add(f.x); y.add(f.y); z.add(f.z);
+            x.c += fc.x; y.c += fc.y; z.c += fc.z;
+        }
+        const float* bk = bpart + S.bpart0;
+        const int64_t base = bpart_group_base(g) + lg;
+        for (int gp = 0; gp < g; ++gp) {
+            const size_t e = 3 * (size_t)(base + (int64_t)gp * kGroup);
+            x.add(-bk[e]); y.add(-bk[e + 1]); z.add(-bk[e + 2]);
+        }
+    }
+    float ax = x.value(), ay = y.value(), az = z.value();
+    const int i = perm ? perm[t] : t;
+    if (finish) {
+        const float4 p = xs[t];
+        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        environment_acceleration(prm[k], p.x, p.y, p.z, v.x, v.y, v.z, ax, ay, az);
+        if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (maxvsig) maxvsig[i] = 0.0f;
+    }
+    acc[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+template <bool GRAV, bool NBR, bool SOFT>
+void launch_batch_near(const SymBatchLaunch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL((sym_batch_near_kernel<GRAV, NBR, SOFT>), dim3(L.n_blk), dim3(kBlock), 0, s, L.xs, L.blk,
+                       L.sinst, L.bits, L.nbr, L.nbr_cnt, L.kmax, L.ntotal, L.near_part, L.prm, L.umass);
 }
 
 template <bool GRAV, bool NBR, bool SOFT>
@@ -623,6 +831,82 @@ hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t 
         hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part,
                            L.apart, L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
                            L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_nbr, int64_t min_n)
+{
+    SymBatchTables T;
+    const int n_inst = (int)offsets.size() - 1;
+    int box0 = 0, row0 = 0;
+    int64_t bits0 = 0, bpart0 = 0;
+    for (int k = 0; k < n_inst; ++k) {
+        const int64_t n = offsets[k + 1] - offsets[k];
+        const bool sym = n >= min_n && n > 0;
+        SymInst S;
+        S.off = (int)offsets[k];
+        S.n = sym ? (int)n : 0;  // instances left to the one-sided sweep: nothing here
+        if (sym)
+            for (int64_t s = offsets[k]; s < offsets[k + 1]; s += 256)
+                T.blk.push_back(make_int4(k, (int)s, (int)std::min<int64_t>(s + 256, offsets[k + 1]), 0));
+        S.nb = (int)((S.n + 63) / 64);
+        S.nwords = (S.nb + 31) / 32;
+        S.nsb = (int)((S.n + kSB - 1) / kSB);
+        S.nrows = (int)rows_with_units(S.nsb);
+        S.box0 = box0;
+        S.row0 = row0;
+        S.bits0 = bits0;
+        S.bpart0 = bpart0;
+        for (int g = 0; g < S.nrows; ++g) T.rows.push_back(SymRow{k, g, (int)row_units(g, S.nsb), 0});
+        T.max_words = std::max(T.max_words, S.nb * S.nwords);
+        box0 += S.nb;
+        row0 += S.nrows;
+        if (want_nbr) bits0 += (int64_t)S.nb * S.nwords;
+        bpart0 += 3 * bpart_group_base((S.nsb + kGroupSB - 1) / kGroupSB);
+        T.insts.push_back(S);
+    }
+    std::stable_sort(T.rows.begin(), T.rows.end(), [](const SymRow& a, const SymRow& b) { return a.units > b.units; });
+    T.box_elems = 2 * (int64_t)box0;
+    T.bits_elems = bits0;
+    T.apart_elems = 2 * (int64_t)kGroup * row0;
+    T.bpart_elems = bpart0;
+    return T;
+}
+
+hipError_t launch_sym_batch(const SymBatchLaunch& L, hipStream_t s)
+{
+    if (L.ntotal <= 0 || L.n_blk <= 0) return hipSuccess;
+    if (L.want_nbr) {
+        hipLaunchKernelGGL(sym_batch_box_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.xs, L.blk, L.sinst, L.box);
+        if (L.max_words > 0)
+            hipLaunchKernelGGL(sym_batch_bits_kernel, dim3((L.max_words + kBlock - 1) / kBlock, L.n_inst),
+                               dim3(kBlock), 0, s, L.box, L.sinst, L.insts, L.bits);
+    }
+    if (L.grav && L.want_nbr) {
+        if (L.soft) launch_batch_near<true, true, true>(L, s); else launch_batch_near<true, true, false>(L, s);
+    } else if (L.grav) {
+        if (L.soft) launch_batch_near<true, false, true>(L, s); else launch_batch_near<true, false, false>(L, s);
+    } else if (L.want_nbr) {
+        if (L.soft) launch_batch_near<false, true, true>(L, s); else launch_batch_near<false, true, false>(L, s);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (L.grav && L.nrows > 0) {
+        if ((e = launch_reset_queue(L.queue, 1, s)) != hipSuccess) return e;
+        if (L.soft)
+            hipLaunchKernelGGL((sym_batch_far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst,
+                               L.insts, L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
+        else
+            hipLaunchKernelGGL((sym_batch_far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst,
+                               L.insts, L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (L.grav || L.finish) {
+        hipLaunchKernelGGL(sym_batch_combine_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.near_part, L.apart,
+                           L.bpart, L.sinst, L.blk, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.vel,
+                           L.xvel, L.maxvsig, L.prm);
         e = hipGetLastError();
     }
     return e;

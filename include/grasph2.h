@@ -155,7 +155,11 @@ int grasph_destroy(grasph_ctx* ctx);
  * neighbour lists are identical), "sym_workgroups" (workgroups of the
  * symmetric sweep's far-field kernel, 0 = two wavefronts per SIMD),
  * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
- * 0 = chosen from the occupancy). */
+ * 0 = chosen from the occupancy), "batch_symmetric" (1 = batches use the
+ * symmetric sweep for their instances of at least "batch_symmetric_min"
+ * particles, default 0; the smaller ones keep the one-sided persistent
+ * sweep.  Off by default: batched results then equal the single-instance
+ * path's only to FP32 rounding instead of bit for bit). */
 int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
 
 /* ---- the hot path ------------------------------------------------------- */

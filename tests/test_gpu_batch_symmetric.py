@@ -1,0 +1,90 @@
+"""The symmetric sweep over a batch (option batch_symmetric = 1, for the
+instances of at least batch_symmetric_min particles): per instance, gravity
+between groups of 1024 is evaluated once per pair, the rest through the
+candidate sweep; smaller instances stay with the persistent one-sided
+sweep.  Checked against the one-sided batched sweep (same neighbour lists, so the list passes agree bit for bit; gravity
+to FP32 rounding), against the FP64 oracle for one instance with several
+groups, for graph replays against eager steps, and for run-to-run
+determinism.  Sizes cover instances without far work (<= 1024 particles),
+ragged last super-blocks and several groups."""
+import numpy as np
+import pytest
+
+import grasph2_amd as g
+from oracle import pyoracle as O
+from tests._util import vec_errors
+from tests.test_gpu_batch import FIELDS_S, make_batch
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (300, 5000, 1024, 2500, 1025, 4100, 64, 1, 3333)
+
+
+@pytest.fixture
+def sym_batch_ctx(gpu_ctx):
+    """Instances of 1000 particles or more take the symmetric sweep, the
+    smaller ones (300, 64, 1) the one-sided persistent sweep, in one batch."""
+    gpu_ctx.set_option("batch_symmetric", 1)
+    gpu_ctx.set_option("batch_symmetric_min", 1000)
+    yield gpu_ctx
+    gpu_ctx.set_option("batch_symmetric", 0)
+    gpu_ctx.set_option("batch_symmetric_min", 0)
+
+
+def run_batch(ctx, host, offsets, params):
+    b = ctx.batch(offsets, params)
+    d = g.DeviceParticles.from_host(host)
+    ctx.batch_compute_derivatives(d, b)
+    g.hip.synchronize()
+    out = g.unpack_batch(d.download(), offsets)
+    d.free()
+    return out
+
+
+@pytest.mark.parametrize("gravity_only", [False, True])
+def test_matches_one_sided_batch_and_fp64(sym_batch_ctx, gravity_only):
+    print("batch symmetric gravity_only=%s" % gravity_only, flush=True)
+    ctx = sym_batch_ctx
+    insts, params, host, offsets = make_batch(sizes=SIZES, seed=5)
+    if gravity_only:
+        for p in params:
+            p.flags = g.SELF_GRAVITY
+    got = run_batch(ctx, host, offsets, params)
+    ctx.set_option("batch_symmetric", 0)
+    ref = run_batch(ctx, host, offsets, params)
+    ctx.set_option("batch_symmetric", 1)
+    for k, (a, r) in enumerate(zip(got, ref)):
+        if SIZES[k] > 1:
+            mx, nr = vec_errors(a["acc"], r["acc"])
+            assert nr <= 1e-6 and mx <= 1e-5, (k, mx, nr)
+        if not gravity_only:
+            assert np.array_equal(a["density"], r["density"]), k  # same lists, same order
+            assert np.array_equal(a["dstress_dt"], r["dstress_dt"]), k
+    o = O.compute_derivatives(params[1], insts[1], "f64")
+    mx, nr = vec_errors(got[1]["acc"], o["acc"])
+    assert nr <= 1e-5 and mx <= 1e-4, (mx, nr)
+
+
+def test_graph_steps_match_eager_and_repeat(sym_batch_ctx):
+    print("batch symmetric graph", flush=True)
+    ctx = sym_batch_ctx
+    insts, params, host, offsets = make_batch(sizes=SIZES, seed=6)
+    b = ctx.batch(offsets, params)
+    dg = g.DeviceParticles.from_host(host)
+    de = g.DeviceParticles.from_host(host)
+    for step in range(3):
+        ctx.batch_step(dg, b, 1)
+        g.hip.synchronize()
+        ctx.batch_compute_derivatives(de, b)
+        ctx.batch_integrate(de, b, True)
+        g.hip.synchronize()
+        a, e = dg.download(), de.download()
+        for f in FIELDS_S:
+            assert np.array_equal(a[f], e[f]), "replay %d field %s" % (step, f)
+    dg.free()
+    de.free()
+    x = run_batch(ctx, host, offsets, params)
+    y = run_batch(ctx, host, offsets, params)
+    for k, (a, r) in enumerate(zip(x, y)):
+        for f in ("acc", "density", "dstress_dt", "maxvsig"):
+            assert np.array_equal(a[f], r[f]), (k, f)
