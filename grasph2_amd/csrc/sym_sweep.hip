@@ -218,13 +218,18 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
     // the own group's blocks lie in one word (groups of 16 blocks are 16-aligned)
     const int gw = G0 >> 5;
     const uint32_t gmask = (uint32_t)(((1ull << (G1 - G0)) - 1) << (G0 - 32 * gw));
+    // the row through the constant address space: a uniform address there
+    // becomes a scalar load (a global pointer gives a vector load and a wait
+    // per word)
+    const __attribute__((address_space(4))) uint32_t* crow = (const __attribute__((address_space(4))) uint32_t*)row;
     auto word_of = [&](int w) -> uint32_t {
-        uint32_t word = use_bits ? __builtin_amdgcn_readfirstlane(row[w]) : 0u;
+        uint32_t word = use_bits ? crow[w] : 0u;
         if ((GRAV || !use_bits) && w == gw) word |= gmask;
         return word;
     };
     const int w0 = use_bits ? 0 : gw, w1 = use_bits ? nwords : gw + 1;
     int total = 0;
+#pragma unroll 8
     for (int w = w0; w < w1; ++w) total += __builtin_popcount(word_of(w));
     const int r0 = (int)((int64_t)sl * total / ns), r1 = (int)((int64_t)(sl + 1) * total / ns);
     // blocks of this share, consecutive ones of one kind (own group or not) swept as one range
