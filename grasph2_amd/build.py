@@ -74,6 +74,20 @@ def build_cpp_example(verbose=True):
             print("built", exe)
 
 
+def build_host_checks(verbose=True):
+    """Compile tests/cpp/sym_geometry_check (host code that includes the
+    symmetric sweep's geometry header; hipcc for its __device__ __host__
+    functions, no GPU needed to run it)."""
+    src = os.path.join(ROOT, "tests", "cpp", "sym_geometry_check.cpp")
+    exe = os.path.join(ROOT, "tests", "cpp", "sym_geometry_check")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wno-unused-command-line-argument", src, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host check build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    if verbose:
+        print("built", exe)
+
+
 def build_oracle(verbose=True):
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)

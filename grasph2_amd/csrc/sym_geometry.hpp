@@ -1,0 +1,59 @@
+// sym_geometry.hpp -- how the symmetric sweep (sym_sweep.hip) cuts its work:
+// slots in super-blocks of 256 (one far-field wavefront, 4 slots per lane)
+// and groups of 1024 (one far-field workgroup).  The far field's rows are
+// group g against the super-blocks of the later groups; one unit is one
+// partner super-block.  Host and device code use these functions, and
+// tests/cpp/sym_geometry_check.cpp checks them on the host (every far pair
+// exactly once, indices in bounds, every partial that is read was written).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+namespace grasph {
+
+// own slots per lane in the far field (8 was measured 12% slower: the
+// registers cap it at 2 wavefronts per SIMD with half as many workgroups'
+// worth of slots in flight)
+constexpr int kNI = 4;
+constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefront's slots)
+constexpr int kGroup = 1024;            // slots per group
+constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)
+
+// far units (partner super-blocks) of row g: super-blocks [kGroupSB (g + 1), nsb)
+__device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb)
+{
+    const int64_t u = nsb - (int64_t)kGroupSB * (g + 1);
+    return u > 0 ? u : 0;
+}
+// units before row g (g <= rows_with_units): sum over g' < g of (nsb - kGroupSB (g' + 1))
+__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb)
+{
+    return g * nsb - (int64_t)kGroupSB * (g * (g + 1) / 2);
+}
+// rows with far work: nsb - kGroupSB (g + 1) >= 1
+__device__ __host__ inline int64_t rows_with_units(int64_t nsb) { return nsb >= 1 ? (nsb - 1) / kGroupSB : 0; }
+// the row holding unit u (0 <= u < row_prefix(rows_with_units(nsb), nsb)): the largest g with row_prefix(g) <= u
+__device__ __host__ inline int64_t row_of_unit(int64_t u, int64_t nsb)
+{
+    int64_t lo = 0, hi = rows_with_units(nsb) - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (row_prefix(mid, nsb) <= u) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// the workgroup whose share [k U / W, (k + 1) U / W) holds unit u
+__device__ __host__ inline int64_t wg_of_unit(int64_t u, int64_t U, int64_t W) { return ((u + 1) * W - 1) / U; }
+// workgroups of the far kernel: at most the units, so every one has a share and a row's
+// fragments come from consecutive workgroups
+__host__ inline int far_workgroups(int64_t wanted, int64_t units)
+{
+    const int64_t w = wanted < units ? wanted : units;
+    return (int)(w > 1 ? w : 1);
+}
+// first element of the partner-side partials of group h (the h groups before it contribute)
+__device__ __host__ inline int64_t bpart_group_base(int64_t h) { return (int64_t)kGroup * (h * (h - 1) / 2); }
+
+}  // namespace grasph

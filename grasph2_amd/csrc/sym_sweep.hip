@@ -36,40 +36,14 @@
 #include <vector>
 
 #include "sweep_core.hpp"
+#include "sym_geometry.hpp"
 
 namespace grasph {
 namespace {
 
-// own slots per lane in the far field (8 was measured 12% slower: the
-// registers cap it at 2 wavefronts per SIMD with half as many workgroups'
-// worth of slots in flight)
-constexpr int kNI = 4;
-constexpr int kSB = 64 * kNI;     // slots per super-block (a far wavefront's slots)
-constexpr int kGroup = 1024;      // slots per group
-constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)
 constexpr int kFarThreads = 64 * kGroupSB;
 constexpr int kSegChunks = kSB / kChunk;  // chunks of 8 partners per far segment (one super-block)
 constexpr int kStage = 24;        // partner values per chunk: 8 partners x 3 components
-
-// ------------------------------------------------------------------ geometry of the work
-
-// far units (partner super-blocks) of row g: super-blocks [kGroupSB (g + 1), nsb)
-__device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb)
-{
-    const int64_t u = nsb - (int64_t)kGroupSB * (g + 1);
-    return u > 0 ? u : 0;
-}
-// units before row g (g <= rows_with_units): sum over g' < g of (nsb - kGroupSB (g' + 1))
-__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb)
-{
-    return g * nsb - (int64_t)kGroupSB * (g * (g + 1) / 2);
-}
-// rows with far work: nsb - kGroupSB (g + 1) >= 1
-__device__ __host__ inline int64_t rows_with_units(int64_t nsb) { return nsb >= 1 ? (nsb - 1) / kGroupSB : 0; }
-// the workgroup whose share [k U / W, (k + 1) U / W) holds unit u
-__device__ __host__ inline int64_t wg_of_unit(int64_t u, int64_t U, int64_t W) { return ((u + 1) * W - 1) / U; }
-// first element of the partner-side partials of group h (the h groups before it contribute)
-__device__ __host__ inline int64_t bpart_group_base(int64_t h) { return (int64_t)kGroup * (h * (h - 1) / 2); }
 
 // ------------------------------------------------------------------ cross-lane reduction
 
@@ -491,14 +465,7 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
     const int64_t u0 = (int64_t)k * U / W, u1 = (int64_t)(k + 1) * U / W;
     if (u0 >= u1) return;  // the whole workgroup leaves: no barrier is reached
-    const int64_t npos = rows_with_units(nsb);
-    // row of u0: the largest g < npos with row_prefix(g) <= u0
-    int64_t lo = 0, hi = npos - 1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) / 2;
-        if (row_prefix(mid, nsb) <= u0) lo = mid; else hi = mid - 1;
-    }
-    int64_t g = lo, s = u0 - row_prefix(g, nsb);
+    int64_t g = row_of_unit(u0, nsb), s = u0 - row_prefix(g, nsb);
     const int stg = stage_index(lane);
     float* const my_stage = stage + (size_t)w * kSegChunks * kStage;
     FarOwn o;
@@ -790,8 +757,7 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
     G.nsb = (n + kSB - 1) / kSB;
     G.groups = (int)((G.nsb + kGroupSB - 1) / kGroupSB);
     G.units = row_prefix(rows_with_units(G.nsb), G.nsb);
-    // every workgroup gets at least one unit, so a row's fragments come from consecutive workgroups
-    G.wgs = (int)std::max<int64_t>(1, std::min<int64_t>(wgs, G.units));
+    G.wgs = far_workgroups(wgs, G.units);
     G.nwords = (G.nb + 31) / 32;
     G.box_elems = 2 * (int64_t)G.nb;
     G.bits_elems = want_nbr ? (int64_t)G.nb * G.nwords : 0;

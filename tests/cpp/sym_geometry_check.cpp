@@ -1,0 +1,146 @@
+// Host check of the symmetric sweep's work geometry
+// (grasph2_amd/csrc/sym_geometry.hpp), no GPU: replays on the host how the
+// far kernel's workgroups walk their shares of the rows (as far_work in
+// sym_sweep.hip does) and how the combine reads the partials back
+// (sym_combine_kernel), for many particle counts and workgroup counts, and
+// checks that
+//  - every pair of super-blocks in different groups is computed exactly once
+//    and no other pair is;
+//  - every own-side fragment and partner-side partial index is in bounds of
+//    the arrays sym_geometry() sizes, and no partial is written twice;
+//  - every fragment and partial the combine reads was written, and every
+//    fragment written is read.
+// Also the batch form: the rows table covers each (instance, row) once.
+// Prints "ok <cases>" and exits 0, or prints the first failure and exits 1.
+#include <cstdio>
+#include <cstdlib>
+#include <set>
+#include <utility>
+#include <vector>
+
+#include "../../grasph2_amd/csrc/sym_geometry.hpp"
+
+using namespace grasph;
+
+static int fail(const char* what, int64_t n, int64_t W, int64_t a = -1, int64_t b = -1)
+{
+    std::printf("FAIL %s n=%lld W=%lld (%lld, %lld)\n", what, (long long)n, (long long)W, (long long)a, (long long)b);
+    return 1;
+}
+
+static int check(int64_t n, int64_t wanted)
+{
+    const int64_t nsb = (n + kSB - 1) / kSB;
+    const int64_t G = (nsb + kGroupSB - 1) / kGroupSB;
+    const int64_t npos = rows_with_units(nsb);
+    const int64_t U = row_prefix(npos, nsb);
+    int64_t sum = 0;
+    for (int64_t g = 0; g < G; ++g) sum += row_units(g, nsb);
+    if (sum != U) return fail("unit total", n, wanted, sum, U);
+    if (U == 0) return 0;  // no far field: the near field covers the one group
+    const int64_t W = far_workgroups(wanted, U);
+    const int64_t apart_elems = 2 * (int64_t)kGroup * (G + wanted);
+    const int64_t bpart_elems = 3 * bpart_group_base(G);
+
+    std::vector<char> seen((size_t)(nsb * nsb), 0);
+    std::vector<char> bw((size_t)(bpart_elems / 3), 0);
+    std::set<std::pair<int64_t, int64_t>> frag;  // (row, workgroup)
+    for (int64_t k = 0; k < W; ++k) {
+        const int64_t u0 = k * U / W, u1 = (k + 1) * U / W;
+        if (u0 >= u1) return fail("empty share", n, W, k);
+        int64_t g = row_of_unit(u0, nsb), s = u0 - row_prefix(g, nsb);
+        if (s < 0 || s >= row_units(g, nsb)) return fail("row search", n, W, u0, g);
+        for (int64_t u = u0; u < u1; ++u) {
+            if (s == row_units(g, nsb)) {
+                frag.insert({g, k});
+                ++g;
+                s = 0;
+                if (g >= npos) return fail("row past the last", n, W, g);
+            }
+            const int64_t J = kGroupSB * (g + 1) + s;
+            if (J >= nsb) return fail("partner super-block", n, W, g, J);
+            for (int w = 0; w < kGroupSB; ++w) {
+                const int64_t sbA = g * kGroupSB + w;
+                if (sbA >= nsb) continue;
+                char& c = seen[(size_t)(sbA * nsb + J)];
+                if (c) return fail("pair twice", n, W, sbA, J);
+                c = 1;
+            }
+            for (int64_t B = kSB * J; B < kSB * (J + 1) && B < n; ++B) {
+                const int64_t h = B / kGroup;
+                const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
+                if (x < 0 || 3 * x + 2 >= bpart_elems) return fail("partial bounds", n, W, g, B);
+                if (bw[(size_t)x]) return fail("partial twice", n, W, g, B);
+                bw[(size_t)x] = 1;
+            }
+            ++s;
+        }
+        frag.insert({g, k});
+    }
+    for (int64_t I = 0; I < nsb; ++I)
+        for (int64_t J = 0; J < nsb; ++J)
+            if ((I / kGroupSB < J / kGroupSB) != (seen[(size_t)(I * nsb + J)] != 0)) return fail("coverage", n, W, I, J);
+    for (const auto& f : frag)
+        if (2 * ((f.first + f.second + 1) * (int64_t)kGroup) > apart_elems) return fail("fragment bounds", n, W, f.first, f.second);
+    // what the combine reads
+    size_t reads = 0;
+    for (int64_t g = 0; g < G; ++g) {
+        if (g < npos) {
+            const int64_t k0 = wg_of_unit(row_prefix(g, nsb), U, W), k1 = wg_of_unit(row_prefix(g + 1, nsb) - 1, U, W);
+            for (int64_t k = k0; k <= k1; ++k) {
+                if (k * U / W == (k + 1) * U / W) continue;
+                if (!frag.count({g, k})) return fail("fragment read, not written", n, W, g, k);
+                ++reads;
+            }
+        }
+        for (int64_t t = g * kGroup; t < (g + 1) * kGroup && t < n; ++t)
+            for (int64_t gp = 0; gp < g; ++gp)
+                if (!bw[(size_t)(bpart_group_base(g) + gp * kGroup + (t - g * kGroup))])
+                    return fail("partial read, not written", n, W, gp, t);
+    }
+    if (reads != frag.size()) return fail("fragment written, not read", n, W, (int64_t)reads, (int64_t)frag.size());
+    return 0;
+}
+
+// batch form: one row per (instance, g < rows_with_units), fragments numbered row0 + g
+static int check_batch(const std::vector<int64_t>& sizes)
+{
+    int64_t row0 = 0, bpart0 = 0;
+    std::set<std::pair<int64_t, int64_t>> rows;
+    for (size_t k = 0; k < sizes.size(); ++k) {
+        const int64_t n = sizes[k], nsb = (n + kSB - 1) / kSB, G = (nsb + kGroupSB - 1) / kGroupSB;
+        const int64_t nrows = rows_with_units(nsb);
+        for (int64_t g = 0; g < nrows; ++g) {
+            if (row_units(g, nsb) <= 0) return fail("batch empty row", n, (int64_t)k, g);
+            if (!rows.insert({(int64_t)k, g}).second) return fail("batch row twice", n, (int64_t)k, g);
+        }
+        if (row_units(nrows, nsb) != 0) return fail("batch row missing", n, (int64_t)k, nrows);
+        row0 += nrows;
+        bpart0 += 3 * bpart_group_base(G);
+    }
+    return bpart0 >= 0 && row0 == (int64_t)rows.size() ? 0 : fail("batch totals", row0, (int64_t)rows.size());
+}
+
+int main()
+{
+    const int64_t ns[] = {1, 63, 64, 255, 256, 257, 1023, 1024, 1025, 2047, 2048, 3000, 4096, 5000, 6000,
+                          8191, 16384, 20037, 65536, 100000, 131072};
+    const int64_t ws[] = {1, 2, 3, 7, 12, 64, 512, 1024, 100000};
+    int cases = 0;
+    for (int64_t n : ns)
+        for (int64_t w : ws) {
+            if (check(n, w)) return 1;
+            ++cases;
+        }
+    srand(7);
+    for (int i = 0; i < 60; ++i) {
+        const int64_t n = 1 + rand() % 150000, w = 1 + rand() % 2048;
+        if (check(n, w)) return 1;
+        ++cases;
+    }
+    std::vector<int64_t> sizes;
+    for (int i = 0; i < 200; ++i) sizes.push_back(rand() % 17000);
+    if (check_batch(sizes)) return 1;
+    std::printf("ok %d\n", cases + 1);
+    return 0;
+}
