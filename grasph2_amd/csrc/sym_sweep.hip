@@ -163,12 +163,19 @@ __global__ __launch_bounds__(kBlock) void near_bits_kernel(const float4* __restr
 // blocks are the blocks of its group (gravity, and the distance test when
 // lists are wanted) and, with bits, the other blocks near it (distance test
 // only); in increasing order they are cut into ns runs of equal count, and
-// this wavefront takes run sl.  Writes the slot's compensated gravity
+// this wavefront takes run sl.  A block of the distance test only is swept
+// only if some slot of block I comes within H of its box (box: the
+// instance's block boxes): the point-to-box gaps and their squared sum
+// round like the exact pair distance (lo <= x <= hi, rounding is monotonic),
+// so they never exceed the distance to a particle in the box, and a skipped
+// block holds no neighbour.  Blocks whose own box is large (a jump of the
+// slot order) skip most of their box neighbours this way.  Writes the slot's compensated gravity
 // partial (slot-indexed, two float4) and its list and count -- all in the
 // planes of share sl, which later fold in share order.
 template <bool GRAV, bool NBR, bool SOFT, bool UM>
 __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const int base, const int n, const int nb,
-                                            const int nwords, const uint32_t* __restrict__ bits, const int I,
+                                            const int nwords, const uint32_t* __restrict__ bits,
+                                            const float4* __restrict__ box, const int I,
                                             const int sl, const int ns, const int lane, int* __restrict__ nbr_cnt,
                                             float4* __restrict__ near_part, const DevParams& P, const Rare& R,
                                             const float m)
@@ -201,6 +208,15 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
         if ((GRAV || !use_bits) && w == gw) word |= gmask;
         return word;
     };
+    // the boxes through the constant address space too (scalar loads)
+    const __attribute__((address_space(4))) float* cbox = (const __attribute__((address_space(4))) float*)box;
+    auto lane_near = [&](int J) -> bool {
+        const __attribute__((address_space(4))) float* c = cbox + 8 * J;  // lo.xyzw, hi.xyzw
+        const float gx = fmaxf(0.0f, fmaxf(c[0] - xi[0], xi[0] - c[4]));
+        const float gy = fmaxf(0.0f, fmaxf(c[1] - yi[0], yi[0] - c[5]));
+        const float gz = fmaxf(0.0f, fmaxf(c[2] - zi[0], zi[0] - c[6]));
+        return __builtin_amdgcn_ballot_w64(act && dist2_exact(gx, gy, gz) <= P.H2) != 0;
+    };
     const int w0 = use_bits ? 0 : gw, w1 = use_bits ? nwords : gw + 1;
     int total = 0;
 #pragma unroll 8
@@ -230,6 +246,7 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
             word &= word - 1;
             if (rank++ < r0) continue;
             const bool g = J >= G0 && J < G1;
+            if (use_bits && !(GRAV && g) && !lane_near(J)) continue;
             if (run0 >= 0 && (J != run1 || g != run_grav)) flush();
             if (run0 < 0) {
                 run0 = J;
@@ -250,7 +267,8 @@ __device__ __forceinline__ void near_blocks(const float4* __restrict__ xs, const
 template <bool GRAV, bool NBR, bool SOFT>
 __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__ xs, const int n, const int nb,
                                                       const int nwords, const uint32_t* __restrict__ bits,
-                                                      int* __restrict__ nbr, int* __restrict__ nbr_cnt,
+                                                      const float4* __restrict__ box, int* __restrict__ nbr,
+                                                      int* __restrict__ nbr_cnt,
                                                       const int kmax, float4* __restrict__ near_part,
                                                       const DevParams P, const float* __restrict__ umass)
 {
@@ -264,11 +282,11 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
     float4* const part = near_part + 2 * (size_t)sl * n;
     if constexpr (GRAV) {
         if (umass && umass[0] != 0.0f) {
-            near_blocks<GRAV, NBR, SOFT, true>(xs, 0, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, umass[1]);
+            near_blocks<GRAV, NBR, SOFT, true>(xs, 0, n, nb, nwords, bits, box, I, sl, ns, lane, cnt, part, P, R, umass[1]);
             return;
         }
     }
-    near_blocks<GRAV, NBR, SOFT, false>(xs, 0, n, nb, nwords, bits, I, sl, ns, lane, cnt, part, P, R, 1.0f);
+    near_blocks<GRAV, NBR, SOFT, false>(xs, 0, n, nb, nwords, bits, box, I, sl, ns, lane, cnt, part, P, R, 1.0f);
 }
 
 // Own slots of a lane in the far field: position, mass, plain and
@@ -616,9 +634,9 @@ __global__ __launch_bounds__(kBlock) void sym_batch_bits_kernel(const float4* __
 template <bool GRAV, bool NBR, bool SOFT>
 __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
     const float4* __restrict__ xs, const int4* __restrict__ blk, const SymInst* __restrict__ sinst,
-    const uint32_t* __restrict__ bits, int* __restrict__ nbr, int* __restrict__ nbr_cnt, const int kmax,
-    const int ntotal, float4* __restrict__ near_part, const DevParams* __restrict__ prm,
-    const float* __restrict__ umass)
+    const uint32_t* __restrict__ bits, const float4* __restrict__ box, int* __restrict__ nbr,
+    int* __restrict__ nbr_cnt, const int kmax, const int ntotal, float4* __restrict__ near_part,
+    const DevParams* __restrict__ prm, const float* __restrict__ umass)
 {
     const int4 bl = blk[blockIdx.x];
     const int lane = threadIdx.x & 63;
@@ -630,14 +648,15 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
     const DevParams P = prm[k];
     const Rare R = make_rare(nbr, (size_t)ntotal, kmax, P.H2);
     const uint32_t* b = NBR ? bits + S.bits0 : nullptr;
+    const float4* bx = NBR ? box + 2 * (size_t)S.box0 : nullptr;
     if constexpr (GRAV) {
         if (umass[2 * k] != 0.0f) {
-            near_blocks<GRAV, NBR, SOFT, true>(xs, S.off, S.n, S.nb, S.nwords, b, I, 0, 1, lane, nbr_cnt, near_part, P,
+            near_blocks<GRAV, NBR, SOFT, true>(xs, S.off, S.n, S.nb, S.nwords, b, bx, I, 0, 1, lane, nbr_cnt, near_part, P,
                                                R, umass[2 * k + 1]);
             return;
         }
     }
-    near_blocks<GRAV, NBR, SOFT, false>(xs, S.off, S.n, S.nb, S.nwords, b, I, 0, 1, lane, nbr_cnt, near_part, P, R,
+    near_blocks<GRAV, NBR, SOFT, false>(xs, S.off, S.n, S.nb, S.nwords, b, bx, I, 0, 1, lane, nbr_cnt, near_part, P, R,
                                         1.0f);
 }
 
@@ -731,7 +750,7 @@ template <bool GRAV, bool NBR, bool SOFT>
 void launch_batch_near(const SymBatchLaunch& L, hipStream_t s)
 {
     hipLaunchKernelGGL((sym_batch_near_kernel<GRAV, NBR, SOFT>), dim3(L.n_blk), dim3(kBlock), 0, s, L.xs, L.blk,
-                       L.sinst, L.bits, L.nbr, L.nbr_cnt, L.kmax, L.ntotal, L.near_part, L.prm, L.umass);
+                       L.sinst, L.bits, L.box, L.nbr, L.nbr_cnt, L.kmax, L.ntotal, L.near_part, L.prm, L.umass);
 }
 
 template <bool GRAV, bool NBR, bool SOFT>
@@ -740,7 +759,8 @@ void launch_near(const SymLaunch& L, const SymGeom& G, const DevParams& P, hipSt
     const int blocks = (G.nb + kBlock / 64 - 1) / (kBlock / 64);
     hipLaunchKernelGGL((near_kernel<GRAV, NBR, SOFT>), dim3(blocks, L.nshares), dim3(kBlock), 0, s, L.xs, L.n, G.nb,
                        G.nwords,
-                       L.want_nbr ? L.bits : nullptr, L.nbr, L.nbr_cnt, L.kmax, L.near_part, P, L.umass);
+                       L.want_nbr ? L.bits : nullptr, L.want_nbr ? L.box : nullptr, L.nbr, L.nbr_cnt, L.kmax,
+                       L.near_part, P, L.umass);
 }
 
 }  // namespace
