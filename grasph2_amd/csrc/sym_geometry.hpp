@@ -16,7 +16,7 @@ namespace grasph {
 // own slots per lane in the far field (8 was measured 12% slower: the
 // registers cap it at 2 wavefronts per SIMD with half as many workgroups'
 // worth of slots in flight)
-constexpr int kNI = 4;
+constexpr int kNI = 8;
 constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefront's slots)
 constexpr int kGroup = 1024;            // slots per group
 constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)

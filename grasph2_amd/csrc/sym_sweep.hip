@@ -33,6 +33,7 @@
 // ~12 for one interaction in the one-sided sweep.
 // Every sum is in a fixed order, so results do not vary from run to run.
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "sweep_core.hpp"
@@ -42,8 +43,7 @@ namespace grasph {
 namespace {
 
 constexpr int kFarThreads = 64 * kGroupSB;
-constexpr int kSegChunks = kSB / kChunk;  // chunks of 8 partners per far segment (one super-block)
-constexpr int kStage = 24;        // partner values per chunk: 8 partners x 3 components
+constexpr int kStageWave = 3 * kSB;  // a wavefront's staged partner side of one unit: 3 components per partner
 
 // ------------------------------------------------------------------ cross-lane reduction
 
@@ -57,53 +57,6 @@ __device__ __forceinline__ float swap16_sum(float p, float q)
     const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(p), __float_as_uint(q), false, false);
     return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-// lanes with hi keep q, the others p, each added to its DPP partner's copy
-template <int CTRL>
-__device__ __forceinline__ float dpp_pair_sum(float p, float q, bool hi)
-{
-    const float keep = hi ? q : p, give = hi ? p : q;
-    return keep + dpp<CTRL>(give);
-}
-constexpr int kRowRor8 = 0x128;       // lane ^ 8 within a row of 16
-constexpr int kRowHalfMirror = 0x141; // lane i <-> 7 - i within 8
-constexpr int kQuadXor2 = 0x4E;       // quad_perm [2,3,0,1]
-constexpr int kQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]
-
-// Sum over the 64 lanes of px[b], py[b], pz[b] for b < 8 (recursive halving:
-// value index b * 4 + component, its bits consumed by lane bits 5..1).
-// Afterwards an even lane L holds the sum for partner b = L >> 3 and
-// component (L >> 1) & 3 (3 is no component); lane L + 1 holds the same.
-// Needs all 64 lanes active.
-__device__ __forceinline__ float reduce_partners(const float (&px)[kChunk], const float (&py)[kChunk],
-                                                 const float (&pz)[kChunk], const int lane)
-{
-    float y[4][3];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        y[b][0] = swap32_sum(px[b], px[b + 4]);
-        y[b][1] = swap32_sum(py[b], py[b + 4]);
-        y[b][2] = swap32_sum(pz[b], pz[b + 4]);
-    }
-    float z[2][3];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) z[b][c] = swap16_sum(y[b][c], y[b + 2][c]);
-    const bool h3 = lane & 8, h2 = lane & 4, h1 = lane & 2;
-    float w[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) w[c] = dpp_pair_sum<kRowRor8>(z[0][c], z[1][c], h3);
-    const float v0 = dpp_pair_sum<kRowHalfMirror>(w[0], w[2], h2);  // components 0 / 2
-    const float v1 = w[1] + dpp<kRowHalfMirror>(w[1]);              // component 1 (the lanes of 3 unused)
-    const float t = dpp_pair_sum<kQuadXor2>(v0, v1, h1);
-    return t + dpp<kQuadXor1>(t);
-}
-
 // ------------------------------------------------------------------ kernels
 
 // Bounding box of every block of 64 slots (one wavefront per block).
@@ -297,93 +250,132 @@ struct FarOwn {
     CSum sx[kNI], sy[kNI], sz[kNI];
 };
 
-// Four partners (half a chunk) against the lane's kNI slots: own-side sums,
-// and the partner side summed over the slots into px/py/pz[H .. H + 3]
-// (UM: without the common mass; else with the slots' masses).
-template <int H, bool UM, bool SOFT>
-__device__ __forceinline__ void far_half(const float4 (&pj)[4], FarOwn& o, const float H2, float (&px)[kChunk],
-                                         float (&py)[kChunk], float (&pz)[kChunk])
+// ------------------------------------------------------------------ rotated far segment
+// The partners of a chunk sit one per lane of a DPP row (16 partners, the
+// same 16 in all four rows).  Rotation k pairs lane l's slots with the
+// partner of row lane (l - k) & 15: DPP row_ror:k reads it straight into the
+// subtraction, so no instruction moves partner data.  The partner side of
+// rotation k is summed over the lane's slots into its own accumulator
+// (P[k]: lane l holds partner (l - k) & 15's sum), so nothing crosses lanes
+// until the chunk's 16 x 16 x 4 pairs are done; then 15 DPP adds per
+// component bring partner p's rotations to row lane p (row_ror:16-k reads
+// lane (p + k) & 15), and two swaps add the four rows.  Per chunk and lane:
+// 64 pairs at 15 instructions, plus 51 for the reduction (0.8 per pair,
+// against 54 / 32 for a reduction of 8 broadcast partners).
+
+template <int K>
+__device__ __forceinline__ float ror(float v)
 {
+    if constexpr (K == 0)
+        return v;
+    else
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + K, 0xf, 0xf, true));
+}
+
+constexpr int kRow = 16;  // partners per rotated chunk (one DPP row)
+
+// Rotation K of a chunk: the lane's kNI slots against partner (l - K) & 15
+// of its row; own side into o.a*, partner side into P[K].
+template <int K, bool UM, bool SOFT>
+__device__ __forceinline__ void far_rotation(const float4 pj, FarOwn& o, const float H2, float (&P)[kRow][3])
+{
+    const float jx = ror<K>(pj.x), jy = ror<K>(pj.y), jz = ror<K>(pj.z);
+    const float jm = UM ? 1.0f : ror<K>(pj.w);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-#pragma unroll
-        for (int q = 0; q < kNI; ++q) {
-            const float dx = pj[c].x - o.x[q], dy = pj[c].y - o.y[q], dz = pj[c].z - o.z[q];
-            const float d2 =
-                SOFT ? fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, H2))) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float inv = __builtin_amdgcn_rsqf(d2);
-            float s = inv * inv;
-            s = s * inv;
-            if (!SOFT) s = (d2 > 0.0f) ? s : 0.0f;  // coincident particles exert nothing (computeDerivatives.h:226)
-            float sa = s, sb = s;
-            if (!UM) {
-                sa = s * pj[c].w;
-                sb = s * o.m[q];
-            }
-            o.ax[q] = fmaf(dx, sa, o.ax[q]); o.ay[q] = fmaf(dy, sa, o.ay[q]); o.az[q] = fmaf(dz, sa, o.az[q]);
-            if (q == 0) {
-                px[H + c] = dx * sb; py[H + c] = dy * sb; pz[H + c] = dz * sb;
-            } else {
-                px[H + c] = fmaf(dx, sb, px[H + c]); py[H + c] = fmaf(dy, sb, py[H + c]);
-                pz[H + c] = fmaf(dz, sb, pz[H + c]);
-            }
+    for (int q = 0; q < kNI; ++q) {
+        const float dx = jx - o.x[q], dy = jy - o.y[q], dz = jz - o.z[q];
+        const float d2 = SOFT ? fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, H2))) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float inv = __builtin_amdgcn_rsqf(d2);
+        float s = inv * inv;
+        s = s * inv;
+        if (!SOFT) s = (d2 > 0.0f) ? s : 0.0f;  // coincident particles exert nothing (computeDerivatives.h:226)
+        float sa = s, sb = s;
+        if (!UM) {
+            sa = jm * s;
+            sb = s * o.m[q];
+        }
+        o.ax[q] = fmaf(dx, sa, o.ax[q]); o.ay[q] = fmaf(dy, sa, o.ay[q]); o.az[q] = fmaf(dz, sa, o.az[q]);
+        if (q == 0) {
+            P[K][0] = dx * sb; P[K][1] = dy * sb; P[K][2] = dz * sb;
+        } else {
+            P[K][0] = fmaf(dx, sb, P[K][0]); P[K][1] = fmaf(dy, sb, P[K][1]); P[K][2] = fmaf(dz, sb, P[K][2]);
         }
     }
 }
 
-// partners b .. b + 3; BOUNDED: slots at or past n are far away and
-// massless; else b is clamped to n - 4 (a look-ahead past the end loads
-// values nobody uses)
+// partner p's rotations to row lane p: P[K] of lane (p + K) & 15 holds them
+template <int K>
+__device__ __forceinline__ void fold_rotation(const float (&P)[kRow][3], float (&T)[3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) T[c] = ror<kRow - K>(P[K][c]) + T[c];
+}
+
+template <bool UM, bool SOFT, int... K>
+__device__ __forceinline__ void far_rotations(const float4 pj, FarOwn& o, const float H2, float (&P)[kRow][3],
+                                              std::integer_sequence<int, K...>)
+{
+    (far_rotation<K, UM, SOFT>(pj, o, H2, P), ...);
+}
+template <int... K>
+__device__ __forceinline__ void fold_rotations(const float (&P)[kRow][3], float (&T)[3],
+                                               std::integer_sequence<int, K...>)
+{
+    (fold_rotation<K + 1>(P, T), ...);
+}
+
+// One chunk of 16 partners (pj: row lane l holds partner l) against the
+// lane's kNI slots.  Own side into o.a*; returns the partner side: lanes
+// 0-15 hold partner l's x, lanes 16-31 partner l - 16's z, lanes 32-47
+// partner l - 32's y (UM: without the common mass; else with the slots' masses).
+template <bool UM, bool SOFT>
+__device__ __forceinline__ float far_chunk_rot(const float4 pj, FarOwn& o, const float H2)
+{
+    float P[kRow][3];
+    far_rotations<UM, SOFT>(pj, o, H2, P, std::make_integer_sequence<int, kRow>{});
+    float T[3] = {P[0][0], P[0][1], P[0][2]};
+    fold_rotations(P, T, std::make_integer_sequence<int, kRow - 1>{});
+    // rows 0 + 2 and 1 + 3: x to lanes 0-31, y to lanes 32-63; z everywhere
+    const float xy = swap32_sum(T[0], T[1]);
+    const float zz = swap32_sum(T[2], T[2]);
+    // row 0: x, row 1: z, row 2: y (row 3: z again)
+    return swap16_sum(xy, zz);
+}
+
+// partner b + (lane & 15) of an instance of n particles; BOUNDED: slots at
+// or past n are far away and massless, else b + 15 < n or the values are
+// unused (a look-ahead past the end of the segment, clamped to n - 1)
 template <bool BOUNDED>
-__device__ __forceinline__ void load_partners(const float4* __restrict__ xs, const int b, const int n, float4 (&pj)[4])
+__device__ __forceinline__ float4 load_partner_row(const float4* __restrict__ xs, const int b, const int n,
+                                                   const int lane)
 {
-    if (BOUNDED) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float4 v = xs[min(b + c, n - 1)];
-            pj[c] = (b + c < n) ? v : make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
-        }
-    } else {
-        const int bb = min(b, n - 4);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) pj[c] = xs[bb + c];
-    }
+    const int j = b + (lane & (kRow - 1));
+    const float4 v = xs[min(j, n - 1)];
+    if (BOUNDED && j >= n) return make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
+    return v;
 }
 
-// s_waitcnt lgkmcnt(0), vmcnt and expcnt left free: scalar loads may
-// return out of order, so waiting for one waits for all; doing it at the
-// end of each half keeps the next half's load in flight through a whole
-// half of compute.
-__device__ __forceinline__ void wait_scalar_loads() { __builtin_amdgcn_s_waitcnt(0xC07F); }
-
-// stage slot of this lane's reduced value, or -1
-__device__ __forceinline__ int stage_index(const int lane)
+// stage index of this lane's value from far_chunk_rot: 3 partner + component
+// (rows 1 and 3 hold the same z, bit for bit, and store it to the same
+// place: every lane stores, so the store needs no branch, behind which the
+// compiler would sink the chunk's own-side sums)
+__device__ __forceinline__ int stage_index_rot(const int lane)
 {
-    const int comp = (lane >> 1) & 3;
-    return ((lane & 1) == 0 && comp < 3) ? (lane >> 3) * 3 + comp : -1;
+    const int row = lane >> 4, p = lane & (kRow - 1);
+    return 3 * p + (row == 0 ? 0 : row == 2 ? 1 : 2);
 }
 
 // One partner super-block [b0, b0 + 256) against the wavefront's slots;
-// the reduced partner side of chunk q goes to my_stage[q * 24 + stg].
+// the reduced partner side of partner p, component c goes to my_stage[3 p + c].
 template <bool UM, bool SOFT, bool BOUNDED>
-__device__ __forceinline__ void far_segment(const float4* __restrict__ xs, const int n, const int b0, FarOwn& o,
-                                            const float H2, const int lane, const int stg, float* my_stage)
+__device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, const int n, const int b0, FarOwn& o,
+                                                const float H2, const int lane, const int stg, float* my_stage)
 {
-    float4 A[4], B[4];
-    load_partners<BOUNDED>(xs, b0, n, A);
-    wait_scalar_loads();
-    for (int q = 0; q < kSegChunks; ++q) {
-        float px[kChunk], py[kChunk], pz[kChunk];
-        load_partners<BOUNDED>(xs, b0 + kChunk * q + 4, n, B);
-        __builtin_amdgcn_sched_barrier(0);  // issue the load before the half it overlaps
-        far_half<0, UM, SOFT>(A, o, H2, px, py, pz);
-        wait_scalar_loads();
-        load_partners<BOUNDED>(xs, b0 + kChunk * (q + 1), n, A);  // past the segment on the last chunk: unused
-        __builtin_amdgcn_sched_barrier(0);
-        far_half<4, UM, SOFT>(B, o, H2, px, py, pz);
-        const float r = reduce_partners(px, py, pz, lane);
-        if (stg >= 0) my_stage[q * kStage + stg] = r;
-        wait_scalar_loads();
+    float4 A = load_partner_row<BOUNDED>(xs, b0, n, lane);
+    for (int q = 0; q < kSB / kRow; ++q) {
+        const float4 B = load_partner_row<BOUNDED>(xs, b0 + kRow * (q + 1), n, lane);  // unused after the last
+        my_stage[3 * kRow * q + stg] = far_chunk_rot<UM, SOFT>(A, o, H2);
+        A = B;
     }
 }
 
@@ -433,12 +425,11 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
                                          float* my_stage, const int lane, const int tid, const int stg)
 {
     if (!wave_on) {
-        if (stg >= 0)
-            for (int q = 0; q < kSegChunks; ++q) my_stage[q * kStage + stg] = 0.0f;
+        for (int q = 0; q < kSB / kRow; ++q) my_stage[3 * kRow * q + stg] = 0.0f;
     } else if ((n % kSB) != 0 && J == nsb - 1) {
-        far_segment<UM, SOFT, true>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+        far_segment_rot<UM, SOFT, true>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
     } else {
-        far_segment<UM, SOFT, false>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+        far_segment_rot<UM, SOFT, false>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
     }
     // own side: plain sums over the segment, then compensated
 #pragma unroll
@@ -453,14 +444,13 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
     __syncthreads();
     // partner side: the wavefronts' values in wavefront order
 #pragma unroll
-    for (int r = 0; r < kSegChunks * kStage / kFarThreads; ++r) {
-        const int idx = tid + kFarThreads * r;
+    for (int r = 0; r < kStageWave / kFarThreads; ++r) {
+        const int idx = tid + kFarThreads * r;  // 3 partner + component
         float v = 0.0f;
 #pragma unroll
-        for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kSegChunks * kStage + idx];
-        const int q = idx / kStage, e = idx - q * kStage;
-        const int b = e / 3, comp = e - 3 * b;
-        const int B = kSB * J + kChunk * q + b;
+        for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kStageWave + idx];
+        const int b = idx / 3, comp = idx - 3 * b;
+        const int B = kSB * J + b;
         if (B < n) {
             const int64_t h = B / kGroup;
             const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
@@ -484,8 +474,8 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
     const int64_t u0 = (int64_t)k * U / W, u1 = (int64_t)(k + 1) * U / W;
     if (u0 >= u1) return;  // the whole workgroup leaves: no barrier is reached
     int64_t g = row_of_unit(u0, nsb), s = u0 - row_prefix(g, nsb);
-    const int stg = stage_index(lane);
-    float* const my_stage = stage + (size_t)w * kSegChunks * kStage;
+    const int stg = stage_index_rot(lane);
+    float* const my_stage = stage + (size_t)w * kStageWave;
     FarOwn o;
     bool wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
     for (int64_t u = u0; u < u1; ++u) {
@@ -503,12 +493,12 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
 }
 
 template <bool SOFT>
-__global__ __launch_bounds__(kFarThreads) void far_kernel(const float4* __restrict__ xs, const int n,
+__global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __restrict__ xs, const int n,
                                                             const int64_t nsb, const int64_t U, const int W,
                                                             float4* __restrict__ apart, float* __restrict__ bpart,
                                                             const float H2, const float* __restrict__ umass)
 {
-    __shared__ float stage[kGroupSB * kSegChunks * kStage];
+    __shared__ float stage[kGroupSB * kStageWave];
     if (umass[0] != 0.0f)
         far_work<true, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, umass[1], stage);
     else
@@ -663,16 +653,16 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
 // Persistent far field over the batch's rows; every wavefront leaves once
 // the counter passes nrows.
 template <bool SOFT>
-__global__ __launch_bounds__(kFarThreads) void sym_batch_far_kernel(
+__global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
     const float4* __restrict__ xs, const SymInst* __restrict__ sinst, const BatchInst* __restrict__ insts,
     const SymRow* __restrict__ rows, const int nrows, int* __restrict__ queue, float4* __restrict__ apart,
     float* __restrict__ bpart, const float* __restrict__ umass)
 {
-    __shared__ float stage[kGroupSB * kSegChunks * kStage];
+    __shared__ float stage[kGroupSB * kStageWave];
     __shared__ int row_sh;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
-    const int stg = stage_index(lane);
-    float* const my_stage = stage + (size_t)w * kSegChunks * kStage;
+    const int stg = stage_index_rot(lane);
+    float* const my_stage = stage + (size_t)w * kStageWave;
     for (;;) {
         if (tid == 0) row_sh = atomicAdd(queue, 1);
         __syncthreads();
