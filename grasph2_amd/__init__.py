@@ -135,6 +135,11 @@ class Context:
     def set_option(self, key, value):
         return self._check(self._lib.grasph_set_option(self.handle, key.encode(), int(value)))
 
+    def get_option(self, key):
+        v = ctypes.c_int64()
+        self._check(self._lib.grasph_get_option(self.handle, key.encode(), ctypes.byref(v)))
+        return v.value
+
     @staticmethod
     def _s(stream):
         return stream.handle if stream is not None else None
@@ -235,7 +240,9 @@ class Context:
         ni = (ctypes.c_int32 * 4)()
         w, bp = ctypes.c_int32(), ctypes.c_int32()
         self._check(self._lib.grasph_get_batch_tuning(self.handle, ni, ctypes.byref(w), ctypes.byref(bp)))
-        return {"ni": list(ni), "waves_per_block": w.value, "blocks_per_cu": bp.value}
+        sym = self.get_option("batch_symmetric")
+        return {"ni": list(ni), "waves_per_block": w.value, "blocks_per_cu": bp.value,
+                "symmetric_min": self.get_option("batch_symmetric_min") if sym else None}
 
     def close(self):
         if self.handle is not None and self.handle.value:

@@ -133,6 +133,7 @@ _SIGNATURES = {
     "grasph_batch_step_loop": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_int,
                                               ctypes.c_void_p]),
     "grasph_autotune": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Batch), ctypes.c_char_p, ctypes.c_void_p]),
+    "grasph_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_int64)]),
     "grasph_get_batch_tuning": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32),
                                                _P(ctypes.c_int32)]),
     "grasph_batch_get_current_timesteps": (ctypes.c_int, [ctypes.c_void_p, _P(Batch), _P(ctypes.c_double)]),

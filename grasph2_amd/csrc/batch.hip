@@ -620,8 +620,9 @@ std::string graph_key(const Ctx& c, const BatchState& b, const grasph_particles*
     return k.str();
 }
 
-// time the batched sweep alone (min of reps), milliseconds
-int time_sweep(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, int reps, double* ms)
+// time the batched sweep (min of reps), milliseconds: the one-sided
+// persistent sweep alone, or (all) with the symmetric sweep of the larger instances
+int time_sweep(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, int reps, double* ms, bool all = false)
 {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
@@ -632,7 +633,8 @@ int time_sweep(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, 
     c.profile = false;
     for (int k = 0; k < reps + 1 && !r; ++k) {
         hipEventRecord(e0, s);
-        r = launch_sweep_only(c, b, p, s);
+        if (!all || b.n_items > 0) r = launch_sweep_only(c, b, p, s);
+        if (!r && all && c.batch_symmetric) r = launch_sym_sweep_batch(c, b, p, s, b.n_items > 0);
         hipEventRecord(e1, s);
         if (!r && hipEventSynchronize(e1) == hipSuccess) {
             float t = 0;
@@ -656,7 +658,7 @@ std::string tune_signature(Ctx& c, const BatchState& b)
     for (int k = 0; k < b.n_inst; ++k) cnt[bucket_of(b.offsets[k + 1] - b.offsets[k])]++;
     std::ostringstream s;
     s << dev << ":cus=" << device_cus(c) << ":buckets=" << cnt[0] << "," << cnt[1] << "," << cnt[2] << "," << cnt[3]
-      << ":flags=" << b.flags;
+      << ":flags=" << b.flags << ":sym=" << c.batch_symmetric;
     return s.str();
 }
 
@@ -823,16 +825,23 @@ int grasph_autotune(grasph_ctx* ctx, const grasph_particles* p, const grasph_bat
             std::istringstream ls(line);
             std::string key;
             int wpb, bpc, n0, n1, n2, n3;
-            if ((ls >> key >> wpb >> bpc >> n0 >> n1 >> n2 >> n3) && key == sig && wpb >= 1 && wpb <= 4 && bpc >= 1 &&
-                bpc <= 64 && n0 >= 1 && n0 <= 2 && n1 >= 1 && n1 <= 2 && n2 >= 1 && n2 <= 2 && n3 >= 1 && n3 <= 2) {
+            int64_t smin;
+            if ((ls >> key >> wpb >> bpc >> n0 >> n1 >> n2 >> n3 >> smin) && key == sig && wpb >= 1 && wpb <= 4 &&
+                bpc >= 1 && bpc <= 64 && n0 >= 1 && n0 <= 2 && n1 >= 1 && n1 <= 2 && n2 >= 1 && n2 <= 2 && n3 >= 1 &&
+                n3 <= 2 && smin >= -1 && smin <= (int64_t(1) << 30) && (smin < 0 || c.batch_symmetric)) {
                 c.batch_wpb = wpb;
                 c.batch_bpc = bpc;
                 c.batch_ni[0] = n0; c.batch_ni[1] = n1; c.batch_ni[2] = n2; c.batch_ni[3] = n3;
+                c.batch_symmetric = smin >= 0;
+                c.batch_symmetric_min = (int)std::max<int64_t>(smin, 0);
                 return grasph::prepare(c, p, b);
             }
         }
     }
     if ((r = grasph::ensure_order(c, B, p, s, false))) return r;
+    // the one-sided persistent sweep over every instance for its launch shape
+    const bool sym_allowed = c.batch_symmetric && (B.flags & (GRASPH_SPH | GRASPH_SELF_GRAVITY));
+    c.batch_symmetric = 0;
     // launch shape with one particle per lane everywhere
     for (int q = 0; q < 4; ++q) c.batch_ni[q] = 1;
     if ((r = grasph::prepare(c, p, b))) return r;
@@ -867,11 +876,36 @@ int grasph_autotune(grasph_ctx* ctx, const grasph_particles* p, const grasph_bat
         else
             c.batch_ni[q] = 1;
     }
+    // the symmetric sweep from which size bucket on (or for none), timed
+    // with the one-sided sweep of the instances below it
+    int64_t sym_min = -1;
+    if (sym_allowed) {
+        const int64_t cands[] = {0, 2048, 4096, 8192, -1};
+        double best_all = 1e30;
+        int last = -2;
+        for (const int64_t cand : cands) {
+            int count = 0;  // instances the symmetric sweep takes
+            for (int k = 0; k < B.n_inst; ++k) count += cand >= 0 && B.offsets[k + 1] - B.offsets[k] >= cand;
+            if (count == last) continue;  // the same split as the candidate before
+            last = count;
+            c.batch_symmetric = cand >= 0;
+            c.batch_symmetric_min = (int)std::max<int64_t>(cand, 0);
+            if ((r = grasph::prepare(c, p, b))) return r;
+            double t;
+            if ((r = grasph::time_sweep(c, B, p, s, 3, &t, true))) return r;
+            if (t < 0.99 * best_all) {
+                best_all = t;
+                sym_min = cand;
+            }
+        }
+    }
+    c.batch_symmetric = sym_min >= 0;
+    c.batch_symmetric_min = (int)std::max<int64_t>(sym_min, 0);
     if ((r = grasph::prepare(c, p, b))) return r;
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::app);
         out << sig << ' ' << c.batch_wpb << ' ' << c.batch_bpc << ' ' << c.batch_ni[0] << ' ' << c.batch_ni[1] << ' '
-            << c.batch_ni[2] << ' ' << c.batch_ni[3] << '\n';
+            << c.batch_ni[2] << ' ' << c.batch_ni[3] << ' ' << sym_min << '\n';
     }
     return 0;
 }

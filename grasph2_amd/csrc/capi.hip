@@ -593,6 +593,28 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     return 0;
 }
 
+int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value)
+{
+    if (!ctx || !key || !value) return GRASPH_E_INVALID;
+    const Ctx& c = ctx->c;
+    const std::string k(key);
+    if (k == "max_neighbours") *value = c.max_neighbours;
+    else if (k == "reorder_interval") *value = c.reorder_interval;
+    else if (k == "sweep_i_per_lane") *value = c.sweep_ni;
+    else if (k == "timestep_reset") *value = c.timestep_reset ? 1 : 0;
+    else if (k == "sweep_slices") *value = c.sweep_slices;
+    else if (k == "sweep_symmetric") *value = c.sweep_symmetric;
+    else if (k == "sym_workgroups") *value = c.sym_workgroups;
+    else if (k == "batch_symmetric") *value = c.batch_symmetric;
+    else if (k == "batch_symmetric_min") *value = c.batch_symmetric_min;
+    else if (k == "sym_near_shares") *value = c.sym_near_shares;
+    else if (k == "batch_waves_per_block") *value = c.batch_wpb;
+    else if (k == "batch_blocks_per_cu") *value = c.batch_bpc;
+    else if (k == "profile") *value = c.profile ? 1 : 0;
+    else return GRASPH_E_INVALID;
+    return 0;
+}
+
 int grasph_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
 {
     return grasph::derivatives(ctx, p, prm, stream, 0);

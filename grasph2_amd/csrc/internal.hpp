@@ -34,7 +34,7 @@ struct Ctx {
     int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 49152 particles
     int sym_workgroups = 0;   // workgroups of its far kernel (0: 2 wavefronts per SIMD)
     int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
-    int batch_symmetric = 0;  // batches: the symmetric sweep for instances of batch_symmetric_min particles or more
+    int batch_symmetric = 1;  // batches: the symmetric sweep for instances of batch_symmetric_min particles or more
     int batch_symmetric_min = 0;
     int cus = 256;         // compute units of the device
     bool profile = false;

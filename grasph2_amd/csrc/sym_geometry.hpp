@@ -1,6 +1,6 @@
 // sym_geometry.hpp -- how the symmetric sweep (sym_sweep.hip) cuts its work:
-// slots in super-blocks of 256 (one far-field wavefront, 4 slots per lane)
-// and groups of 1024 (one far-field workgroup).  The far field's rows are
+// slots in super-blocks of 512 (one far-field wavefront, 8 slots per lane)
+// and groups of 1024 (one far-field workgroup of 2 wavefronts).  The far field's rows are
 // group g against the super-blocks of the later groups; one unit is one
 // partner super-block.  Host and device code use these functions, and
 // tests/cpp/sym_geometry_check.cpp checks them on the host (every far pair
@@ -13,9 +13,9 @@
 
 namespace grasph {
 
-// own slots per lane in the far field (8 was measured 12% slower: the
-// registers cap it at 2 wavefronts per SIMD with half as many workgroups'
-// worth of slots in flight)
+// own slots per lane in the far field: the partner-side reduction and the
+// partners' DPP moves of a chunk are shared by the lane's slots (8: ~216
+// VGPRs, 2 wavefronts per SIMD; 4 was measured 7% slower)
 constexpr int kNI = 8;
 constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefront's slots)
 constexpr int kGroup = 1024;            // slots per group

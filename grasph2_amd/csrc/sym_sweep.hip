@@ -3,8 +3,8 @@
 //
 // Same results as the sweep of sweep.hip (softened gravity of
 // calcAcceleration, src/computeDerivatives.h:229-235, plus the neighbour
-// lists of :72,122,238), organised by super-blocks of 256 consecutive slots
-// and groups of 4 super-blocks (1024 slots):
+// lists of :72,122,238), organised by super-blocks of 512 consecutive slots
+// and groups of 2 super-blocks (1024 slots):
 //  - near field, one wavefront per block of 64 slots, partner blocks in
 //    increasing slot order through the candidate sweep of sweep_core.hpp:
 //    the blocks of its own group with gravity, and (when lists are wanted)
@@ -14,23 +14,28 @@
 //    it), so the lists come out complete and in slot order, exactly as the
 //    one-range sweep writes them;
 //  - far field: every super-block pair of different groups (I < J) is
-//    computed once, for both sides (Newton's third law).  A workgroup of 4 wavefronts owns a
-//    group, a wavefront one super-block (4 slots per lane), and streams the
-//    later groups' particles through SGPRs, 4 at a time in ping-pong.  A
-//    lane sums its own slots' gravity in registers and the partners' side
-//    over its 4 slots; every 8 partners, the partner side (8 x 3 values) is
-//    reduced across the 64 lanes with v_permlane32_swap, v_permlane16_swap
-//    and DPP adds (~54 instructions), staged in LDS, summed over the 4
-//    wavefronts in wavefront order and stored as one partial per (group,
-//    partner);
+//    computed once, for both sides (Newton's third law).  A workgroup of 2
+//    wavefronts owns a group, a wavefront one super-block (8 slots per
+//    lane).  The later groups' particles come 16 at a time, one per lane of
+//    each 16-lane DPP row; rotation k (row_ror:k) pairs a lane's slots with
+//    the partner k lanes away in its row.  A lane sums its own slots'
+//    gravity in registers, and the partner side of each rotation over its
+//    8 slots in one accumulator per rotation, so nothing crosses lanes
+//    until the 16 x 16 x 8 pairs of the chunk are done; 15 DPP adds per
+//    component then gather each partner's rotations in its own lane, two
+//    swaps add the four rows, and the values are staged in LDS, summed
+//    over the 2 wavefronts in wavefront order and stored as one partial per
+//    (group, partner);
 //  - a combine kernel adds, per slot and in a fixed order, the near sum,
 //    the far own-side fragments and the far partner-side partials
 //    (compensated), then finishes the acceleration as the one-range sweep.
 // Per far pair: 3 sub, 3 FMA for r^2 + H^2, one v_rsq_f32, 2 mul, 3 FMA for
 // the own side and 3 FMA for the partner side (2 more mul with unequal
-// masses), plus 54 / 32 for the reduction: ~16.7 VALU instructions per
-// wavefront and partner for two interactions per lane and slot, against
-// ~12 for one interaction in the one-sided sweep.
+// masses), plus, per chunk and lane, 45 DPP moves of the partners' x, y, z
+// (a DPP move with several uses is not folded into them) and 51 for the
+// reduction, over 128 pairs: ~15.8 VALU instructions per wavefront and
+// partner for two interactions per lane and slot, against ~12 for one
+// interaction in the one-sided sweep.
 // Every sum is in a fixed order, so results do not vary from run to run.
 #include <algorithm>
 #include <utility>
@@ -365,7 +370,7 @@ __device__ __forceinline__ int stage_index_rot(const int lane)
     return 3 * p + (row == 0 ? 0 : row == 2 ? 1 : 2);
 }
 
-// One partner super-block [b0, b0 + 256) against the wavefront's slots;
+// One partner super-block [b0, b0 + kSB) against the wavefront's slots;
 // the reduced partner side of partner p, component c goes to my_stage[3 p + c].
 template <bool UM, bool SOFT, bool BOUNDED>
 __device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, const int n, const int b0, FarOwn& o,
@@ -379,8 +384,8 @@ __device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, c
     }
 }
 
-// The own slots of wavefront w in row g (slots 1024 g + 256 w + 64 q + lane,
-// q < 4) of an instance of n particles at xs; slots past n sit far away and
+// The own slots of wavefront w in row g (slots 1024 g + kSB w + 64 q + lane,
+// q < kNI) of an instance of n particles at xs; slots past n sit far away and
 // massless.  Returns whether the wavefront has any slot.
 __device__ __forceinline__ bool far_load_row(const float4* __restrict__ xs, const int n, const int64_t nsb,
                                              const int64_t g, const int w, const int lane, FarOwn& o)

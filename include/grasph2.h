@@ -157,10 +157,15 @@ int grasph_destroy(grasph_ctx* ctx);
  * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
  * 0 = chosen from the occupancy), "batch_symmetric" (1 = batches use the
  * symmetric sweep for their instances of at least "batch_symmetric_min"
- * particles, default 0; the smaller ones keep the one-sided persistent
- * sweep.  Off by default: batched results then equal the single-instance
- * path's only to FP32 rounding instead of bit for bit). */
+ * particles (default 0), the smaller ones the one-sided persistent sweep,
+ * the default; gravity then equals the single-instance path's to FP32
+ * rounding, the neighbour sums bit for bit; 0 = every instance takes the
+ * one-sided persistent sweep, whose results equal the single-instance
+ * path's bit for bit when that path is not j-split). */
 int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
+/* The current value of an option of grasph_set_option (the autotuner sets
+ * some of them); GRASPH_E_INVALID for an unknown key. */
+int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value);
 
 /* ---- the hot path ------------------------------------------------------- */
 

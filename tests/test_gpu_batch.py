@@ -1,7 +1,10 @@
-"""Batched ensembles: the batched launches, the captured-graph steps, the
-per-instance launch loop and the autotuned configurations give the same
-particles, bit for bit, as the single-instance path on each instance (same
-kernels, same summation order per particle); the single-instance path is
+"""Batched ensembles: with the one-sided batched sweep for every instance
+(option batch_symmetric = 0), the batched launches, the captured-graph
+steps, the per-instance launch loop and the autotuned configurations give
+the same particles, bit for bit, as the single-instance path on each
+instance (same kernels, same summation order per particle); the default
+symmetric batched sweep is checked in test_gpu_batch_symmetric.py.  The
+single-instance path is
 itself checked against the oracle in test_gpu_parity.py, and one instance
 is checked against the FP64 oracle here too."""
 import numpy as np
@@ -58,12 +61,15 @@ def assert_same(got_list, ref_list, fields):
 
 @pytest.fixture
 def unsplit(gpu_ctx):
-    """The single-instance sweep without its j-split (the batched sweep never
-    splits an instance's j range), so both sum in the same order; the
-    launch-loop contexts take the option from the parent context."""
+    """The single-instance sweep without its j-split and the batch with its
+    one-sided persistent sweep for every instance (which never splits an
+    instance's j range), so both sum in the same order; the launch-loop
+    contexts take the options from the parent context."""
     gpu_ctx.set_option("sweep_slices", 1)
+    gpu_ctx.set_option("batch_symmetric", 0)
     yield gpu_ctx
     gpu_ctx.set_option("sweep_slices", 0)
+    gpu_ctx.set_option("batch_symmetric", 1)
 
 
 def test_batched_derivatives_match_single_instances(unsplit):
@@ -110,7 +116,8 @@ def test_graph_steps_match_loop_and_eager(unsplit):
         x.free()
 
 
-def test_autotune_keeps_results(gpu_ctx, tmp_path):
+def test_autotune_keeps_results(unsplit, tmp_path):
+    gpu_ctx = unsplit  # the one-sided batched sweep, whose launch shape the autotuner picks
     insts, params, host, offsets = make_batch(sizes=(900, 2100, 4500, 9000, 1500), seed=2)
     b = gpu_ctx.batch(offsets, params)
     d = g.DeviceParticles.from_host(host)

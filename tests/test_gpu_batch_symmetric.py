@@ -27,7 +27,7 @@ def sym_batch_ctx(gpu_ctx):
     gpu_ctx.set_option("batch_symmetric", 1)
     gpu_ctx.set_option("batch_symmetric_min", 1000)
     yield gpu_ctx
-    gpu_ctx.set_option("batch_symmetric", 0)
+    gpu_ctx.set_option("batch_symmetric", 1)  # the defaults
     gpu_ctx.set_option("batch_symmetric_min", 0)
 
 
@@ -88,3 +88,49 @@ def test_graph_steps_match_eager_and_repeat(sym_batch_ctx):
     for k, (a, r) in enumerate(zip(x, y)):
         for f in ("acc", "density", "dstress_dt", "maxvsig"):
             assert np.array_equal(a[f], r[f]), (k, f)
+
+
+def test_autotune_picks_symmetric_split(gpu_ctx, tmp_path):
+    """The autotuner also picks from which size bucket on the batch takes the
+    symmetric sweep (or none); results stay within FP32 rounding of the
+    one-sided batch, the list passes bit for bit, and the cached choice is
+    found again."""
+    print("batch symmetric autotune", flush=True)
+    ctx = gpu_ctx
+    insts, params, host, offsets = make_batch(sizes=(1500, 3000, 5000, 9000, 1100, 2200), seed=3)
+    ref = run_batch_with(ctx, host, offsets, params, {"batch_symmetric": 0})
+    cache = str(tmp_path / "tune.txt")
+    try:
+        b = ctx.batch(offsets, params)
+        d = g.DeviceParticles.from_host(host)
+        ctx.autotune(d, b, cache)
+        t = ctx.batch_tuning()
+        assert t["symmetric_min"] in (None, 0, 2048, 4096, 8192), t
+        d.upload(host)
+        ctx.batch_compute_derivatives(d, b)
+        g.hip.synchronize()
+        got = g.unpack_batch(d.download(), offsets)
+        for k, (a, r) in enumerate(zip(got, ref)):
+            mx, nr = vec_errors(a["acc"], r["acc"])
+            assert nr <= 1e-6 and mx <= 1e-5, (k, mx, nr)
+            assert np.array_equal(a["density"], r["density"]), k
+        # from the cache: same choice
+        ctx.set_option("batch_symmetric", 1)
+        ctx.set_option("batch_symmetric_min", 0)
+        ctx.autotune(d, b, cache)
+        assert ctx.batch_tuning() == t
+        d.free()
+    finally:
+        ctx.set_option("batch_symmetric", 1)  # the defaults
+        ctx.set_option("batch_symmetric_min", 0)
+
+
+def run_batch_with(ctx, host, offsets, params, opts):
+    old = {k: ctx.get_option(k) for k in opts}
+    for k, v in opts.items():
+        ctx.set_option(k, v)
+    try:
+        return run_batch(ctx, host, offsets, params)
+    finally:
+        for k, v in old.items():
+            ctx.set_option(k, v)
