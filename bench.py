@@ -10,11 +10,16 @@ FP64 position drift).  Prints one JSON line.
 
 value = pair interactions per second = N (N - 1) / (seconds per step): every
 step evaluates softened gravity on all ordered pairs i != j.
-roofline: the all-pairs sweep kernel (the dominant one), FP32 vector ALU
-bound; its algorithmic FLOPs per launch are 20 per gravity pair (FMA = 2,
-rsqrt = 1) over N (N - 1) pairs (its neighbour search rides on the same r^2
-and adds no counted FLOPs), divided by its mean duration from HIP events
-recorded around every launch on the launch stream over the timed steps.
+roofline: the dominant kernel, FP32 vector ALU bound, at 20 algorithmic
+FLOPs per ordered gravity pair (FMA = 2, rsqrt = 1), divided by its mean
+duration from HIP events recorded around every launch on the stream it
+runs on, over the timed steps.  From 49152 particles (the symmetric sweep)
+that kernel is the far field (far_kernel): the ordered pairs of particles in
+different groups of 1024 slots, N (N - 1) minus the in-group pairs, 99% of
+the pairs at N = 131072; it runs on a stream of its own, overlapping the
+near field, the list merge and the density and strain passes.  Below, it
+is the one-sided sweep over all N (N - 1) pairs (its neighbour search rides
+on the same r^2 and adds no counted FLOPs).
 cpu_baseline: the CPU oracle in FP32 (the same op structure as the
 reference's host path) on a bounded sample of target particles i against
 all j, all three passes, OpenMP threads as reported.
@@ -34,6 +39,7 @@ PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (AMD published)
 # from this many particles on, gravity between groups of 1024 is evaluated once per pair for both sides.
 # The roofline stays algorithmic either way: 20 FLOP per ordered pair.
 SYM_AUTO_MIN = 49152
+SYM_GROUP = 1024  # slots per group of the symmetric sweep; pairs inside a group are the near field's
 FLOP_GRAVITY_PAIR = 20
 FLOP_DENSITY_NBR = 9
 FLOP_STRAIN_NBR = 63
@@ -162,9 +168,23 @@ def main():
     pairs = float(n) * (n - 1)
     value = pairs / (ms_step * 1e-3)
 
-    sweep_ms = kt["sweep"][0] / max(kt["sweep"][1], 1)
+    sym = n >= SYM_AUTO_MIN
+    # all kernels of the all-pairs sweep, summed per evaluation (with the symmetric sweep, intervals that overlap)
+    sweep_ms = (kt["sweep"][0] + kt["far"][0]) / args.steps
     sweep_flops = FLOP_GRAVITY_PAIR * pairs
-    achieved = sweep_flops / (sweep_ms * 1e-3) / 1e12
+    if sym:
+        sizes = np.full(n // SYM_GROUP, SYM_GROUP, dtype=np.float64)
+        if n % SYM_GROUP:
+            sizes = np.append(sizes, n % SYM_GROUP)
+        dom_pairs = pairs - float(np.sum(sizes * (sizes - 1)))
+        dom_ms = kt["far"][0] / max(kt["far"][1], 1)
+        dom_name = "far_kernel (symmetric sweep, far field)"
+    else:
+        dom_pairs = pairs
+        dom_ms = sweep_ms
+        dom_name = "sweep (one-sided)"
+    dom_flops = FLOP_GRAVITY_PAIR * dom_pairs
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
     integ_ms = kt["integrate"][0] / max(kt["integrate"][1], 1)
     integ_bytes = n * (INTEGRATOR_BYTES_SOLID if solid else INTEGRATOR_BYTES_NBODY)
     step_flops = sweep_flops
@@ -173,7 +193,8 @@ def main():
                                   (FLOP_ACCEL_NBR_SOLID if solid else FLOP_ACCEL_NBR_LIQUID))
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sweep_%s_%d_traffic.json" % (args.workload, n))
+    # HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    tpath = os.path.join(ROOT, "profiles", "%s_%s_%d_traffic.json" % ("far_kernel" if sym else "sweep", args.workload, n))
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -195,8 +216,11 @@ def main():
                    "dt": prm.fixed_timestep, "fp64_drift": True},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "sweep", "kernel_ms": sweep_ms, "flops_per_launch": sweep_flops,
-                     "sweep": "symmetric" if n >= SYM_AUTO_MIN else "one-sided"},
+                     "kernel": dom_name, "kernel_ms": dom_ms, "flops_per_launch": dom_flops,
+                     "pairs_per_launch": dom_pairs},
+        "sweep": {"kind": "symmetric" if sym else "one-sided", "kernel_ms": sweep_ms,
+                  "tflops_all_pairs": sweep_flops / (sweep_ms * 1e-3) / 1e12,
+                  "frac_all_pairs": sweep_flops / (sweep_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1]},
         "step_tflops": step_flops / (ms_step * 1e-3) / 1e12,
         "neighbour_pairs": nb_pairs,

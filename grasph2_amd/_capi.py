@@ -33,8 +33,8 @@ E_INVALID, E_HIP, E_UNSUPPORTED, E_NOMEM = -1, -2, -3, -4
 
 INV_EKIN, INV_EPOT, INV_EINT, INV_PX, INV_PY, INV_PZ, INV_LX, INV_LY, INV_LZ, INV_MASS = range(10)
 NINV = 10
-NKERNEL_CLASSES = 7
-KERNEL_CLASSES = ("sweep", "density", "strain", "accel", "integrate", "reorder", "other")
+NKERNEL_CLASSES = 8
+KERNEL_CLASSES = ("sweep", "density", "strain", "accel", "integrate", "reorder", "other", "far")
 
 
 class Params(ctypes.Structure):

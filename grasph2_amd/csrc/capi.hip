@@ -200,7 +200,13 @@ int sym_near_shares(const Ctx& c, int64_t n)
     return (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + blocks - 1) / blocks));
 }
 
-int run_sym_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s)
+// The symmetric sweep, begun: buffers, the far field forked onto the side
+// stream (launched first, so its workgroups are placed first), the near
+// field on s.  The far field needs only the slot-ordered positions and the
+// mass record, both written on s before the fork; the work enqueued on s
+// after this (list merge, density and strain passes) overlaps it.
+// sym_sweep_end joins the streams and combines.
+int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s, SymLaunch& S)
 {
     const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.nslices);
     int r;
@@ -210,7 +216,6 @@ int run_sym_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t 
         (r = grow(c, c.sym_apart, c.sym_apart_cap, G.apart_elems, "symmetric sweep own-side sums")) ||
         (r = grow(c, c.sym_bpart, c.sym_bpart_cap, G.bpart_elems, "symmetric sweep partner-side sums")))
         return r;
-    SymLaunch S;
     std::memset(&S, 0, sizeof S);
     S.xs = L.xs ? L.xs : L.posm;
     S.perm = L.perm;
@@ -233,9 +238,33 @@ int run_sym_sweep(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t 
     S.near_part = c.sym_near;
     S.apart = c.sym_apart;
     S.bpart = c.sym_bpart;
+    hipError_t e;
+    if (sym_has_far(S)) {
+        if (!c.side && (e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking)) != hipSuccess)
+            return hip_fail(c, e, "far-field stream");
+        if ((!c.ev_fork && (e = hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming)) != hipSuccess) ||
+            (!c.ev_join && (e = hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming)) != hipSuccess))
+            return hip_fail(c, e, "far-field events");
+        if ((e = hipEventRecord(c.ev_fork, s)) != hipSuccess || (e = hipStreamWaitEvent(c.side, c.ev_fork, 0)) != hipSuccess)
+            return hip_fail(c, e, "far-field fork");
+        {
+            ProfScope ps(&c, c.side, KC_FAR);
+            if ((e = launch_sym_far(S, P, c.side)) != hipSuccess) return hip_fail(c, e, "far-field launch");
+        }
+        if ((e = hipEventRecord(c.ev_join, c.side)) != hipSuccess) return hip_fail(c, e, "far-field join event");
+    }
     ProfScope ps(&c, s, KC_SWEEP);
-    hipError_t e = launch_sym_sweep(S, P, s);
-    if (e != hipSuccess) return hip_fail(c, e, "symmetric sweep launch");
+    if ((e = launch_sym_near(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field launch");
+    return 0;
+}
+
+int sym_sweep_end(Ctx& c, const SymLaunch& S, const DevParams& P, hipStream_t s)
+{
+    hipError_t e;
+    if (sym_has_far(S) && (e = hipStreamWaitEvent(s, c.ev_join, 0)) != hipSuccess)
+        return hip_fail(c, e, "far-field join");
+    ProfScope ps(&c, s, KC_SWEEP);
+    if ((e = launch_sym_combine(S, P, s)) != hipSuccess) return hip_fail(c, e, "symmetric combine launch");
     return 0;
 }
 
@@ -372,8 +401,9 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         if (e != hipSuccess) return hip_fail(c, e, "mass check launch");
         L.umass = c.umass;
     }
+    SymLaunch S;
     if (sym) {
-        if ((r = run_sym_sweep(c, L, P, s))) return r;
+        if ((r = sym_sweep_begin(c, L, P, s, S))) return r;
     } else if ((r = run_sweep(c, L, P, s))) {
         return r;
     }
@@ -399,12 +429,14 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         hipError_t e = launch_strain(N, P, s);
         if (e != hipSuccess) return hip_fail(c, e, "strain pass launch");
     }
+    if (accel_pass && sph && !strain_pass) {
+        ProfScope ps(&c, s, KC_OTHER);
+        hipError_t e = launch_prep(N, P, s);
+        if (e != hipSuccess) return hip_fail(c, e, "record pass launch");
+    }
+    // the acceleration pass adds the SPH terms to the gravity of the sweep
+    if (sym && (r = sym_sweep_end(c, S, P, s))) return r;
     if (accel_pass && sph) {
-        if (!strain_pass) {
-            ProfScope ps(&c, s, KC_OTHER);
-            hipError_t e = launch_prep(N, P, s);
-            if (e != hipSuccess) return hip_fail(c, e, "record pass launch");
-        }
         N.acc_has_gravity = grav;
         ProfScope ps(&c, s, KC_ACCEL);
         hipError_t e = launch_accel(N, P, s);
@@ -536,6 +568,9 @@ int grasph_destroy(grasph_ctx* ctx)
         hipEventDestroy(ev.b);
     }
     for (auto e : c.ev_free) hipEventDestroy(e);
+    if (c.ev_fork) hipEventDestroy(c.ev_fork);
+    if (c.ev_join) hipEventDestroy(c.ev_join);
+    if (c.side) hipStreamDestroy(c.side);
     delete ctx;
     return 0;
 }

@@ -18,7 +18,8 @@ enum KernelClass {
     KC_ACCEL = 3,
     KC_INTEGRATE = 4,
     KC_REORDER = 5,
-    KC_OTHER = 6
+    KC_OTHER = 6,
+    KC_FAR = 7  // far field of the symmetric sweep (its own stream)
 };
 
 // Per-device library state.  Scratch grows on demand and is reused.
@@ -107,6 +108,11 @@ struct Ctx {
     int64_t launches[GRASPH_NKERNEL_CLASSES] = {};
     bool capturing = false;  // no timing events while a graph is being captured
 
+    // the symmetric sweep's far field runs on a second stream (default
+    // priority, non-blocking), forked from and joined to the caller's
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     struct BatchState* batch = nullptr;
 };
 
@@ -182,7 +188,12 @@ struct SymLaunch {
     float4* apart;
     float* bpart;
 };
-hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t s);
+// the symmetric sweep in three launches: near field and far field (no
+// dependence between them: they may run on two streams), then the combine
+hipError_t launch_sym_near(const SymLaunch& L, const DevParams& P, hipStream_t s);
+hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s);
+hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_t s);
+bool sym_has_far(const SymLaunch& L);
 
 // The symmetric sweep over a batch (slot indices are the batch's).
 struct BatchInst;

@@ -247,12 +247,16 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
     near_blocks<GRAV, NBR, SOFT, false>(xs, 0, n, nb, nwords, bits, box, I, sl, ns, lane, cnt, part, P, R, 1.0f);
 }
 
-// Own slots of a lane in the far field: position, mass, plain and
-// compensated gravity sums.
+// Own slots of a lane in the far field: position, mass, plain gravity
+// sums, and where its compensated sums of the row are kept: in LDS (sums,
+// the wavefront's kFarSums floats, element (6 q + e) * 64 + lane: e = 0-2
+// sum x y z, 3-5 their corrections), touched once per unit, so they take no
+// registers from the pair loop.
+constexpr int kFarSums = kNI * 6 * 64;
 struct FarOwn {
     float x[kNI], y[kNI], z[kNI], m[kNI];
     float ax[kNI], ay[kNI], az[kNI];
-    CSum sx[kNI], sy[kNI], sz[kNI];
+    float* sums;
 };
 
 // ------------------------------------------------------------------ rotated far segment
@@ -401,7 +405,8 @@ __device__ __forceinline__ bool far_load_row(const float4* __restrict__ xs, cons
             o.x[q] = p.x; o.y[q] = p.y; o.z[q] = p.z; o.m[q] = p.w;
         }
         o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
-        o.sx[q] = o.sy[q] = o.sz[q] = CSum{0.0f, 0.0f};
+#pragma unroll
+        for (int e = 0; e < 6; ++e) o.sums[(6 * q + e) * 64 + lane] = 0.0f;
     }
     return sbA < nsb;
 }
@@ -413,8 +418,9 @@ __device__ __forceinline__ void far_flush(float4* __restrict__ apart, const size
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
         const size_t e = 2 * (f * kGroup + kSB * w + 64 * q + lane);
-        apart[e] = make_float4(o.sx[q].s, o.sy[q].s, o.sz[q].s, 0.0f);
-        apart[e + 1] = make_float4(o.sx[q].c, o.sy[q].c, o.sz[q].c, 0.0f);
+        const float* v = o.sums + 6 * 64 * q + lane;
+        apart[e] = make_float4(v[0], v[64], v[128], 0.0f);
+        apart[e + 1] = make_float4(v[192], v[256], v[320], 0.0f);
     }
 }
 
@@ -439,10 +445,14 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
     // own side: plain sums over the segment, then compensated
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
-        if (UM) {
-            o.sx[q].add(o.ax[q] * m); o.sy[q].add(o.ay[q] * m); o.sz[q].add(o.az[q] * m);
-        } else {
-            o.sx[q].add(o.ax[q]); o.sy[q].add(o.ay[q]); o.sz[q].add(o.az[q]);
+        const float a[3] = {o.ax[q], o.ay[q], o.az[q]};
+        float* v = o.sums + 6 * 64 * q + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            CSum t{v[64 * c], v[64 * (c + 3)]};
+            t.add(UM ? a[c] * m : a[c]);
+            v[64 * c] = t.s;
+            v[64 * (c + 3)] = t.c;
         }
         o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
     }
@@ -472,7 +482,8 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
 template <bool UM, bool SOFT>
 __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
                                          const int64_t U, const int W, float4* __restrict__ apart,
-                                         float* __restrict__ bpart, const float H2, const float m, float* stage)
+                                         float* __restrict__ bpart, const float H2, const float m, float* stage,
+                                         float* sums)
 {
     const int k = blockIdx.x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
@@ -482,6 +493,7 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
     const int stg = stage_index_rot(lane);
     float* const my_stage = stage + (size_t)w * kStageWave;
     FarOwn o;
+    o.sums = sums + (size_t)w * kFarSums;
     bool wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
     for (int64_t u = u0; u < u1; ++u) {
         if (s == row_units(g, nsb)) {
@@ -504,10 +516,11 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
                                                             const float H2, const float* __restrict__ umass)
 {
     __shared__ float stage[kGroupSB * kStageWave];
+    __shared__ float sums[kGroupSB * kFarSums];
     if (umass[0] != 0.0f)
-        far_work<true, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, umass[1], stage);
+        far_work<true, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, umass[1], stage, sums);
     else
-        far_work<false, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, 1.0f, stage);
+        far_work<false, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, 1.0f, stage, sums);
 }
 
 // Per slot, in a fixed order: the near partials (share order), the far own-side fragments
@@ -664,6 +677,7 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
     float* __restrict__ bpart, const float* __restrict__ umass)
 {
     __shared__ float stage[kGroupSB * kStageWave];
+    __shared__ float sums[kGroupSB * kFarSums];
     __shared__ int row_sh;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
     const int stg = stage_index_rot(lane);
@@ -682,6 +696,7 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
         const bool um = umass[2 * R.inst] != 0.0f;
         const float m = umass[2 * R.inst + 1];
         FarOwn o;
+        o.sums = sums + (size_t)w * kFarSums;
         const bool wave_on = far_load_row(xk, S.n, S.nsb, R.g, w, lane, o);
         for (int s = 0; s < R.units; ++s) {
             const int J = kGroupSB * (R.g + 1) + s;
@@ -782,7 +797,8 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
     return G;
 }
 
-hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t s)
+// near field (boxes, near bits, near kernel): in-group gravity partials and the lists
+hipError_t launch_sym_near(const SymLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
     if (L.nshares < 1) return hipErrorInvalidValue;
@@ -802,24 +818,34 @@ hipError_t launch_sym_sweep(const SymLaunch& L, const DevParams& P, hipStream_t 
     } else if (L.want_nbr) {
         if (soft) launch_near<false, true, true>(L, G, P, s); else launch_near<false, true, false>(L, G, P, s);
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (L.grav && G.units > 0) {
-        if (soft)
-            hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units,
-                               L.wgs, L.apart, L.bpart, P.H2, L.umass);
-        else
-            hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units,
-                               L.wgs, L.apart, L.bpart, P.H2, L.umass);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (L.grav || L.finish) {
-        hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part,
-                           L.apart, L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
-                           L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
-        e = hipGetLastError();
-    }
-    return e;
+    return hipGetLastError();
+}
+
+bool sym_has_far(const SymLaunch& L) { return L.n > 0 && L.grav && sym_geometry(L.n, L.wgs, false, 1).units > 0; }
+
+// far field: gravity between groups, once per pair (needs only xs and umass)
+hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (!sym_has_far(L)) return hipSuccess;
+    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
+    if (P.H2 > 0.0f)
+        hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
+                           L.apart, L.bpart, P.H2, L.umass);
+    else
+        hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
+                           L.apart, L.bpart, P.H2, L.umass);
+    return hipGetLastError();
+}
+
+// combine (after both fields): the acceleration
+hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0 || !(L.grav || L.finish)) return hipSuccess;
+    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
+    hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part, L.apart,
+                       L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
+                       L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+    return hipGetLastError();
 }
 
 SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_nbr, int64_t min_n)
