@@ -258,9 +258,10 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
     return 0;
 }
 
-int sym_sweep_end(Ctx& c, const SymLaunch& S, const DevParams& P, hipStream_t s)
+int sym_sweep_end(Ctx& c, SymLaunch& S, const DevParams& P, hipStream_t s, bool add_acc)
 {
     hipError_t e;
+    S.add_acc = add_acc;
     if (sym_has_far(S) && (e = hipStreamWaitEvent(s, c.ev_join, 0)) != hipSuccess)
         return hip_fail(c, e, "far-field join");
     ProfScope ps(&c, s, KC_SWEEP);
@@ -434,9 +435,19 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         hipError_t e = launch_prep(N, P, s);
         if (e != hipSuccess) return hip_fail(c, e, "record pass launch");
     }
-    // the acceleration pass adds the SPH terms to the gravity of the sweep
-    if (sym && (r = sym_sweep_end(c, S, P, s))) return r;
-    if (accel_pass && sph) {
+    // With the symmetric sweep, the acceleration pass runs while the far
+    // field is still going and writes the SPH terms alone; the combine then
+    // adds the gravity to them.  Otherwise it adds the SPH terms to the
+    // gravity of the sweep.
+    const bool accel_first = sym && accel_pass && sph;
+    if (accel_first) {
+        N.acc_has_gravity = false;
+        ProfScope ps(&c, s, KC_ACCEL);
+        hipError_t e = launch_accel(N, P, s);
+        if (e != hipSuccess) return hip_fail(c, e, "acceleration pass launch");
+    }
+    if (sym && (r = sym_sweep_end(c, S, P, s, accel_first && grav))) return r;
+    if (accel_pass && sph && !accel_first) {
         N.acc_has_gravity = grav;
         ProfScope ps(&c, s, KC_ACCEL);
         hipError_t e = launch_accel(N, P, s);

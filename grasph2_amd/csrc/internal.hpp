@@ -29,7 +29,7 @@ struct Ctx {
 
     // options
     int max_neighbours = 160;
-    int reorder_interval = 1;
+    int reorder_interval = 8;  // particles move little in 8 steps: the order only affects speed
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
     int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 49152 particles
@@ -172,6 +172,7 @@ struct SymLaunch {
     int n;
     bool grav, want_nbr;
     bool finish;        // as SweepLaunch::finish
+    bool add_acc;       // acc already holds the SPH terms: the combine adds the gravity to them
     int nshares;        // near field cut into this many shares per block (own list planes, merged after)
     float4* acc;
     int* nbr;           // a list per slot and share

@@ -531,8 +531,8 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
     const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
     const int n, const int nshares, const int64_t nsb, const int64_t U, const int W, const int grav,
     const int* __restrict__ perm,
-    const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const float4* __restrict__ vel,
-    float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams P)
+    const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const int add_acc,
+    const float4* __restrict__ vel, float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams P)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
@@ -562,8 +562,12 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
             x.add(-bpart[e]); y.add(-bpart[e + 1]); z.add(-bpart[e + 2]);
         }
     }
-    float ax = x.value(), ay = y.value(), az = z.value();
     const int i = perm ? perm[t] : t;
+    if (add_acc) {  // the SPH terms (and environment), summed before the far field was done
+        const float4 a = acc[i];
+        x.add(a.x); y.add(a.y); z.add(a.z);
+    }
+    float ax = x.value(), ay = y.value(), az = z.value();
     if (finish) {
         const float4 p = xs[t];
         const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -844,7 +848,7 @@ hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
     hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part, L.apart,
                        L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
-                       L.finish ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+                       L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
     return hipGetLastError();
 }
 

@@ -142,7 +142,9 @@ int grasph_create(int device, grasph_ctx** out);
 int grasph_destroy(grasph_ctx* ctx);
 
 /* Tuning knobs.  Keys: "max_neighbours" (default 160), "reorder_interval"
- * (Morton re-sort every k derivative evaluations, default 1; 0 = never),
+ * (Morton re-sort every k derivative evaluations, default 8; 0 = never;
+ * the order changes only the order of compensated sums, not which pairs
+ * are found),
  * "profile" (1 = record HIP events around each kernel), "sweep_i_per_lane"
  * (1 or 2), "sweep_slices" (j-split of the sweep, 0 = automatic),
  * "batch_waves_per_block", "batch_blocks_per_cu" (batched sweep launch

@@ -63,13 +63,17 @@ def assert_same(got_list, ref_list, fields):
 def unsplit(gpu_ctx):
     """The single-instance sweep without its j-split and the batch with its
     one-sided persistent sweep for every instance (which never splits an
-    instance's j range), so both sum in the same order; the launch-loop
-    contexts take the options from the parent context."""
+    instance's j range), so both sum in the same order; the Morton order
+    refreshed at every evaluation, so the paths compared refresh it at the
+    same times (it changes the order of the compensated sums); the
+    launch-loop contexts take the options from the parent context."""
     gpu_ctx.set_option("sweep_slices", 1)
     gpu_ctx.set_option("batch_symmetric", 0)
+    gpu_ctx.set_option("reorder_interval", 1)
     yield gpu_ctx
-    gpu_ctx.set_option("sweep_slices", 0)
+    gpu_ctx.set_option("sweep_slices", 0)  # the defaults
     gpu_ctx.set_option("batch_symmetric", 1)
+    gpu_ctx.set_option("reorder_interval", 8)
 
 
 def test_batched_derivatives_match_single_instances(unsplit):
@@ -214,6 +218,7 @@ def test_batched_variable_step_matches_single_instances(unsplit, reset):
         for k, (h, prm) in enumerate(zip(insts, params)):
             c1 = g.Context(0)
             c1.set_option("sweep_slices", 1)
+            c1.set_option("reorder_interval", 1)
             c1.set_option("timestep_reset", reset)
             d1 = g.DeviceParticles.from_host(h)
             c1.compute_derivatives(d1, prm)

@@ -26,9 +26,11 @@ def sym_batch_ctx(gpu_ctx):
     smaller ones (300, 64, 1) the one-sided persistent sweep, in one batch."""
     gpu_ctx.set_option("batch_symmetric", 1)
     gpu_ctx.set_option("batch_symmetric_min", 1000)
+    gpu_ctx.set_option("reorder_interval", 1)  # graph, eager and repeat runs refresh the order alike
     yield gpu_ctx
     gpu_ctx.set_option("batch_symmetric", 1)  # the defaults
     gpu_ctx.set_option("batch_symmetric_min", 0)
+    gpu_ctx.set_option("reorder_interval", 8)
 
 
 def run_batch(ctx, host, offsets, params):

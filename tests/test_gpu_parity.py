@@ -252,7 +252,7 @@ def test_order_and_list_capacity_do_not_change_results(gpu_ctx):
         gpu_ctx.set_option("max_neighbours", 8)  # every list overflows: all-j fallback
         small = gpu_run(gpu_ctx, h, prm)
     finally:
-        gpu_ctx.set_option("reorder_interval", 1)
+        gpu_ctx.set_option("reorder_interval", 8)  # the defaults
         gpu_ctx.set_option("max_neighbours", 160)
     for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
         assert np.array_equal(base[f], small[f]), f
