@@ -88,6 +88,9 @@ struct BatchState {
     // graph of one step
     hipStream_t gs = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // the symmetric sweep's far field: its own stream, forked and joined
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string graph_key;
@@ -290,7 +293,10 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
     if (!b.gs) {
         if (hipStreamCreateWithFlags(&b.gs, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b.ev_in, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b.ev_out, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&b.ev_out, hipEventDisableTiming) != hipSuccess ||
+            hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming) != hipSuccess)
             return hip_fail(c, hipGetLastError(), "batch stream");
     }
     return 0;
@@ -369,7 +375,7 @@ int batch_gather(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s
 }
 
 // the persistent sweep; with a slot order, batch_gather must have run for the current positions
-int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
+int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, bool mass_done = false)
 {
     const uint32_t f = b.flags;
     const DevParams P = make_dev_params(&b.params[0]);
@@ -397,8 +403,10 @@ int launch_sweep_only(Ctx& c, BatchState& b, const grasph_particles* p, hipStrea
     for (const auto& q : b.params) soft = soft && q.H > 0.0f;
     L.soft = soft;
     if (L.grav) {
-        e = launch_mass_check((const float4*)p->posm, b.d_insts, b.n_inst, 0, b.d_um, s);
-        if (e != hipSuccess) return hip_fail(c, e, "batch mass check launch");
+        if (!mass_done) {
+            e = launch_mass_check((const float4*)p->posm, b.d_insts, b.n_inst, 0, b.d_um, s);
+            if (e != hipSuccess) return hip_fail(c, e, "batch mass check launch");
+        }
         L.umass = b.d_um;
     }
     ProfScope ps(&c, s, KC_SWEEP);
@@ -442,14 +450,17 @@ int ensure_sym(Ctx& c, BatchState& b)
     return 0;
 }
 
-// the symmetric sweep over the batch's larger instances (after
-// launch_sweep_only, which has then run the mass check: mass_done)
-int launch_sym_sweep_batch(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, bool mass_done)
+// The symmetric sweep over the batch's larger instances, begun (after
+// launch_sweep_only, which has then run the mass check: mass_done): the far
+// field forked onto b.side (side == s: in order), the near field on s.
+// The far field reads only the slot-ordered positions and the mass
+// records, written on s before the fork.
+int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, bool mass_done, bool fork,
+                    SymBatchLaunch& L)
 {
     int r;
     if ((r = ensure_sym(c, b))) return r;
     const uint32_t f = b.flags;
-    SymBatchLaunch L;
     std::memset(&L, 0, sizeof L);
     L.perm = slot_order(c, b);
     L.xs = L.perm ? b.d_xs : (const float4*)p->posm;
@@ -492,10 +503,43 @@ int launch_sym_sweep_batch(Ctx& c, BatchState& b, const grasph_particles* p, hip
         }
         L.umass = b.d_um;
     }
+    if (sym_batch_has_far(L)) {
+        hipStream_t fs = s;
+        if (fork) {
+            if ((e = hipEventRecord(b.ev_fork, s)) != hipSuccess || (e = hipStreamWaitEvent(b.side, b.ev_fork, 0)))
+                return hip_fail(c, e, "batched far-field fork");
+            fs = b.side;
+        }
+        {
+            ProfScope ps(&c, fs, KC_FAR);
+            if ((e = launch_sym_batch_far(L, fs)) != hipSuccess) return hip_fail(c, e, "batched far-field launch");
+        }
+        if (fork && (e = hipEventRecord(b.ev_join, b.side)) != hipSuccess)
+            return hip_fail(c, e, "batched far-field join event");
+    }
     ProfScope ps(&c, s, KC_SWEEP);
-    e = launch_sym_batch(L, s);
-    if (e != hipSuccess) return hip_fail(c, e, "batched symmetric sweep launch");
+    if ((e = launch_sym_batch_near(L, s)) != hipSuccess) return hip_fail(c, e, "batched near-field launch");
     return 0;
+}
+
+int sym_batch_end(Ctx& c, BatchState& b, SymBatchLaunch& L, hipStream_t s, bool forked, bool add_acc)
+{
+    hipError_t e;
+    if (forked && sym_batch_has_far(L) && (e = hipStreamWaitEvent(s, b.ev_join, 0)) != hipSuccess)
+        return hip_fail(c, e, "batched far-field join");
+    L.add_acc = add_acc;
+    ProfScope ps(&c, s, KC_SWEEP);
+    if ((e = launch_sym_batch_combine(L, s)) != hipSuccess) return hip_fail(c, e, "batched symmetric combine launch");
+    return 0;
+}
+
+// the whole symmetric sweep in order on s (the autotuner times it)
+int launch_sym_sweep_batch(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, bool mass_done)
+{
+    SymBatchLaunch L;
+    int r;
+    if ((r = sym_batch_begin(c, b, p, s, mass_done, false, L))) return r;
+    return sym_batch_end(c, b, L, s, false, false);
 }
 
 // derivative passes of all instances (the order must be current)
@@ -508,12 +552,13 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     const DevParams P = make_dev_params(&b.params[0]);  // flags; values come from the per-instance table
     int r;
     if ((r = batch_gather(c, b, p, s))) return r;
-    if (sph || grav) {
-        // the one-sided persistent sweep takes the items (with the symmetric
-        // sweep on: the smaller instances), the symmetric sweep the rest
-        if (b.n_items > 0 && (r = launch_sweep_only(c, b, p, s))) return r;
-        if (c.batch_symmetric && (r = launch_sym_sweep_batch(c, b, p, s, b.n_items > 0))) return r;
-    }
+    // the one-sided persistent sweep takes the items (with the symmetric
+    // sweep on: the smaller instances), the symmetric sweep the rest; its
+    // far field is forked onto a stream of its own and overlaps the rest
+    const bool sym = (sph || grav) && c.batch_symmetric;
+    SymBatchLaunch SL;
+    if (sym && (r = sym_batch_begin(c, b, p, s, false, true, SL))) return r;
+    if ((sph || grav) && b.n_items > 0 && (r = launch_sweep_only(c, b, p, s, sym))) return r;
 
     const int* perm = slot_order(c, b);
     NbrLaunch N;
@@ -539,6 +584,7 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
     N.T = tables(b);
     hipError_t e;
     if (!sph) {
+        if (sym && (r = sym_batch_end(c, b, SL, s, true, false))) return r;
         ProfScope ps(&c, s, KC_OTHER);
         e = launch_finish(N, P, s);
         if (e != hipSuccess) return hip_fail(c, e, "batched finish launch");
@@ -558,6 +604,19 @@ int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, 
         e = launch_prep(N, P, s);
         if (e != hipSuccess) return hip_fail(c, e, "batched record launch");
     }
+    // With every instance in the symmetric sweep, the acceleration pass runs
+    // while the far field is still going and writes the SPH terms alone; the
+    // combine then adds the gravity to them.  Otherwise it adds the SPH
+    // terms to the gravity of the sweeps.
+    const bool accel_first = sym && b.n_items == 0;
+    if (accel_first) {
+        N.acc_has_gravity = false;
+        ProfScope ps(&c, s, KC_ACCEL);
+        e = launch_accel(N, P, s);
+        if (e != hipSuccess) return hip_fail(c, e, "batched acceleration launch");
+    }
+    if (sym && (r = sym_batch_end(c, b, SL, s, true, accel_first && grav))) return r;
+    if (accel_first) return 0;
     ProfScope ps(&c, s, KC_ACCEL);
     e = launch_accel(N, P, s);
     if (e != hipSuccess) return hip_fail(c, e, "batched acceleration launch");
@@ -675,6 +734,9 @@ void destroy_batch_state(BatchState* b)
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     if (b->gs) (void)hipStreamDestroy(b->gs);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     for (grasph_ctx* x : b->loop)
         if (x) grasph_destroy(x);
     delete b;

@@ -223,6 +223,7 @@ struct SymBatchLaunch {
     const int* perm;      // slot -> particle, or null
     int ntotal, n_inst;
     bool grav, want_nbr, finish, soft;
+    bool add_acc;         // acc already holds the SPH terms: the combine adds the gravity to them
     const SymInst* sinst;
     const BatchInst* insts;
     const DevParams* prm;
@@ -248,7 +249,11 @@ struct SymBatchLaunch {
     float4* apart;
     float* bpart;
 };
-hipError_t launch_sym_batch(const SymBatchLaunch& L, hipStream_t s);
+// the batched symmetric sweep in three launches, as the single-instance one
+hipError_t launch_sym_batch_near(const SymBatchLaunch& L, hipStream_t s);
+hipError_t launch_sym_batch_far(const SymBatchLaunch& L, hipStream_t s);
+hipError_t launch_sym_batch_combine(const SymBatchLaunch& L, hipStream_t s);
+inline bool sym_batch_has_far(const SymBatchLaunch& L) { return L.ntotal > 0 && L.n_blk > 0 && L.grav && L.nrows > 0; }
 
 // Fold the per-slice lists of a j-split sweep into one list per particle
 // (slice 0's arrays; a count above kmax means "scan the instance").

@@ -720,8 +720,9 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
 __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
     const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
     const SymInst* __restrict__ sinst, const int4* __restrict__ blk, const int grav, const int* __restrict__ perm,
-    const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const float4* __restrict__ vel,
-    float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams* __restrict__ prm)
+    const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const int add_acc,
+    const float4* __restrict__ vel, float4* __restrict__ xvel, float* __restrict__ maxvsig,
+    const DevParams* __restrict__ prm)
 {
     const int4 bl = blk[blockIdx.x];
     const int t = bl.y + threadIdx.x;
@@ -748,8 +749,12 @@ __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
             x.add(-bk[e]); y.add(-bk[e + 1]); z.add(-bk[e + 2]);
         }
     }
-    float ax = x.value(), ay = y.value(), az = z.value();
     const int i = perm ? perm[t] : t;
+    if (add_acc) {  // the SPH terms (and environment), summed before the far field was done
+        const float4 a = acc[i];
+        x.add(a.x); y.add(a.y); z.add(a.z);
+    }
+    float ax = x.value(), ay = y.value(), az = z.value();
     if (finish) {
         const float4 p = xs[t];
         const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -891,7 +896,7 @@ SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_n
     return T;
 }
 
-hipError_t launch_sym_batch(const SymBatchLaunch& L, hipStream_t s)
+hipError_t launch_sym_batch_near(const SymBatchLaunch& L, hipStream_t s)
 {
     if (L.ntotal <= 0 || L.n_blk <= 0) return hipSuccess;
     if (L.want_nbr) {
@@ -907,25 +912,30 @@ hipError_t launch_sym_batch(const SymBatchLaunch& L, hipStream_t s)
     } else if (L.want_nbr) {
         if (L.soft) launch_batch_near<false, true, true>(L, s); else launch_batch_near<false, true, false>(L, s);
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (L.grav && L.nrows > 0) {
-        if ((e = launch_reset_queue(L.queue, 1, s)) != hipSuccess) return e;
-        if (L.soft)
-            hipLaunchKernelGGL((sym_batch_far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst,
-                               L.insts, L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
-        else
-            hipLaunchKernelGGL((sym_batch_far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst,
-                               L.insts, L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (L.grav || L.finish) {
-        hipLaunchKernelGGL(sym_batch_combine_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.near_part, L.apart,
-                           L.bpart, L.sinst, L.blk, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.vel,
-                           L.xvel, L.maxvsig, L.prm);
-        e = hipGetLastError();
-    }
-    return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_batch_far(const SymBatchLaunch& L, hipStream_t s)
+{
+    if (!sym_batch_has_far(L)) return hipSuccess;
+    hipError_t e;
+    if ((e = launch_reset_queue(L.queue, 1, s)) != hipSuccess) return e;
+    if (L.soft)
+        hipLaunchKernelGGL((sym_batch_far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst, L.insts,
+                           L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
+    else
+        hipLaunchKernelGGL((sym_batch_far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst, L.insts,
+                           L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_batch_combine(const SymBatchLaunch& L, hipStream_t s)
+{
+    if (L.ntotal <= 0 || L.n_blk <= 0 || !(L.grav || L.finish)) return hipSuccess;
+    hipLaunchKernelGGL(sym_batch_combine_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.near_part, L.apart, L.bpart,
+                       L.sinst, L.blk, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.add_acc ? 1 : 0,
+                       L.vel, L.xvel, L.maxvsig, L.prm);
+    return hipGetLastError();
 }
 
 }  // namespace grasph
