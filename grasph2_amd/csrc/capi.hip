@@ -202,10 +202,11 @@ int sym_near_shares(const Ctx& c, int64_t n)
 
 // The symmetric sweep, begun: buffers, the far field forked onto the side
 // stream (launched first, so its workgroups are placed first), the near
-// field on s.  The far field needs only the slot-ordered positions and the
-// mass record, both written on s before the fork; the work enqueued on s
-// after this (list merge, density and strain passes) overlaps it.
-// sym_sweep_end joins the streams and combines.
+// field's lists on s.  The far field needs only the slot-ordered positions
+// and the mass record, both written on s before the fork; the work enqueued
+// on s after this (list merge, the SPH passes) overlaps it.  sym_sweep_end
+// enqueues the near field's in-group gravity, joins the streams and
+// combines.
 int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s, SymLaunch& S)
 {
     const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.nslices);
@@ -254,7 +255,7 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
         if ((e = hipEventRecord(c.ev_join, c.side)) != hipSuccess) return hip_fail(c, e, "far-field join event");
     }
     ProfScope ps(&c, s, KC_SWEEP);
-    if ((e = launch_sym_near(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field launch");
+    if ((e = launch_sym_lists(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field list launch");
     return 0;
 }
 
@@ -262,6 +263,11 @@ int sym_sweep_end(Ctx& c, SymLaunch& S, const DevParams& P, hipStream_t s, bool 
 {
     hipError_t e;
     S.add_acc = add_acc;
+    {
+        // the gravity inside the groups last on s: only the combine needs it
+        ProfScope ps(&c, s, KC_SWEEP);
+        if ((e = launch_sym_ingroup(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field gravity launch");
+    }
     if (sym_has_far(S) && (e = hipStreamWaitEvent(s, c.ev_join, 0)) != hipSuccess)
         return hip_fail(c, e, "far-field join");
     ProfScope ps(&c, s, KC_SWEEP);

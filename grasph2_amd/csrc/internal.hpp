@@ -189,9 +189,11 @@ struct SymLaunch {
     float4* apart;
     float* bpart;
 };
-// the symmetric sweep in three launches: near field and far field (no
-// dependence between them: they may run on two streams), then the combine
-hipError_t launch_sym_near(const SymLaunch& L, const DevParams& P, hipStream_t s);
+// the symmetric sweep in four launches: the near field's lists and its
+// in-group gravity, the far field (no dependence among them: they may run
+// on two streams), then the combine
+hipError_t launch_sym_lists(const SymLaunch& L, const DevParams& P, hipStream_t s);
+hipError_t launch_sym_ingroup(const SymLaunch& L, const DevParams& P, hipStream_t s);
 hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s);
 hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_t s);
 bool sym_has_far(const SymLaunch& L);

@@ -806,27 +806,30 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
     return G;
 }
 
-// near field (boxes, near bits, near kernel): in-group gravity partials and the lists
-hipError_t launch_sym_near(const SymLaunch& L, const DevParams& P, hipStream_t s)
+// near field, neighbour lists: boxes, near bits, the distance tests
+hipError_t launch_sym_lists(const SymLaunch& L, const DevParams& P, hipStream_t s)
 {
-    if (L.n <= 0) return hipSuccess;
+    if (L.n <= 0 || !L.want_nbr) return hipSuccess;
     if (L.nshares < 1) return hipErrorInvalidValue;
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
-    const bool soft = P.H2 > 0.0f;
-    if (L.want_nbr) {
-        hipLaunchKernelGGL(block_box_kernel, dim3((G.nb + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, L.xs,
-                           L.n, G.nb, L.box);
-        const int64_t words = (int64_t)G.nb * G.nwords;
-        hipLaunchKernelGGL(near_bits_kernel, dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                           L.box, G.nb, G.nwords, P.H2, L.bits);
-    }
-    if (L.grav && L.want_nbr) {
-        if (soft) launch_near<true, true, true>(L, G, P, s); else launch_near<true, true, false>(L, G, P, s);
-    } else if (L.grav) {
-        if (soft) launch_near<true, false, true>(L, G, P, s); else launch_near<true, false, false>(L, G, P, s);
-    } else if (L.want_nbr) {
-        if (soft) launch_near<false, true, true>(L, G, P, s); else launch_near<false, true, false>(L, G, P, s);
-    }
+    hipLaunchKernelGGL(block_box_kernel, dim3((G.nb + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, L.xs, L.n,
+                       G.nb, L.box);
+    const int64_t words = (int64_t)G.nb * G.nwords;
+    hipLaunchKernelGGL(near_bits_kernel, dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, L.box,
+                       G.nb, G.nwords, P.H2, L.bits);
+    if (P.H2 > 0.0f) launch_near<false, true, true>(L, G, P, s); else launch_near<false, true, false>(L, G, P, s);
+    return hipGetLastError();
+}
+
+// near field, gravity inside the groups (the partials of the combine)
+hipError_t launch_sym_ingroup(const SymLaunch& L, const DevParams& P, hipStream_t s)
+{
+    if (L.n <= 0 || !L.grav) return hipSuccess;
+    if (L.nshares < 1) return hipErrorInvalidValue;
+    SymLaunch M = L;
+    M.want_nbr = false;  // own group only, no bits
+    const SymGeom G = sym_geometry(M.n, M.wgs, false, M.nshares);
+    if (P.H2 > 0.0f) launch_near<true, false, true>(M, G, P, s); else launch_near<true, false, false>(M, G, P, s);
     return hipGetLastError();
 }
 
