@@ -452,7 +452,8 @@ int ensure_sym(Ctx& c, BatchState& b)
 
 // The symmetric sweep over the batch's larger instances, begun (after
 // launch_sweep_only, which has then run the mass check: mass_done): the far
-// field forked onto b.side (side == s: in order), the near field on s.
+// field forked onto b.side (fork; else in order on s), the near field's
+// lists on s; sym_batch_end adds its in-group gravity, joins and combines.
 // The far field reads only the slot-ordered positions and the mass
 // records, written on s before the fork.
 int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, bool mass_done, bool fork,
@@ -518,13 +519,19 @@ int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_
             return hip_fail(c, e, "batched far-field join event");
     }
     ProfScope ps(&c, s, KC_SWEEP);
-    if ((e = launch_sym_batch_near(L, s)) != hipSuccess) return hip_fail(c, e, "batched near-field launch");
+    if ((e = launch_sym_batch_lists(L, s)) != hipSuccess) return hip_fail(c, e, "batched near-field list launch");
     return 0;
 }
 
 int sym_batch_end(Ctx& c, BatchState& b, SymBatchLaunch& L, hipStream_t s, bool forked, bool add_acc)
 {
     hipError_t e;
+    {
+        // the gravity inside the groups last on s: only the combine needs it
+        ProfScope ps(&c, s, KC_SWEEP);
+        if ((e = launch_sym_batch_ingroup(L, s)) != hipSuccess)
+            return hip_fail(c, e, "batched near-field gravity launch");
+    }
     if (forked && sym_batch_has_far(L) && (e = hipStreamWaitEvent(s, b.ev_join, 0)) != hipSuccess)
         return hip_fail(c, e, "batched far-field join");
     L.add_acc = add_acc;

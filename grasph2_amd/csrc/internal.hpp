@@ -251,8 +251,9 @@ struct SymBatchLaunch {
     float4* apart;
     float* bpart;
 };
-// the batched symmetric sweep in three launches, as the single-instance one
-hipError_t launch_sym_batch_near(const SymBatchLaunch& L, hipStream_t s);
+// the batched symmetric sweep in four launches, as the single-instance one
+hipError_t launch_sym_batch_lists(const SymBatchLaunch& L, hipStream_t s);
+hipError_t launch_sym_batch_ingroup(const SymBatchLaunch& L, hipStream_t s);
 hipError_t launch_sym_batch_far(const SymBatchLaunch& L, hipStream_t s);
 hipError_t launch_sym_batch_combine(const SymBatchLaunch& L, hipStream_t s);
 inline bool sym_batch_has_far(const SymBatchLaunch& L) { return L.ntotal > 0 && L.n_blk > 0 && L.grav && L.nrows > 0; }

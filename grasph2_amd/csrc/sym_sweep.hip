@@ -899,22 +899,21 @@ SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_n
     return T;
 }
 
-hipError_t launch_sym_batch_near(const SymBatchLaunch& L, hipStream_t s)
+hipError_t launch_sym_batch_lists(const SymBatchLaunch& L, hipStream_t s)
 {
-    if (L.ntotal <= 0 || L.n_blk <= 0) return hipSuccess;
-    if (L.want_nbr) {
-        hipLaunchKernelGGL(sym_batch_box_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.xs, L.blk, L.sinst, L.box);
-        if (L.max_words > 0)
-            hipLaunchKernelGGL(sym_batch_bits_kernel, dim3((L.max_words + kBlock - 1) / kBlock, L.n_inst),
-                               dim3(kBlock), 0, s, L.box, L.sinst, L.insts, L.bits);
-    }
-    if (L.grav && L.want_nbr) {
-        if (L.soft) launch_batch_near<true, true, true>(L, s); else launch_batch_near<true, true, false>(L, s);
-    } else if (L.grav) {
-        if (L.soft) launch_batch_near<true, false, true>(L, s); else launch_batch_near<true, false, false>(L, s);
-    } else if (L.want_nbr) {
-        if (L.soft) launch_batch_near<false, true, true>(L, s); else launch_batch_near<false, true, false>(L, s);
-    }
+    if (L.ntotal <= 0 || L.n_blk <= 0 || !L.want_nbr) return hipSuccess;
+    hipLaunchKernelGGL(sym_batch_box_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.xs, L.blk, L.sinst, L.box);
+    if (L.max_words > 0)
+        hipLaunchKernelGGL(sym_batch_bits_kernel, dim3((L.max_words + kBlock - 1) / kBlock, L.n_inst), dim3(kBlock),
+                           0, s, L.box, L.sinst, L.insts, L.bits);
+    if (L.soft) launch_batch_near<false, true, true>(L, s); else launch_batch_near<false, true, false>(L, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_batch_ingroup(const SymBatchLaunch& L, hipStream_t s)
+{
+    if (L.ntotal <= 0 || L.n_blk <= 0 || !L.grav) return hipSuccess;
+    if (L.soft) launch_batch_near<true, false, true>(L, s); else launch_batch_near<true, false, false>(L, s);
     return hipGetLastError();
 }
 
