@@ -351,16 +351,27 @@ __device__ __forceinline__ float far_chunk_rot(const float4 pj, FarOwn& o, const
     return swap16_sum(xy, zz);
 }
 
-// partner b + (lane & 15) of an instance of n particles; BOUNDED: slots at
-// or past n are far away and massless, else b + 15 < n or the values are
-// unused (a look-ahead past the end of the segment, clamped to n - 1)
-template <bool BOUNDED>
+// partner b + (lane & 15) of an instance of n particles (clamped to n - 1:
+// a look-ahead past the end loads values nobody uses).  UM loads x y z
+// alone: a loaded register left unused would be reused at once, and that
+// write would wait for the load, a chunk too early.
+template <bool UM>
 __device__ __forceinline__ float4 load_partner_row(const float4* __restrict__ xs, const int b, const int n,
                                                    const int lane)
 {
-    const int j = b + (lane & (kRow - 1));
-    const float4 v = xs[min(j, n - 1)];
-    if (BOUNDED && j >= n) return make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
+    const int jc = min(b + (lane & (kRow - 1)), n - 1);
+    if (UM) {
+        const float3 p = *reinterpret_cast<const float3*>(xs + jc);
+        return make_float4(p.x, p.y, p.z, 0.0f);
+    }
+    return xs[jc];
+}
+// BOUNDED: partners at or past n are far away and massless (applied where
+// the chunk is used, so the load's wait stays there)
+template <bool BOUNDED>
+__device__ __forceinline__ float4 bound_partner_row(const float4 v, const int b, const int n, const int lane)
+{
+    if (BOUNDED && b + (lane & (kRow - 1)) >= n) return make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.0f);
     return v;
 }
 
@@ -380,10 +391,11 @@ template <bool UM, bool SOFT, bool BOUNDED>
 __device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, const int n, const int b0, FarOwn& o,
                                                 const float H2, const int lane, const int stg, float* my_stage)
 {
-    float4 A = load_partner_row<BOUNDED>(xs, b0, n, lane);
+    float4 A = load_partner_row<UM>(xs, b0, n, lane);
     for (int q = 0; q < kSB / kRow; ++q) {
-        const float4 B = load_partner_row<BOUNDED>(xs, b0 + kRow * (q + 1), n, lane);  // unused after the last
-        my_stage[3 * kRow * q + stg] = far_chunk_rot<UM, SOFT>(A, o, H2);
+        const float4 B = load_partner_row<UM>(xs, b0 + kRow * (q + 1), n, lane);  // unused after the last
+        my_stage[3 * kRow * q + stg] =
+            far_chunk_rot<UM, SOFT>(bound_partner_row<BOUNDED>(A, b0 + kRow * q, n, lane), o, H2);
         A = B;
     }
 }
