@@ -16,8 +16,8 @@ duration from HIP events recorded around every launch on the stream it
 runs on, over the timed steps.  From 49152 particles (the symmetric sweep)
 that kernel is the far field (far_kernel): the ordered pairs of particles in
 different groups of 1024 slots, N (N - 1) minus the in-group pairs, 99% of
-the pairs at N = 131072; it runs on a stream of its own, overlapping the
-near field, the list merge and the density and strain passes.  Below, it
+the pairs at N = 131072; the near field, the list merge and the SPH
+passes run beside it on a second stream.  Below, it
 is the one-sided sweep over all N (N - 1) pairs (its neighbour search rides
 on the same r^2 and adds no counted FLOPs).
 cpu_baseline: the CPU oracle in FP32 (the same op structure as the

@@ -200,14 +200,16 @@ int sym_near_shares(const Ctx& c, int64_t n)
     return (int)std::min<int64_t>(16, std::max<int64_t>(1, (target + blocks - 1) / blocks));
 }
 
-// The symmetric sweep, begun: buffers, the far field forked onto the side
-// stream (launched first, so its workgroups are placed first), the near
-// field's lists on s.  The far field needs only the slot-ordered positions
-// and the mass record, both written on s before the fork; the work enqueued
-// on s after this (list merge, the SPH passes) overlaps it.  sym_sweep_end
-// enqueues the near field's in-group gravity, joins the streams and
-// combines.
-int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s, SymLaunch& S)
+// The symmetric sweep, begun: buffers; the rest of the evaluation forked
+// onto the context's second stream (ws) after the gather and the mass
+// check; the far field on s, launched first so that its workgroups are
+// placed first and no event stands between it and the work before it; the
+// near field's lists on ws.  The far field needs only the slot-ordered
+// positions and the mass record; the work enqueued on ws after this (list
+// merge, the SPH passes) runs beside it.  sym_sweep_end enqueues the near
+// field's in-group gravity on ws, joins ws to s and combines on s.
+int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s, SymLaunch& S,
+                    hipStream_t& ws)
 {
     const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.nslices);
     int r;
@@ -240,36 +242,37 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
     S.apart = c.sym_apart;
     S.bpart = c.sym_bpart;
     hipError_t e;
+    ws = s;
     if (sym_has_far(S)) {
         if (!c.side && (e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking)) != hipSuccess)
-            return hip_fail(c, e, "far-field stream");
+            return hip_fail(c, e, "second stream");
         if ((!c.ev_fork && (e = hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming)) != hipSuccess) ||
             (!c.ev_join && (e = hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming)) != hipSuccess))
-            return hip_fail(c, e, "far-field events");
+            return hip_fail(c, e, "fork and join events");
         if ((e = hipEventRecord(c.ev_fork, s)) != hipSuccess || (e = hipStreamWaitEvent(c.side, c.ev_fork, 0)) != hipSuccess)
-            return hip_fail(c, e, "far-field fork");
+            return hip_fail(c, e, "fork");
         {
-            ProfScope ps(&c, c.side, KC_FAR);
-            if ((e = launch_sym_far(S, P, c.side)) != hipSuccess) return hip_fail(c, e, "far-field launch");
+            ProfScope ps(&c, s, KC_FAR);
+            if ((e = launch_sym_far(S, P, s)) != hipSuccess) return hip_fail(c, e, "far-field launch");
         }
-        if ((e = hipEventRecord(c.ev_join, c.side)) != hipSuccess) return hip_fail(c, e, "far-field join event");
+        ws = c.side;
     }
-    ProfScope ps(&c, s, KC_SWEEP);
-    if ((e = launch_sym_lists(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field list launch");
+    ProfScope ps(&c, ws, KC_SWEEP);
+    if ((e = launch_sym_lists(S, P, ws)) != hipSuccess) return hip_fail(c, e, "near-field list launch");
     return 0;
 }
 
-int sym_sweep_end(Ctx& c, SymLaunch& S, const DevParams& P, hipStream_t s, bool add_acc)
+int sym_sweep_end(Ctx& c, SymLaunch& S, const DevParams& P, hipStream_t s, hipStream_t ws, bool add_acc)
 {
     hipError_t e;
     S.add_acc = add_acc;
     {
-        // the gravity inside the groups last on s: only the combine needs it
-        ProfScope ps(&c, s, KC_SWEEP);
-        if ((e = launch_sym_ingroup(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field gravity launch");
+        // the gravity inside the groups last: only the combine needs it
+        ProfScope ps(&c, ws, KC_SWEEP);
+        if ((e = launch_sym_ingroup(S, P, ws)) != hipSuccess) return hip_fail(c, e, "near-field gravity launch");
     }
-    if (sym_has_far(S) && (e = hipStreamWaitEvent(s, c.ev_join, 0)) != hipSuccess)
-        return hip_fail(c, e, "far-field join");
+    if (ws != s && ((e = hipEventRecord(c.ev_join, ws)) != hipSuccess || (e = hipStreamWaitEvent(s, c.ev_join, 0))))
+        return hip_fail(c, e, "join");
     ProfScope ps(&c, s, KC_SWEEP);
     if ((e = launch_sym_combine(S, P, s)) != hipSuccess) return hip_fail(c, e, "symmetric combine launch");
     return 0;
@@ -409,15 +412,16 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         L.umass = c.umass;
     }
     SymLaunch S;
+    hipStream_t ws = s;  // the near field and the passes: the forked stream beside the far field, or s
     if (sym) {
-        if ((r = sym_sweep_begin(c, L, P, s, S))) return r;
+        if ((r = sym_sweep_begin(c, L, P, s, S, ws))) return r;
     } else if ((r = run_sweep(c, L, P, s))) {
         return r;
     }
     if (need_list && slices > 1) {
         // one list per particle for the passes (walking per-slice lists costs them up to 2x)
-        ProfScope ps(&c, s, KC_OTHER);
-        hipError_t e = launch_merge_lists(c.nbr, c.nbr_cnt, (int)n, c.max_neighbours, slices, s);
+        ProfScope ps(&c, ws, KC_OTHER);
+        hipError_t e = launch_merge_lists(c.nbr, c.nbr_cnt, (int)n, c.max_neighbours, slices, ws);
         if (e != hipSuccess) return hip_fail(c, e, "list merge launch");
         c.last_list_slices = 1;
     }
@@ -427,18 +431,18 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     N.xs = xs;
     N.rec = c.rec;
     if (dens_pass) {
-        ProfScope ps(&c, s, KC_DENSITY);
-        hipError_t e = launch_density(N, p->density, P, s);
+        ProfScope ps(&c, ws, KC_DENSITY);
+        hipError_t e = launch_density(N, p->density, P, ws);
         if (e != hipSuccess) return hip_fail(c, e, "density pass launch");
     }
     if (strain_pass) {
-        ProfScope ps(&c, s, KC_STRAIN);
-        hipError_t e = launch_strain(N, P, s);
+        ProfScope ps(&c, ws, KC_STRAIN);
+        hipError_t e = launch_strain(N, P, ws);
         if (e != hipSuccess) return hip_fail(c, e, "strain pass launch");
     }
     if (accel_pass && sph && !strain_pass) {
-        ProfScope ps(&c, s, KC_OTHER);
-        hipError_t e = launch_prep(N, P, s);
+        ProfScope ps(&c, ws, KC_OTHER);
+        hipError_t e = launch_prep(N, P, ws);
         if (e != hipSuccess) return hip_fail(c, e, "record pass launch");
     }
     // With the symmetric sweep, the acceleration pass runs while the far
@@ -448,11 +452,11 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     const bool accel_first = sym && accel_pass && sph;
     if (accel_first) {
         N.acc_has_gravity = false;
-        ProfScope ps(&c, s, KC_ACCEL);
-        hipError_t e = launch_accel(N, P, s);
+        ProfScope ps(&c, ws, KC_ACCEL);
+        hipError_t e = launch_accel(N, P, ws);
         if (e != hipSuccess) return hip_fail(c, e, "acceleration pass launch");
     }
-    if (sym && (r = sym_sweep_end(c, S, P, s, accel_first && grav))) return r;
+    if (sym && (r = sym_sweep_end(c, S, P, s, ws, accel_first && grav))) return r;
     if (accel_pass && sph && !accel_first) {
         N.acc_has_gravity = grav;
         ProfScope ps(&c, s, KC_ACCEL);

@@ -19,7 +19,7 @@ enum KernelClass {
     KC_INTEGRATE = 4,
     KC_REORDER = 5,
     KC_OTHER = 6,
-    KC_FAR = 7  // far field of the symmetric sweep (its own stream)
+    KC_FAR = 7  // far field of the symmetric sweep (the rest of the evaluation runs beside it)
 };
 
 // Per-device library state.  Scratch grows on demand and is reused.
@@ -108,8 +108,9 @@ struct Ctx {
     int64_t launches[GRASPH_NKERNEL_CLASSES] = {};
     bool capturing = false;  // no timing events while a graph is being captured
 
-    // the symmetric sweep's far field runs on a second stream (default
-    // priority, non-blocking), forked from and joined to the caller's
+    // beside the symmetric sweep's far field (on the caller's stream), the
+    // rest of the evaluation runs on a second stream (default priority,
+    // non-blocking), forked from and joined to the caller's
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 

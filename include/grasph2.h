@@ -279,11 +279,11 @@ int grasph_reduce_invariants(grasph_ctx* ctx, const grasph_particles* p,
 /* Per-kernel timing recorded while option "profile" is 1: accumulated
  * milliseconds and launch counts (recorded intervals) per kernel class, in
  * the order sweep, density, strain, accel, integrate, reorder, other, far.
- * "far" is the far-field kernel of the symmetric sweep, which runs on a
- * stream of its own, overlapping the near field, the list merge and the
- * density and strain passes; "sweep" holds the rest of the all-pairs sweep
- * (with the symmetric sweep two intervals per evaluation: near field, then
- * combine).  Synchronises the recorded events.  Resets the counters when
+ * "far" is the far-field kernel of the symmetric sweep, which runs on the
+ * caller's stream while the near field, the list merge and the SPH passes
+ * run beside it on the context's second stream; "sweep" holds the rest of
+ * the all-pairs sweep (with the symmetric sweep three intervals per
+ * evaluation: near-field lists, in-group gravity, combine).  Synchronises the recorded events.  Resets the counters when
  * reset != 0. */
 #define GRASPH_NKERNEL_CLASSES 8
 int grasph_get_kernel_times(grasph_ctx* ctx, double ms[GRASPH_NKERNEL_CLASSES],
