@@ -569,8 +569,23 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
             }
         }
         const int64_t base = bpart_group_base(g) + local;
-        for (int64_t gp = 0; gp < g; ++gp) {
-            const size_t e = 3 * (size_t)(base + gp * kGroup);
+        const int ng = (int)g;
+        // eight groups' loads in flight before their (ordered) adds
+        int gp = 0;
+        for (; gp + 8 <= ng; gp += 8) {
+            float v[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const size_t e = 3 * (size_t)(base + (int64_t)(gp + u) * kGroup);
+                v[u][0] = bpart[e]; v[u][1] = bpart[e + 1]; v[u][2] = bpart[e + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                x.add(-v[u][0]); y.add(-v[u][1]); z.add(-v[u][2]);
+            }
+        }
+        for (; gp < ng; ++gp) {
+            const size_t e = 3 * (size_t)(base + (int64_t)gp * kGroup);
             x.add(-bpart[e]); y.add(-bpart[e + 1]); z.add(-bpart[e + 2]);
         }
     }
@@ -756,7 +771,20 @@ __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
         }
         const float* bk = bpart + S.bpart0;
         const int64_t base = bpart_group_base(g) + lg;
-        for (int gp = 0; gp < g; ++gp) {
+        int gp = 0;  // eight groups' loads in flight before their (ordered) adds
+        for (; gp + 8 <= g; gp += 8) {
+            float v[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const size_t e = 3 * (size_t)(base + (int64_t)(gp + u) * kGroup);
+                v[u][0] = bk[e]; v[u][1] = bk[e + 1]; v[u][2] = bk[e + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                x.add(-v[u][0]); y.add(-v[u][1]); z.add(-v[u][2]);
+            }
+        }
+        for (; gp < g; ++gp) {
             const size_t e = 3 * (size_t)(base + (int64_t)gp * kGroup);
             x.add(-bk[e]); y.add(-bk[e + 1]); z.add(-bk[e + 2]);
         }
