@@ -13,7 +13,8 @@ step evaluates softened gravity on all ordered pairs i != j.
 roofline: the dominant kernel, FP32 vector ALU bound, at 20 algorithmic
 FLOPs per ordered gravity pair (FMA = 2, rsqrt = 1), divided by its mean
 duration from HIP events recorded around every launch on the stream it
-runs on, over the timed steps.  From 49152 particles (the symmetric sweep)
+runs on, over a second pass of as many steps (the timed pass records no
+events: they lengthen a step by ~2%).  From 49152 particles (the symmetric sweep)
 that kernel is the far field (far_kernel): the ordered pairs of particles in
 different groups of 1024 slots, N (N - 1) minus the in-group pairs, 99% of
 the pairs at N = 131072; the near field, the list merge and the SPH
@@ -152,14 +153,19 @@ def main():
     for _ in range(args.warmup):
         step()
     g.hip.synchronize()
-    ctx.set_option("profile", 1)
-    ctx.kernel_times(reset=True)
-    g.hip.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     g.hip.synchronize()
     t1 = time.perf_counter()
+    # per-kernel times from HIP events in a second timed pass of as many
+    # steps (recording events around every launch lengthens a step, so
+    # ms_per_step comes from the pass above, which records none)
+    ctx.set_option("profile", 1)
+    ctx.kernel_times(reset=True)
+    for _ in range(args.steps):
+        step()
+    g.hip.synchronize()
     kt = ctx.kernel_times(reset=True)
     ctx.set_option("profile", 0)
     nb_pairs, nb_max, nb_over = ctx.neighbour_stats() if sph else (0, 0, 0)
