@@ -121,6 +121,41 @@ class Stream:
             pass
 
 
+class Graph:
+    """Work enqueued on a stream by fn(stream), captured (hipStreamBeginCapture,
+    global mode) into an executable HIP graph; launch() replays it."""
+
+    def __init__(self, stream, fn):
+        h = _lib()
+        h.hipStreamBeginCapture.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        h.hipStreamEndCapture.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        h.hipGraphInstantiate.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_size_t]
+        h.hipGraphLaunch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h.hipGraphExecDestroy.argtypes = [ctypes.c_void_p]
+        h.hipGraphDestroy.argtypes = [ctypes.c_void_p]
+        self.graph, self.exec = ctypes.c_void_p(), ctypes.c_void_p()
+        check(h.hipStreamBeginCapture(stream.handle, 0), "hipStreamBeginCapture")
+        try:
+            fn(stream)
+        finally:
+            code = h.hipStreamEndCapture(stream.handle, ctypes.byref(self.graph))
+        check(code, "hipStreamEndCapture")
+        check(h.hipGraphInstantiate(ctypes.byref(self.exec), self.graph, None, None, 0), "hipGraphInstantiate")
+
+    def launch(self, stream):
+        check(_lib().hipGraphLaunch(self.exec, stream.handle), "hipGraphLaunch")
+
+    def __del__(self):
+        try:
+            if self.exec.value:
+                _lib().hipGraphExecDestroy(self.exec)
+            if self.graph.value:
+                _lib().hipGraphDestroy(self.graph)
+        except Exception:
+            pass
+
+
 class Event:
     def __init__(self):
         self.handle = ctypes.c_void_p()

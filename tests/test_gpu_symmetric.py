@@ -116,3 +116,47 @@ def test_deterministic(sym_ctx):
     b = gpu_run(sym_ctx, h, prm)
     for f in ("acc", "density", "dstress_dt", "maxvsig"):
         assert np.array_equal(a[f], b[f]), f
+
+
+def test_step_captured_in_graph():
+    """computeDerivatives + integrate of the symmetric sweep (the far field on
+    the caller's stream, the rest forked onto the context's second stream)
+    captured into a HIP graph on the caller's stream: every replay, with
+    eager work and synchronisation between replays, equals the same step
+    run eagerly, bit for bit.  The Morton order is computed by the warm-up
+    step and kept (a long re-sort interval), so both paths sum in the same
+    order and the captured step holds no sort."""
+    print("symmetric step in a graph", flush=True)
+    h, prm = solid_case(20000, seed=77)
+    ctx = g.Context(0)
+    try:
+        ctx.set_option("sweep_symmetric", 1)
+        ctx.set_option("reorder_interval", 1000)
+        s = g.hip.Stream()
+        de, dg = g.DeviceParticles.from_host(h), g.DeviceParticles.from_host(h)
+
+        def step(d):
+            return lambda st: (ctx.compute_derivatives(d, prm, st), ctx.integrate(d, prm, True, st))
+
+        step(dg)(s)  # warm-up: order, scratch, second stream and events exist before the capture
+        s.synchronize()
+        graph = g.hip.Graph(s, step(dg))
+        ctx2 = g.Context(0)  # the eager run, with the same options and its own order
+        ctx2.set_option("sweep_symmetric", 1)
+        ctx2.set_option("reorder_interval", 1000)
+        ctx2.compute_derivatives(de, prm, s)  # the warm-up step
+        ctx2.integrate(de, prm, True, s)
+        for rep in range(3):
+            graph.launch(s)
+            ctx2.compute_derivatives(de, prm, s)
+            ctx2.integrate(de, prm, True, s)
+            s.synchronize()
+            a, e = dg.download(), de.download()
+            for f in ("posm", "pos64", "vel", "acc", "density", "dstress", "dstress_dt", "balsara", "maxvsig"):
+                assert np.array_equal(a[f], e[f]), "replay %d field %s" % (rep, f)
+        del graph
+        de.free()
+        dg.free()
+        ctx2.close()
+    finally:
+        ctx.close()
