@@ -698,19 +698,19 @@ int time_sweep(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s, 
     const bool prof = c.profile;
     c.profile = false;
     for (int k = 0; k < reps + 1 && !r; ++k) {
-        hipEventRecord(e0, s);
+        (void)hipEventRecord(e0, s);
         if (!all || b.n_items > 0) r = launch_sweep_only(c, b, p, s);
         if (!r && all && c.batch_symmetric) r = launch_sym_sweep_batch(c, b, p, s, b.n_items > 0);
-        hipEventRecord(e1, s);
+        (void)hipEventRecord(e1, s);
         if (!r && hipEventSynchronize(e1) == hipSuccess) {
             float t = 0;
-            hipEventElapsedTime(&t, e0, e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
             if (k > 0) best = std::min(best, (double)t);  // the first run warms up
         }
     }
     c.profile = prof;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     *ms = best;
     return r;
 }

@@ -26,7 +26,7 @@ ProfScope::ProfScope(Ctx* ctx, hipStream_t st, int k) : c(ctx), s(st), cls(k)
         }
         (q == 0 ? a : b) = e;
     }
-    if (a) hipEventRecord(a, s);
+    if (a) (void)hipEventRecord(a, s);
 }
 
 ProfScope::~ProfScope()
@@ -36,7 +36,7 @@ ProfScope::~ProfScope()
         if (b) c->ev_free.push_back(b);
         return;
     }
-    hipEventRecord(b, s);
+    (void)hipEventRecord(b, s);
     c->ev_pending.push_back(Ctx::Ev{a, b, cls});
 }
 
@@ -49,7 +49,7 @@ int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_ev
     if (c.reorder_cap < n || !c.perm) {
         void* bufs[] = {c.perm, c.keys, c.keys_alt, c.vals};
         for (void* b : bufs)
-            if (b) hipFree(b);
+            if (b) (void)hipFree(b);
         c.perm = nullptr; c.keys = nullptr; c.keys_alt = nullptr; c.vals = nullptr;
         c.reorder_cap = 0;
         c.perm_n = 0;
@@ -71,7 +71,7 @@ int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_ev
     if (stale) {
         const size_t need = reorder_tmp_bytes((int)n);
         if (need > c.sort_tmp_bytes || !c.sort_tmp) {
-            if (c.sort_tmp) hipFree(c.sort_tmp);
+            if (c.sort_tmp) (void)hipFree(c.sort_tmp);
             c.sort_tmp = nullptr;
             c.sort_tmp_bytes = 0;
             if (hipMalloc(&c.sort_tmp, need ? need : 1) != hipSuccess) {
@@ -575,8 +575,8 @@ int grasph_destroy(grasph_ctx* ctx)
 {
     if (!ctx) return GRASPH_E_INVALID;
     Ctx& c = ctx->c;
-    hipSetDevice(c.device);
-    hipDeviceSynchronize();
+    (void)hipSetDevice(c.device);
+    (void)hipDeviceSynchronize();
     grasph::destroy_batch_state(c.batch);
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
@@ -585,13 +585,13 @@ int grasph_destroy(grasph_ctx* ctx)
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {
-        hipEventDestroy(ev.a);
-        hipEventDestroy(ev.b);
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
     }
-    for (auto e : c.ev_free) hipEventDestroy(e);
-    if (c.ev_fork) hipEventDestroy(c.ev_fork);
-    if (c.ev_join) hipEventDestroy(c.ev_join);
-    if (c.side) hipStreamDestroy(c.side);
+    for (auto e : c.ev_free) (void)hipEventDestroy(e);
+    if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+    if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+    if (c.side) (void)hipStreamDestroy(c.side);
     delete ctx;
     return 0;
 }

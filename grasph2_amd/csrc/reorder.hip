@@ -185,8 +185,8 @@ hipError_t launch_gather(const int* perm, int n, const float4* posm, const float
 size_t batch_order_tmp_bytes(int ntotal, int key_bits)
 {
     size_t bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr,
-                                       (int*)nullptr, ntotal, 0, key_bits, (hipStream_t)0);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr,
+                                             (int*)nullptr, ntotal, 0, key_bits, (hipStream_t)0);
     return bytes;
 }
 
@@ -206,8 +206,8 @@ hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s)
 size_t reorder_tmp_bytes(int n)
 {
     size_t bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
-                                       (int*)nullptr, n, 0, 30, (hipStream_t)0);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
+                                             (int*)nullptr, n, 0, 30, (hipStream_t)0);
     return bytes;
 }
 
