@@ -380,15 +380,20 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     // slot-ordered positions for the sweep and the passes' slot records
     const int* perm = need_list ? order_of(c) : nullptr;
     const float4* xs = (const float4*)p->posm;
+    const int gather_blocks = (int)((n + 255) / 256);
+    bool mass_flags = false;  // the gather makes the equal-mass check's per-block flags
     if (need_list) {
         const bool reads_vel = strain_pass || (accel_pass && sph);
         const bool gdens = reads_vel && !dens_pass;  // else the density pass fills the records' densities
+        mass_flags = sweep_grav;
         if ((perm && (r = grow(c, c.xs, c.xs_cap, n, "slot-ordered positions"))) ||
-            (r = grow(c, c.rec, c.rec_cap, n, "slot records")))
+            (r = grow(c, c.rec, c.rec_cap, n, "slot records")) ||
+            (mass_flags && (r = grow(c, c.mass_flags, c.mass_flags_cap, gather_blocks, "mass-check flags"))))
             return r;
         ProfScope ps(&c, s, KC_REORDER);
         hipError_t e = launch_gather(perm, (int)n, (const float4*)p->posm, reads_vel ? (const float4*)p->vel : nullptr,
-                                     gdens ? p->density : nullptr, perm ? c.xs : nullptr, c.rec, s);
+                                     gdens ? p->density : nullptr, perm ? c.xs : nullptr, c.rec, s,
+                                     mass_flags ? c.mass_flags : nullptr);
         if (e != hipSuccess) return hip_fail(c, e, "slot-order gather launch");
         if (perm) xs = c.xs;
     }
@@ -407,7 +412,8 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     if (sweep_grav) {
         // equal masses let the sweep factor the mass out of the gravity sum
         ProfScope ps(&c, s, KC_OTHER);
-        hipError_t e = launch_mass_check(L.posm, nullptr, 1, L.n, c.umass, s);
+        hipError_t e = mass_flags ? launch_mass_final(L.posm, c.mass_flags, gather_blocks, c.umass, s)
+                                  : launch_mass_check(L.posm, nullptr, 1, L.n, c.umass, s);
         if (e != hipSuccess) return hip_fail(c, e, "mass check launch");
         L.umass = c.umass;
     }
@@ -581,7 +587,7 @@ int grasph_destroy(grasph_ctx* ctx)
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
                     c.nbr,  c.nbr_cnt, c.rec,     c.pot,  c.red,      c.part, c.ts, c.umass,
-                    c.xs,   c.sym_box, c.sym_bits, c.sym_near, c.sym_apart, c.sym_bpart};
+                    c.xs,   c.sym_box, c.sym_bits, c.sym_near, c.sym_apart, c.sym_bpart, c.mass_flags};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {

@@ -50,6 +50,8 @@ struct Ctx {
     // {uniform, mass} of the particles, written before each gravity sweep,
     // then the per-block flags of the check (2 + 256 floats)
     float* umass = nullptr;
+    int* mass_flags = nullptr;  // per-256-slot flags of the check when the gather makes them
+    int64_t mass_flags_cap = 0;
 
     // Morton order of the i side (slot t -> particle perm[t])
     int* perm = nullptr;
@@ -446,8 +448,11 @@ hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s);
 // Slot-ordered copies for the sweep and the list passes (perm null: the
 // identity): xs[t] = posm[perm[t]] when xs is given, and the slot records'
 // position/mass and velocity/density (vel or density null: zeros there).
+// slot-ordered copies; with mass_flags (one int per 256 slots) also the
+// per-block half of the equal-mass check, which launch_mass_final completes
 hipError_t launch_gather(const int* perm, int n, const float4* posm, const float4* vel, const float* density,
-                         float4* xs, SlotRec* rec, hipStream_t s);
+                         float4* xs, SlotRec* rec, hipStream_t s, int* mass_flags = nullptr);
+hipError_t launch_mass_final(const float4* posm, const int* mass_flags, int nflags, float* umass, hipStream_t s);
 
 // environmentAcceleration and the zero XVEL / MAXVSIG stores of
 // calcAcceleration for configurations without SPH (batched form).

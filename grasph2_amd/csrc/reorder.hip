@@ -152,33 +152,42 @@ __global__ __launch_bounds__(kBlock) void batch_keys(const float4* __restrict__ 
 }
 
 // slot-ordered copies: xs[t] = posm[perm[t]] (when xs is given) and the
-// slot records' {x y z m} and {vx vy vz rho}
+// slot records' {x y z m} and {vx vy vz rho}; with mass_flags, also the
+// first half of the equal-mass check: flag b is set when a mass of block b
+// differs from particle 0's, bit for bit
 __global__ __launch_bounds__(kBlock) void gather_kernel(const int* __restrict__ perm, const int n,
                                                         const float4* __restrict__ posm, const float4* __restrict__ vel,
                                                         const float* __restrict__ density, float4* __restrict__ xs,
-                                                        SlotRec* __restrict__ rec)
+                                                        SlotRec* __restrict__ rec, int* __restrict__ mass_flags)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    const int i = perm ? perm[t] : t;
-    const float4 p = posm[i];
-    if (xs) xs[t] = p;
-    if (rec) {
-        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        rec[t].pm = p;
-        rec[t].vr = make_float4(v.x, v.y, v.z, density ? density[i] : 0.0f);
+    int bad = 0;
+    if (t < n) {
+        const int i = perm ? perm[t] : t;
+        const float4 p = posm[i];
+        if (xs) xs[t] = p;
+        if (rec) {
+            const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            rec[t].pm = p;
+            rec[t].vr = make_float4(v.x, v.y, v.z, density ? density[i] : 0.0f);
+        }
+        if (mass_flags) bad = __float_as_int(p.w) != __float_as_int(posm[0].w);
+    }
+    if (mass_flags) {  // uniform: the whole workgroup reaches the barrier
+        bad = __syncthreads_or(bad);
+        if (threadIdx.x == 0) mass_flags[blockIdx.x] = bad;
     }
 }
 
 }  // namespace
 
 hipError_t launch_gather(const int* perm, int n, const float4* posm, const float4* vel, const float* density,
-                         float4* xs, SlotRec* rec, hipStream_t s)
+                         float4* xs, SlotRec* rec, hipStream_t s, int* mass_flags)
 {
     if (n <= 0) return hipSuccess;
     if (!posm || (xs && !perm)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(gather_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, perm, n, posm, vel,
-                       density, xs, rec);
+                       density, xs, rec, mass_flags);
     return hipGetLastError();
 }
 

@@ -311,6 +311,13 @@ hipError_t launch_mass_check(const float4* posm, const BatchInst* insts, int n_i
     return hipGetLastError();
 }
 
+hipError_t launch_mass_final(const float4* posm, const int* mass_flags, int nflags, float* umass, hipStream_t s)
+{
+    if (nflags <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mass_final_kernel, dim3(1), dim3(kBlock), 0, s, posm, mass_flags, nflags, umass);
+    return hipGetLastError();
+}
+
 hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s)
 {
     if (n <= 0 || nslices <= 1) return hipSuccess;
