@@ -160,3 +160,27 @@ def test_step_captured_in_graph():
         ctx2.close()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("which", ["spread", "last"])
+@pytest.mark.parametrize("symmetric", [0, 1])
+def test_sph_gravity_unequal_masses(symmetric, which):
+    """With SPH on, the equal-mass check comes from the slot-order gather's
+    per-block flags: masses spread by +-20%, or equal but for the last
+    particle (in the last, partial block), must take the unequal-mass path
+    of the one-sided and of the symmetric sweep (FP64 oracle)."""
+    print("unequal masses with SPH symmetric=%d %s" % (symmetric, which), flush=True)
+    h, prm = solid_case(n=5000 + 37, seed=31)
+    rng = np.random.default_rng(5)
+    if which == "spread":
+        h["posm"][:, 3] *= rng.uniform(0.8, 1.2, size=h["posm"].shape[0]).astype(np.float32)
+    else:
+        h["posm"][-1, 3] *= np.float32(1.5)
+    ctx = g.Context(0)
+    try:
+        ctx.set_option("sweep_symmetric", symmetric)
+        out = gpu_run(ctx, h, prm)
+    finally:
+        ctx.close()
+    ref = O.compute_derivatives(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
