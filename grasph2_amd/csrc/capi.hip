@@ -256,6 +256,12 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
             if ((e = launch_sym_far(S, P, s)) != hipSuccess) return hip_fail(c, e, "far-field launch");
         }
         ws = c.side;
+        // Below ~100k particles the far field's window is short next to the
+        // lists' work: there the lists take issue priority, so they and the
+        // passes after them end inside it (liquid N = 65536: 1.12 -> 1.05 ms
+        // per step); above, their VALU work would lengthen the far field by
+        // more than it saves (collision N = 131072: 3.79 -> 3.86 ms).
+        S.list_prio = c.sym_list_priority == 1 || (c.sym_list_priority == 2 && L.n < 98304);
     }
     ProfScope ps(&c, ws, KC_SWEEP);
     if ((e = launch_sym_lists(S, P, ws)) != hipSuccess) return hip_fail(c, e, "near-field list launch");
@@ -637,6 +643,9 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
         if (value < 0 || value > (int64_t)1 << 30)
             return grasph::fail(c, GRASPH_E_INVALID, "batch_symmetric_min must be in [0, 2^30]");
         c.batch_symmetric_min = (int)value;
+    } else if (k == "sym_list_priority") {
+        if (value < 0 || value > 2) return grasph::fail(c, GRASPH_E_INVALID, "sym_list_priority must be 0, 1 or 2");
+        c.sym_list_priority = (int)value;
     } else if (k == "sym_near_shares") {
         if (value < 0 || value > 16) return grasph::fail(c, GRASPH_E_INVALID, "sym_near_shares must be in [0, 16]");
         c.sym_near_shares = (int)value;
@@ -670,6 +679,7 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value)
     else if (k == "batch_symmetric") *value = c.batch_symmetric;
     else if (k == "batch_symmetric_min") *value = c.batch_symmetric_min;
     else if (k == "sym_near_shares") *value = c.sym_near_shares;
+    else if (k == "sym_list_priority") *value = c.sym_list_priority;
     else if (k == "batch_waves_per_block") *value = c.batch_wpb;
     else if (k == "batch_blocks_per_cu") *value = c.batch_bpc;
     else if (k == "profile") *value = c.profile ? 1 : 0;

@@ -35,6 +35,7 @@ struct Ctx {
     int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 49152 particles
     int sym_workgroups = 0;   // workgroups of its far kernel (0: 2 wavefronts per SIMD)
     int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
+    int sym_list_priority = 2;  // its near-field lists ahead of the far field: 0 no, 1 yes, 2 below 98304 particles
     int batch_symmetric = 1;  // batches: the symmetric sweep for instances of batch_symmetric_min particles or more
     int batch_symmetric_min = 0;
     int cus = 256;         // compute units of the device
@@ -176,6 +177,7 @@ struct SymLaunch {
     bool grav, want_nbr;
     bool finish;        // as SweepLaunch::finish
     bool add_acc;       // acc already holds the SPH terms: the combine adds the gravity to them
+    bool list_prio;     // the lists' wavefronts take issue priority over the far field's beside them
     int nshares;        // near field cut into this many shares per block (own list planes, merged after)
     float4* acc;
     int* nbr;           // a list per slot and share

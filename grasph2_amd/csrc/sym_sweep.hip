@@ -228,8 +228,12 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
                                                       const float4* __restrict__ box, int* __restrict__ nbr,
                                                       int* __restrict__ nbr_cnt,
                                                       const int kmax, float4* __restrict__ near_part,
-                                                      const DevParams P, const float* __restrict__ umass)
+                                                      const DevParams P, const float* __restrict__ umass,
+                                                      const int prio)
 {
+    // issue priority over the far field's wavefronts, which run beside this
+    // kernel on the other stream (see the list_prio option of SymLaunch)
+    if (prio) __builtin_amdgcn_s_setprio(3);
     const int I = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
     const int lane = threadIdx.x & 63;
     if (I >= nb) return;
@@ -819,7 +823,7 @@ void launch_near(const SymLaunch& L, const SymGeom& G, const DevParams& P, hipSt
     hipLaunchKernelGGL((near_kernel<GRAV, NBR, SOFT>), dim3(blocks, L.nshares), dim3(kBlock), 0, s, L.xs, L.n, G.nb,
                        G.nwords,
                        L.want_nbr ? L.bits : nullptr, L.want_nbr ? L.box : nullptr, L.nbr, L.nbr_cnt, L.kmax,
-                       L.near_part, P, L.umass);
+                       L.near_part, P, L.umass, L.list_prio ? 1 : 0);
 }
 
 }  // namespace
@@ -868,6 +872,7 @@ hipError_t launch_sym_ingroup(const SymLaunch& L, const DevParams& P, hipStream_
     if (L.nshares < 1) return hipErrorInvalidValue;
     SymLaunch M = L;
     M.want_nbr = false;  // own group only, no bits
+    M.list_prio = false;
     const SymGeom G = sym_geometry(M.n, M.wgs, false, M.nshares);
     if (P.H2 > 0.0f) launch_near<true, false, true>(M, G, P, s); else launch_near<true, false, false>(M, G, P, s);
     return hipGetLastError();
