@@ -14,7 +14,8 @@ roofline: the dominant kernel, FP32 vector ALU bound, at 20 algorithmic
 FLOPs per ordered gravity pair (FMA = 2, rsqrt = 1), divided by its mean
 duration from HIP events recorded around every launch on the stream it
 runs on, over a second pass of as many steps (the timed pass records no
-events: they lengthen a step by ~2%).  From 49152 particles (the symmetric sweep)
+events: they lengthen a step by ~2%).  From 16384 particles with SPH, 32768
+for gravity alone (the symmetric sweep)
 that kernel is the far field (far_kernel): the ordered pairs of particles in
 different groups of 1024 slots, N (N - 1) minus the in-group pairs, 99% of
 the pairs at N = 131072; the near field, the list merge and the SPH
@@ -39,7 +40,8 @@ PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (AMD published)
 # The library's default for the single-instance sweep (option "sweep_symmetric" = 2, include/grasph2.h):
 # from this many particles on, gravity between groups of 1024 is evaluated once per pair for both sides.
 # The roofline stays algorithmic either way: 20 FLOP per ordered pair.
-SYM_AUTO_MIN = 49152
+SYM_AUTO_MIN_LISTS = 16384    # with neighbour lists (SPH on)
+SYM_AUTO_MIN_GRAVITY = 32768  # gravity alone
 SYM_GROUP = 1024  # slots per group of the symmetric sweep; pairs inside a group are the near field's
 FLOP_GRAVITY_PAIR = 20
 FLOP_DENSITY_NBR = 9
@@ -174,7 +176,7 @@ def main():
     pairs = float(n) * (n - 1)
     value = pairs / (ms_step * 1e-3)
 
-    sym = n >= SYM_AUTO_MIN
+    sym = n >= (SYM_AUTO_MIN_LISTS if sph else SYM_AUTO_MIN_GRAVITY)
     # all kernels of the all-pairs sweep, summed per evaluation (with the symmetric sweep, intervals that overlap)
     sweep_ms = (kt["sweep"][0] + kt["far"][0]) / args.steps
     sweep_flops = FLOP_GRAVITY_PAIR * pairs

@@ -181,7 +181,14 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
 constexpr int64_t kSymMaxPartialBytes = int64_t(12) << 30;
 // From this size on the symmetric sweep is the faster one (measured: 9%
 // slower at 16k particles, 3% faster at 65k, 10-16% at 131k).
-constexpr int64_t kSymAutoMin = 49152;
+// The symmetric sweep's default thresholds (option sweep_symmetric = 2),
+// measured on the collision and Plummer workloads: with neighbour lists the
+// lists and the SPH passes run beside its far field, and it wins from 16384
+// particles (32768: 0.41 against 0.59 ms per step); gravity alone has
+// nothing to overlap, and its coarse far-field units leave the GPU part
+// idle below 32768 (24576: 0.29 against 0.23 ms).
+constexpr int64_t kSymAutoMinLists = 16384;
+constexpr int64_t kSymAutoMinGravity = 32768;
 
 int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : sym_default_workgroups(c.cus); }
 
@@ -368,7 +375,8 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     // the order only matters for the neighbour branch: gravity alone runs in index order
     if (need_list && (r = ensure_order(c, p, s, true))) return r;
     const bool sweep_grav = accel_pass && grav;
-    const bool sym_on = c.sweep_symmetric == 1 || (c.sweep_symmetric == 2 && n >= kSymAutoMin);
+    const bool sym_on = c.sweep_symmetric == 1 ||
+                        (c.sweep_symmetric == 2 && n >= (need_list ? kSymAutoMinLists : kSymAutoMinGravity));
     const bool sym = sym_on && (sweep_grav || need_list) && sym_fits(c, n, need_list);
     int slices = 1, slice_len = (int)n;
     if ((sweep_grav || need_list) && !sym) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);

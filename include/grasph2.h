@@ -153,7 +153,8 @@ int grasph_destroy(grasph_ctx* ctx);
  * reference's running minimum, the default), "sweep_symmetric" (single
  * instance: 1 = gravity between groups of 1024 particles evaluated once per
  * pair for both sides, 0 = the one-sided sweep, 2 = the symmetric sweep from
- * 49152 particles on, the default; results agree to FP32 rounding, the
+ * 16384 particles on when neighbour lists are built, from 32768 for gravity
+ * alone, the default; results agree to FP32 rounding, the
  * neighbour lists are identical), "sym_workgroups" (workgroups of the
  * symmetric sweep's far-field kernel, 0 = two wavefronts per SIMD),
  * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
