@@ -24,7 +24,9 @@ is the one-sided sweep over all N (N - 1) pairs (its neighbour search rides
 on the same r^2 and adds no counted FLOPs).
 cpu_baseline: the CPU oracle in FP32 (the same op structure as the
 reference's host path) on a bounded sample of target particles i against
-all j, all three passes, OpenMP threads as reported.
+all j, all three passes; one OpenMP thread per CPU of the process's
+affinity set (capped by OMP_NUM_THREADS when set), with nproc and the CPU
+model recorded beside it.
 """
 import argparse
 import json
@@ -57,7 +59,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="collision", choices=["collision", "plummer", "liquid", "large"])
+    ap.add_argument("--workload", default="collision", choices=["collision", "contact", "plummer", "liquid", "large"])
+    ap.add_argument("--advance", type=int, default=0,
+                    help="untimed steps before the warm-up (e.g. to carry the collision into contact)")
     ap.add_argument("--n", type=int, default=0, help="override the particle count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
@@ -152,6 +156,11 @@ def main():
         ctx.compute_derivatives(d, prm)
         ctx.integrate(d, prm, True)
 
+    for k in range(args.advance):
+        step()
+        if (k + 1) % 200 == 0:
+            g.hip.synchronize()
+            print("advanced %d of %d steps" % (k + 1, args.advance), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
     g.hip.synchronize()
@@ -171,6 +180,7 @@ def main():
     kt = ctx.kernel_times(reset=True)
     ctx.set_option("profile", 0)
     nb_pairs, nb_max, nb_over = ctx.neighbour_stats() if sph else (0, 0, 0)
+    plastic = plastic_fraction(ctx, d, prm) if solid else None
 
     ms_step = (t1 - t0) * 1e3 / args.steps
     pairs = float(n) * (n - 1)
@@ -221,7 +231,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded initial conditions of the workload)",
         "config": {"workload": args.workload, "n": n, "flags": int(prm.flags), "H": prm.H,
-                   "dt": prm.fixed_timestep, "fp64_drift": True},
+                   "dt": prm.fixed_timestep, "fp64_drift": True, "advance_steps": args.advance},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": dom_name, "kernel_ms": dom_ms, "flops_per_launch": dom_flops,
@@ -234,6 +244,7 @@ def main():
         "neighbour_pairs": nb_pairs,
         "max_neighbours_seen": nb_max,
         "list_overflows": nb_over,
+        "plastic_fraction": plastic,
         "integrator": {"ms": integ_ms, "bytes_per_particle": INTEGRATOR_BYTES_SOLID if solid else INTEGRATOR_BYTES_NBODY,
                        "gbps": integ_bytes / (integ_ms * 1e-3) / 1e9 if integ_ms > 0 else None,
                        "hbm_frac": integ_bytes / (integ_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS if integ_ms > 0 else None},
@@ -248,13 +259,32 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def plastic_fraction(ctx, d, prm):
+    """Share of particles whose stress the next drift carries past yield
+    (von Mises: 3 J2(S + dS/dt dt) > Y^2, so the plasticity step scales S),
+    after one more (untimed) derivative evaluation."""
+    import numpy as np
+
+    import grasph2_amd as g
+
+    if not (prm.flags & g.PLASTICITY_MIESE):
+        return None
+    ctx.compute_derivatives(d, prm)
+    g.hip.synchronize()
+    st = d.download(["dstress", "dstress_dt"])
+    S = st["dstress"].astype(np.float64) + st["dstress_dt"].astype(np.float64) * float(prm.fixed_timestep)
+    J2 = 0.5 * (S[0] ** 2 + S[1] ** 2 + S[2] ** 2) + S[3] ** 2 + S[4] ** 2 + S[5] ** 2
+    return float(np.mean(3.0 * J2 > float(prm.Y) ** 2))
+
+
 def cpu_baseline(d, prm, n, target_s):
     """Time the FP32 oracle's three passes for a sample of i against all j."""
     import numpy as np
 
     from oracle import pyoracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    threads = host["threads_used"]
     O.set_threads(threads)
     state = d.download()
     rng = np.random.default_rng(0)
@@ -285,9 +315,38 @@ def cpu_baseline(d, prm, n, target_s):
             passes += 1
         count = n
         what = "%d pass(es) over all %d target particles" % (passes, n)
-    return {"value": passes * count * (n - 1) / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
-            "sample": "FP32 oracle, calcDensity + calcBalsaraDensityDTDStressDT + calcAcceleration for %s "
-                      "against all %d partners, %.1f s" % (what, n, t)}
+    out = {"value": passes * count * (n - 1) / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+           "sample": "FP32 oracle, calcDensity + calcBalsaraDensityDTDStressDT + calcAcceleration for %s "
+                     "against all %d partners, %.1f s, %d OpenMP threads" % (what, n, t, threads)}
+    out.update(host)
+    return out
+
+
+def host_cpu():
+    """The host the CPU baseline runs on: logical CPUs (nproc), the CPUs this
+    process may run on, the model name, and the OpenMP threads used: every
+    CPU of the affinity set, capped by OMP_NUM_THREADS when the environment
+    sets it (a shared GPU box gives each GPU's jobs that many)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = affinity
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"nproc": nproc, "affinity_cpus": affinity, "cpu_model": model, "omp_num_threads_env": omp or None,
+            "threads_used": threads}
 
 
 if __name__ == "__main__":

@@ -53,7 +53,7 @@ def build_library(verbose=True):
     return LIB
 
 
-CPP_EXAMPLES = ("api_roundtrip", "snapshot_roundtrip", "ic_generate")
+CPP_EXAMPLES = ("api_roundtrip", "snapshot_roundtrip", "ic_generate", "ic_device_roundtrip")
 
 
 def build_cpp_example(verbose=True):

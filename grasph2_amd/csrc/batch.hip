@@ -549,8 +549,22 @@ int launch_sym_sweep_batch(Ctx& c, BatchState& b, const grasph_particles* p, hip
     return sym_batch_end(c, b, L, s, false, false);
 }
 
-// derivative passes of all instances (the order must be current)
+int batch_passes(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s);
+
+// derivative passes of all instances (the order must be current), then, with
+// the variable step, every instance's next dt := max_timestep as
+// calcAcceleration::do_after does (computeDerivatives.h:305-308)
 int batch_derivative_launches(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
+{
+    int r = batch_passes(c, b, p, s);
+    if (r || !(b.flags & GRASPH_VARIABLE_TIMESTEP)) return r;
+    ProfScope ps(&c, s, KC_OTHER);
+    hipError_t e = launch_vts_set_next(b.d_ts, b.n_inst, b.d_prm, make_dev_params(&b.params[0]), 1, s);
+    if (e != hipSuccess) return hip_fail(c, e, "batched time-step reset launch");
+    return 0;
+}
+
+int batch_passes(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_t s)
 {
     const uint32_t f = b.flags;
     const bool sph = f & GRASPH_SPH, grav = f & GRASPH_SELF_GRAVITY;
@@ -650,7 +664,7 @@ int batch_integrate_launch(Ctx& c, BatchState& b, const grasph_particles* p, int
     L.n_instances = b.n_inst;
     L.ts = (b.flags & GRASPH_VARIABLE_TIMESTEP) ? b.d_ts : nullptr;
     ProfScope ps(&c, s, KC_INTEGRATE);
-    hipError_t e = launch_integrate_step(L, make_dev_params(&b.params[0]), c.timestep_reset, s);
+    hipError_t e = launch_integrate_step(L, make_dev_params(&b.params[0]), s);
     if (e != hipSuccess) return hip_fail(c, e, "batched leapfrog launch");
     return 0;
 }
@@ -681,7 +695,7 @@ std::string graph_key(const Ctx& c, const BatchState& b, const grasph_particles*
       << p->density_dt << ' ' << p->balsara << ' ' << p->maxvsig << ' ' << p->dstress << ' ' << p->dstress_dt << ' '
       << (c.reorder_interval > 0) << ' ' << c.batch_wpb << ' ' << c.batch_bpc << ' ' << (const void*)b.d_nbr << ' '
       << (const void*)b.d_rec << ' ' << (const void*)b.d_items << ' ' << (const void*)b.d_xs << ' '
-      << c.timestep_reset << ' ' << c.batch_symmetric << ' ' << c.batch_symmetric_min << ' '
+      << c.batch_symmetric << ' ' << c.batch_symmetric_min << ' '
       << (const void*)b.d_sinst << ' ' << (const void*)b.d_sbpart << ' ' << (const void*)b.d_sblk;
     return k.str();
 }
@@ -859,7 +873,6 @@ int grasph_batch_step_loop(grasph_ctx* ctx, const grasph_particles* p, const gra
         }
         s.sweep_ni = c.sweep_ni;
         s.sweep_slices = c.sweep_slices;
-        s.timestep_reset = c.timestep_reset;
         s.profile = false;
     }
     for (int step = 0; step < n_steps; ++step)

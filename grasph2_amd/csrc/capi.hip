@@ -40,6 +40,27 @@ ProfScope::~ProfScope()
     c->ev_pending.push_back(Ctx::Ev{a, b, cls});
 }
 
+// j-split of the single-instance sweep: enough slices for ~16 wavefronts
+// per SIMD (measured: the sweep keeps gaining up to there, as more resident
+// waves hide the scalar-load and rare-path latency), at most 32 slices of
+// at least 512 j each; slice length a multiple of 8.
+void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len)
+{
+    int S = c.sweep_slices;
+    if (S <= 0) {
+        const int64_t waves = (n + 64 * ni - 1) / (64 * ni);
+        const int64_t target = 16 * 4 * (int64_t)c.cus;
+        S = (int)std::min<int64_t>(32, std::max<int64_t>(1, (target + waves - 1) / waves));
+        S = (int)std::max<int64_t>(1, std::min<int64_t>(S, n / 512));
+    }
+    int64_t len = (n + S - 1) / S;
+    len = (len + 7) / 8 * 8;
+    if (len < 8) len = 8;
+    *slice_len = (int)len;
+    *slices = (int)((n + len - 1) / len);
+    if (*slices < 1) *slices = 1;
+}
+
 namespace {
 
 int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_eval)
@@ -110,27 +131,6 @@ int ensure_nbr(Ctx& c, int64_t n, int slices)
     c.nbr_cap_n = need_n;
     c.nbr_cap_k = k;
     return 0;
-}
-
-// j-split of the single-instance sweep: enough slices for ~16 wavefronts
-// per SIMD (measured: the sweep keeps gaining up to there, as more resident
-// waves hide the scalar-load and rare-path latency), at most 32 slices of
-// at least 512 j each; slice length a multiple of 8.
-void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len)
-{
-    int S = c.sweep_slices;
-    if (S <= 0) {
-        const int64_t waves = (n + 64 * ni - 1) / (64 * ni);
-        const int64_t target = 16 * 4 * (int64_t)c.cus;
-        S = (int)std::min<int64_t>(32, std::max<int64_t>(1, (target + waves - 1) / waves));
-        S = (int)std::max<int64_t>(1, std::min<int64_t>(S, n / 512));
-    }
-    int64_t len = (n + S - 1) / S;
-    len = (len + 7) / 8 * 8;
-    if (len < 8) len = 8;
-    *slice_len = (int)len;
-    *slices = (int)((n + len - 1) / len);
-    if (*slices < 1) *slices = 1;
 }
 
 SweepLaunch sweep_base(Ctx& c, const grasph_particles* p)
@@ -327,7 +327,6 @@ int check_derivative_arrays(Ctx& c, const grasph_particles* p, uint32_t f, int w
         if ((f & GRASPH_BALSARA_SWITCH) && which == 3) { GRASPH_NEED(p->balsara, "balsara"); }
         if (f & GRASPH_XSPH) { GRASPH_NEED(p->xvel, "xvel"); }
     }
-    if (accel_pass && (f & GRASPH_CLOHESSY_WILTSHIRE)) GRASPH_NEED(p->vel, "vel");
     return 0;
 }
 
@@ -475,6 +474,14 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
         ProfScope ps(&c, ws, KC_ACCEL);
         hipError_t e = launch_accel(N, P, ws);
         if (e != hipSuccess) return hip_fail(c, e, "acceleration pass launch");
+    }
+    if (accel_pass && (f & GRASPH_VARIABLE_TIMESTEP)) {
+        // calcAcceleration::do_after sets nextTS = max_timestep
+        // (computeDerivatives.h:305-308): the next integrate(pb, true) takes
+        // min(max_timestep, this step's criteria), so dt can grow again.
+        // Enqueued beside the far field when there is one (ws joins s below).
+        hipError_t e = launch_vts_set_next(c.ts, 1, nullptr, P, 1, ws);
+        if (e != hipSuccess) return hip_fail(c, e, "time-step reset launch");
     }
     if (sym && (r = sym_sweep_end(c, S, P, s, ws, accel_first && grav))) return r;
     if (accel_pass && sph && !accel_first) {
@@ -632,9 +639,6 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sweep_i_per_lane") {
         if (value != 1 && value != 2) return grasph::fail(c, GRASPH_E_INVALID, "sweep_i_per_lane must be 1 or 2");
         c.sweep_ni = (int)value;
-    } else if (k == "timestep_reset") {
-        if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "timestep_reset must be 0 or 1");
-        c.timestep_reset = value == 1;
     } else if (k == "sweep_slices") {
         if (value < 0 || value > 64) return grasph::fail(c, GRASPH_E_INVALID, "sweep_slices must be in [0, 64]");
         c.sweep_slices = (int)value;
@@ -680,7 +684,6 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value)
     if (k == "max_neighbours") *value = c.max_neighbours;
     else if (k == "reorder_interval") *value = c.reorder_interval;
     else if (k == "sweep_i_per_lane") *value = c.sweep_ni;
-    else if (k == "timestep_reset") *value = c.timestep_reset ? 1 : 0;
     else if (k == "sweep_slices") *value = c.sweep_slices;
     else if (k == "sweep_symmetric") *value = c.sweep_symmetric;
     else if (k == "sym_workgroups") *value = c.sym_workgroups;
@@ -740,7 +743,7 @@ int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_pa
     L.ts = (prm->flags & GRASPH_VARIABLE_TIMESTEP) ? c.ts : nullptr;
     hipStream_t s = (hipStream_t)stream;
     grasph::ProfScope ps(&c, s, grasph::KC_INTEGRATE);
-    hipError_t e = grasph::launch_integrate_step(L, grasph::make_dev_params(prm), c.timestep_reset, s);
+    hipError_t e = grasph::launch_integrate_step(L, grasph::make_dev_params(prm), s);
     if (e != hipSuccess) return grasph::hip_fail(c, e, "leapfrog launch");
     return 0;
 }

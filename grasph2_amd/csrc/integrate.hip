@@ -184,8 +184,9 @@ __global__ __launch_bounds__(kBlock) void vts_kick_kernel(float4* __restrict__ v
     }
 }
 
-// next dt := initial_timestep (integration.h:233-234) or max_timestep
-// (the "timestep_reset" option), one thread per instance
+// next dt := initial_timestep (integrate(pb, false), integration.h:233-234)
+// or max_timestep (the reset at the end of every acceleration pass,
+// computeDerivatives.h:305-308), one thread per instance
 __global__ void vts_set_next_kernel(float* __restrict__ ts, const int ninst, const DevParams* __restrict__ prm,
                                     const DevParams P0, const int which)
 {
@@ -312,24 +313,24 @@ hipError_t launch_vts_kick(const IntegLaunch& L, const DevParams& P, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_vts_set_next(const IntegLaunch& L, const DevParams& P, int which, hipStream_t s)
+hipError_t launch_vts_set_next(float* ts, int n_instances, const DevParams* prm, const DevParams& P, int which,
+                               hipStream_t s)
 {
-    if (!L.ts || L.n_instances <= 0) return hipErrorInvalidValue;
-    const DevParams* prm = L.T.nblocks > 0 ? L.T.prm : nullptr;
-    hipLaunchKernelGGL(vts_set_next_kernel, dim3((L.n_instances + 255) / 256), dim3(256), 0, s, L.ts, L.n_instances,
-                       prm, P, which);
+    if (!ts || n_instances <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(vts_set_next_kernel, dim3((n_instances + 255) / 256), dim3(256), 0, s, ts, n_instances, prm, P,
+                       which);
     return hipGetLastError();
 }
 
 // integrate(pb, notFirstStep), src/integration.h:219-244
-hipError_t launch_integrate_step(const IntegLaunch& L, const DevParams& P, bool timestep_reset, hipStream_t s)
+hipError_t launch_integrate_step(const IntegLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (!L.ts) return launch_integrate(L, P, s);
     hipError_t e;
     if (L.not_first_step) {
-        if (timestep_reset && (e = launch_vts_set_next(L, P, 1, s)) != hipSuccess) return e;
         if ((e = launch_vts_kick(L, P, s)) != hipSuccess) return e;
-    } else if ((e = launch_vts_set_next(L, P, 0, s)) != hipSuccess) {
+    } else if ((e = launch_vts_set_next(L.ts, L.n_instances, L.T.nblocks > 0 ? L.T.prm : nullptr, P, 0, s)) !=
+               hipSuccess) {
         return e;
     }
     return launch_integrate(L, P, s);

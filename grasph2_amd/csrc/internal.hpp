@@ -44,7 +44,6 @@ struct Ctx {
     int batch_ni[4] = {1, 1, 1, 1};
     int batch_wpb = 4;
     int batch_bpc = 8;
-    bool timestep_reset = false;  // variable step: restart the dt minimum at max_timestep each step
 
     // variable-step leapfrog state {current dt, next dt}, zero at creation
     float* ts = nullptr;
@@ -422,10 +421,12 @@ hipError_t launch_integrate(const IntegLaunch& L, const DevParams& P, hipStream_
 // First kernel of the variable step: half kick with the current dt and the
 // dt criteria folded into the next dt (atomic minimum per block).
 hipError_t launch_vts_kick(const IntegLaunch& L, const DevParams& P, hipStream_t s);
-// next dt of every instance := initial_timestep (which 0) or max_timestep (which 1)
-hipError_t launch_vts_set_next(const IntegLaunch& L, const DevParams& P, int which, hipStream_t s);
-// Both stages of one integrate(pb, notFirstStep) call; timestep_reset as the option.
-hipError_t launch_integrate_step(const IntegLaunch& L, const DevParams& P, bool timestep_reset, hipStream_t s);
+// next dt of every instance := initial_timestep (which 0) or max_timestep
+// (which 1); per-instance parameters from prm (null: P for one instance)
+hipError_t launch_vts_set_next(float* ts, int n_instances, const DevParams* prm, const DevParams& P, int which,
+                               hipStream_t s);
+// Both stages of one integrate(pb, notFirstStep) call.
+hipError_t launch_integrate_step(const IntegLaunch& L, const DevParams& P, hipStream_t s);
 
 struct InvLaunch {
     const float4* posm;
@@ -500,6 +501,10 @@ inline int check_params(Ctx& c, const grasph_params* prm)
     if (prm->dimension < 1 || prm->dimension > 3) return fail(c, GRASPH_E_INVALID, "dimension must be 1, 2 or 3");
     if (!(prm->H >= 0.0f) || !(prm->H < 3.0e38f)) return fail(c, GRASPH_E_INVALID, "H must be finite and >= 0");
     if ((prm->flags & GRASPH_SPH) && !(prm->H > 0.0f)) return fail(c, GRASPH_E_INVALID, "SPH needs H > 0");
+    if (prm->flags & GRASPH_CLOHESSY_WILTSHIRE)
+        return fail(c, GRASPH_E_UNSUPPORTED,
+                    "CLOHESSY_WILTSHIRE is not supported: src/computeDerivatives.h:37-39 adds a scalar to a "
+                    "float3 through an mpUtils operator whose meaning no available source fixes");
     if ((prm->flags & GRASPH_SOLIDS) && (prm->flags & GRASPH_ARTIFICIAL_STRESS))
         return fail(c, GRASPH_E_UNSUPPORTED,
                     "SOLIDS with ARTIFICIAL_STRESS is not supported (the reference does not compile it, "
@@ -524,6 +529,9 @@ inline int check_particles(Ctx& c, const grasph_particles* p)
     if (p->n > 0 && !p->posm) return fail(c, GRASPH_E_INVALID, "posm is NULL");
     return 0;
 }
+
+// j-split of the single-instance one-sided sweep for n particles at ni per lane
+void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len);
 
 // arrays each pass needs, by configuration; 0 or a GRASPH_E_INVALID code
 int check_derivative_arrays(Ctx& c, const grasph_particles* p, uint32_t f, int which);

@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     });
 
     float fx = ax.value(), fy = ay.value(), fz = az.value();
-    environment_acceleration(P, pi.x, pi.y, pi.z, vi.x, vi.y, vi.z, fx, fy, fz);
+    environment_acceleration(P, fx, fy, fz);
     acc[i] = make_float4(fx, fy, fz, 0.0f);
     if (maxvsig) maxvsig[i] = mv;
     if (xvel) xvel[i] = make_float4(xvx, xvy, xvz, 0.0f);
@@ -360,11 +360,9 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const float4* __restrict
     if (i >= m.slot_end) return;
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const bool grav = P.flags & GRASPH_SELF_GRAVITY;
-    if (!grav || (P.flags & (GRASPH_CLOHESSY_WILTSHIRE | GRASPH_CONSTANT_ACCELERATION))) {
-        const float4 p = posm[i];
-        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!grav || (P.flags & GRASPH_CONSTANT_ACCELERATION)) {
         float4 a = grav ? acc[i] : make_float4(0.f, 0.f, 0.f, 0.f);  // no sweep ran without gravity
-        environment_acceleration(P, p.x, p.y, p.z, v.x, v.y, v.z, a.x, a.y, a.z);
+        environment_acceleration(P, a.x, a.y, a.z);
         acc[i] = make_float4(a.x, a.y, a.z, 0.0f);
     }
     if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -19,7 +19,7 @@ struct DevParams {
     int32_t dim;
     float H, H2, Wpref, dWpref;
     float rho0, BULK, dBULKdP, c, c2, shear, Y, tanfr, cohesion, alpha;
-    float mateps, matexp, normalsep, Wnorm, xsph, cw_n, dt;
+    float mateps, matexp, normalsep, Wnorm, xsph, dt;
     float cax, cay, caz;
     float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, refl;
     float cand;  // d2 = r2 + H2 threshold that admits every pair with r2 <= H2
@@ -168,7 +168,7 @@ __host__ inline DevParams make_dev_params(const grasph_params* p)
     d.shear = p->shear; d.Y = p->Y; d.tanfr = p->tanfr; d.cohesion = p->cohesion; d.alpha = p->alpha;
     d.mateps = p->mateps; d.matexp = p->matexp; d.normalsep = p->normalsep;
     d.Wnorm = w_spline(p->normalsep, d.H, d.Wpref);
-    d.xsph = p->xsph_factor; d.cw_n = p->cw_n; d.dt = p->fixed_timestep;
+    d.xsph = p->xsph_factor; d.dt = p->fixed_timestep;
     d.cax = p->constant_acceleration[0]; d.cay = p->constant_acceleration[1]; d.caz = p->constant_acceleration[2];
     d.bminx = p->box_min[0]; d.bminy = p->box_min[1]; d.bminz = p->box_min[2];
     d.bmaxx = p->box_max[0]; d.bmaxy = p->box_max[1]; d.bmaxz = p->box_max[2];
@@ -181,24 +181,15 @@ __host__ inline DevParams make_dev_params(const grasph_params* p)
     return d;
 }
 
-// src/computeDerivatives.h:33-47 environmentAcceleration.  The
-// Clohessy-Wiltshire terms add a scalar to all three components, which is
-// the literal meaning of `f3 += f1` in the reference expression.
-__device__ inline void environment_acceleration(const DevParams& P, float px, float py, float pz, float vx, float vy,
-                                                float vz, float& ax, float& ay, float& az)
+// src/computeDerivatives.h:33-47 environmentAcceleration: the constant
+// acceleration (:42-44).  CLOHESSY_WILTSHIRE is rejected by the parameter
+// check: its expression adds a scalar to a float3 through an mpUtils
+// operator that no available source defines.
+__device__ inline void environment_acceleration(const DevParams& P, float& ax, float& ay, float& az)
 {
-    if (P.flags & GRASPH_CLOHESSY_WILTSHIRE) {
-        const float a1 = 3.0f * P.cw_n * P.cw_n * px + 2.0f * P.cw_n * vy;
-        ax += a1; ay += a1; az += a1;
-        const float a2 = -2.0f * P.cw_n * vx;
-        ax += a2; ay += a2; az += a2;
-        const float a3 = -P.cw_n * P.cw_n * pz;
-        ax += a3; ay += a3; az += a3;
-    }
     if (P.flags & GRASPH_CONSTANT_ACCELERATION) {
         ax += P.cax; ay += P.cay; az += P.caz;
     }
-    (void)py;
 }
 
 #pragma clang fp contract(fast)

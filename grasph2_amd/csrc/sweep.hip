@@ -173,9 +173,7 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(const float4* __restric
     }
     float ax = x.value(), ay = y.value(), az = z.value();
     if (finish) {
-        const float4 p = posm[i];
-        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        environment_acceleration(P, p.x, p.y, p.z, v.x, v.y, v.z, ax, ay, az);
+        environment_acceleration(P, ax, ay, az);
         if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (maxvsig) maxvsig[i] = 0.0f;
     }

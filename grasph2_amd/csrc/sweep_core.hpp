@@ -242,8 +242,7 @@ __device__ __forceinline__ void sweep_slots(const float4* __restrict__ xs, const
             float ax = GRAV ? a.sx[q].value() : 0.0f, ay = GRAV ? a.sy[q].value() : 0.0f;
             float az = GRAV ? a.sz[q].value() : 0.0f;
             if (finish) {
-                const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                environment_acceleration(P, xi[q], yi[q], zi[q], v.x, v.y, v.z, ax, ay, az);
+                environment_acceleration(P, ax, ay, az);
             }
             acc[i] = make_float4(ax, ay, az, 0.0f);
         }

@@ -600,9 +600,7 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
     }
     float ax = x.value(), ay = y.value(), az = z.value();
     if (finish) {
-        const float4 p = xs[t];
-        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        environment_acceleration(P, p.x, p.y, p.z, v.x, v.y, v.z, ax, ay, az);
+        environment_acceleration(P, ax, ay, az);
         if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (maxvsig) maxvsig[i] = 0.0f;
     }
@@ -800,9 +798,7 @@ __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
     }
     float ax = x.value(), ay = y.value(), az = z.value();
     if (finish) {
-        const float4 p = xs[t];
-        const float4 v = vel ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        environment_acceleration(prm[k], p.x, p.y, p.z, v.x, v.y, v.z, ax, ay, az);
+        environment_acceleration(prm[k], ax, ay, az);
         if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (maxvsig) maxvsig[i] = 0.0f;
     }

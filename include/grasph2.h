@@ -35,7 +35,7 @@ extern "C" {
 #define GRASPH_OK 0
 #define GRASPH_E_INVALID (-1)     /* bad argument: null pointer, bad size, bad flag combination */
 #define GRASPH_E_HIP (-2)         /* a HIP runtime call failed */
-#define GRASPH_E_UNSUPPORTED (-3) /* combination the reference does not build either */
+#define GRASPH_E_UNSUPPORTED (-3) /* combination the reference does not build, or whose meaning it leaves open */
 #define GRASPH_E_NOMEM (-4)       /* device allocation of library scratch failed */
 
 /* ---- physics switches (settings/settings.h) ---------------------------- */
@@ -46,7 +46,10 @@ extern "C" {
 #define GRASPH_SOLIDS                (1u << 4)  /* SOLIDS                    settings.h:42  */
 #define GRASPH_PLASTICITY_MIESE      (1u << 5)  /* PLASTICITY_MIESE          settings.h:116 */
 #define GRASPH_PLASTICITY_MC         (1u << 6)  /* PLASTICITY_MC             settings.h:120 */
-#define GRASPH_CLOHESSY_WILTSHIRE    (1u << 7)  /* CLOHESSY_WILTSHIRE        settings.h:136 */
+#define GRASPH_CLOHESSY_WILTSHIRE    (1u << 7)  /* CLOHESSY_WILTSHIRE        settings.h:136:
+                                                   GRASPH_E_UNSUPPORTED (computeDerivatives.h:37-39
+                                                   adds a scalar to a float3 through an mpUtils
+                                                   operator whose meaning no source here fixes) */
 #define GRASPH_CONSTANT_ACCELERATION (1u << 8)  /* CONSTANT_ACCELERATION     settings.h:141 */
 #define GRASPH_SIMPLE_BOX_BOUNDARY   (1u << 9)  /* SIMPLE_BOX_BOUNDARY       settings.h:146 */
 #define GRASPH_ARTIFICIAL_VISCOSITY  (1u << 10) /* ARTIFICIAL_VISCOSITY      settings.h:155 */
@@ -80,7 +83,7 @@ typedef struct grasph_params {
     float matexp;            /* artificial stress exponent      settings.h:165 */
     float normalsep;         /* artificial stress separation    settings.h:166 */
     float xsph_factor;       /* XSPH factor                     settings.h:171 */
-    float cw_n;              /* Clohessy-Wiltshire n            settings.h:137 */
+    float cw_n;              /* Clohessy-Wiltshire n (unused)   settings.h:137 */
     float constant_acceleration[3];   /*                        settings.h:142 */
     float box_min[3];        /*                                 settings.h:147 */
     float box_max[3];        /*                                 settings.h:148 */
@@ -148,9 +151,7 @@ int grasph_destroy(grasph_ctx* ctx);
  * "profile" (1 = record HIP events around each kernel), "sweep_i_per_lane"
  * (1 or 2), "sweep_slices" (j-split of the sweep, 0 = automatic),
  * "batch_waves_per_block", "batch_blocks_per_cu" (batched sweep launch
- * shape), "timestep_reset" (1 = the variable-step leapfrog starts each
- * step's dt minimum from max_timestep, so dt can grow again; 0 = the
- * reference's running minimum, the default), "sweep_symmetric" (single
+ * shape), "sweep_symmetric" (single
  * instance: 1 = gravity between groups of 1024 particles evaluated once per
  * pair for both sides, 0 = the one-sided sweep, 2 = the symmetric sweep from
  * 16384 particles on when neighbour lists are built, from 32768 for gravity
@@ -178,7 +179,10 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value);
  * Runs the passes the flags select, in the reference's order:
  * density summation (:320), strain rate / Balsara / density rate (:323),
  * acceleration (:327).  Fused here into one all-pairs sweep (gravity,
- * density, neighbour list) plus two neighbour-list passes. */
+ * density, neighbour list) plus two neighbour-list passes.
+ * With GRASPH_VARIABLE_TIMESTEP the acceleration pass (here and in
+ * grasph_calc_acceleration) sets the context's next dt to max_timestep, as
+ * calcAcceleration::do_after sets nextTS (:305-308). */
 int grasph_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p,
                                const grasph_params* prm, void* stream);
 
@@ -198,9 +202,10 @@ int grasph_calc_acceleration(grasph_ctx* ctx, const grasph_particles* p,
  * GRASPH_VARIABLE_TIMESTEP: the two kernels of VARIABLE_TIMESTEP_LEAPFROG,
  * src/integration.h:127-217.  With not_first_step, a half kick with the
  * current dt that also folds the particles' dt criteria into the next dt
- * (a minimum that, as in the reference, is not reset between steps: dt
- * never grows unless the option "timestep_reset" is set); without it the
- * next dt is set to initial_timestep.  Then a half kick and the drift with
+ * (an atomic minimum; the acceleration pass of the derivative call before
+ * it set the next dt to max_timestep, so dt = min(max_timestep, criteria)
+ * and grows again when the criteria loosen); without it the next dt is set
+ * to initial_timestep.  Then a half kick and the drift with
  * dt = max(next dt, min_timestep), which becomes the current dt.  The two
  * values live in the context (zero at creation, like the reference's
  * __device__ currentTS / nextTS), so the calls stay asynchronous. */
@@ -229,7 +234,9 @@ typedef struct grasph_batch {
 
 /* computeDerivatives for every instance: one persistent all-pairs launch
  * over a queue of (instance, tile) items, longest instances first, then one
- * launch per neighbour pass over all instances. */
+ * launch per neighbour pass over all instances.  With
+ * GRASPH_VARIABLE_TIMESTEP every instance's next dt := its max_timestep
+ * afterwards (computeDerivatives.h:305-308). */
 int grasph_batch_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_batch* b,
                                      void* stream);
 /* integrate(pb, notFirstStep) for every instance, one launch (two with the

@@ -39,16 +39,29 @@
  *                                sum m a = 0
  *   calcAcceleration SPH         uniform-pressure interior -> ~0; linear
  *                                pressure field -> a ~ -grad p / rho; sum m a = 0
+ *   artificial pressure          a pair by hand: f = (W(r)/W(normalsep))^matexp,
+ *                                the tension-only gate, f = 1 at normalsep
+ *   XSPH                         a pair by hand, 2 m_j/(rho_i+rho_j)(v_j-v_i)W;
+ *                                uniform flow -> 0; sum m xvel = 0; the drift
+ *   density rate                 v = L x lattice: rho_dot = -rho tr L
+ *   Balsara (liquid curl)        v = L x lattice: |div| / (|div|+|curl|+eps),
+ *                                rigid rotation -> 0, expansion -> 1
  *   fixedTsLeapfrog              circular two-body orbit: energy and angular
  *                                momentum bounded over many periods, period
  *                                matches Kepler; half-kick first step
+ *   Mohr-Coulomb yield           leapfrog by hand: Y = tan(phi) p + c in
+ *                                compression, at rest density and in tension
+ *   variable step                criteria by hand; the acceleration pass's
+ *                                reset of the next dt; two-body main loop
+ *                                dt = min(max_timestep, criteria); constant
+ *                                acceleration integrated exactly
  *   invariants / potential       two-body closed forms
  * Parity unpinned (no reference test, fixture or buildable reference covers
- * them, and the mathematics fixes only their form): the mpUtils operator
- * semantics (component-wise float3 ops, Mat(T) diagonal, Mat*vec by rows),
- * CLOHESSY_WILTSHIRE (a scalar is added to all three components, the literal
- * reading of src/computeDerivatives.h:37-39), ARTIFICIAL_STRESS and XSPH
- * (flagged broken upstream, todo.md:13).
+ * it, and the mathematics fixes only its form): the mpUtils operator
+ * semantics (component-wise float3 ops, Mat(T) diagonal, Mat*vec by rows;
+ * immaterial for the symmetric tensors used here).  CLOHESSY_WILTSHIRE is
+ * rejected (validate() returns -3): src/computeDerivatives.h:37-39 adds a
+ * scalar to an f3_t through an mpUtils operator no available source defines.
  */
 #include "../include/grasph2.h"
 
@@ -95,7 +108,7 @@ template <typename T> inline M3<T> mzero() { M3<T> r; std::memset(&r, 0, sizeof 
 /* ---- settings-derived constants (src/computeDerivatives.h:26-28) -------- */
 template <typename T> struct Consts {
     T H, H2, Wpref, dWpref, rho0, BULK, dBULKdP, c, shear, Y, tanfr, cohesion, alpha;
-    T mateps, matexp, normalsep, xsph, cw_n, dt;
+    T mateps, matexp, normalsep, xsph, dt;
     V3<T> const_acc, bmin, bmax;
     T refl;
     T ts_init, ts_min, ts_max, acc_acc, vel_acc;  // settings.h:56-62
@@ -131,7 +144,7 @@ template <typename T> Consts<T> make_consts(const grasph_params* p)
     c.c = T(p->SOUNDSPEED); c.shear = T(p->shear); c.Y = T(p->Y);
     c.tanfr = T(p->tanfr); c.cohesion = T(p->cohesion); c.alpha = T(p->alpha);
     c.mateps = T(p->mateps); c.matexp = T(p->matexp); c.normalsep = T(p->normalsep);
-    c.xsph = T(p->xsph_factor); c.cw_n = T(p->cw_n); c.dt = T(p->fixed_timestep);
+    c.xsph = T(p->xsph_factor); c.dt = T(p->fixed_timestep);
     c.const_acc = {T(p->constant_acceleration[0]), T(p->constant_acceleration[1]), T(p->constant_acceleration[2])};
     c.bmin = {T(p->box_min[0]), T(p->box_min[1]), T(p->box_min[2])};
     c.bmax = {T(p->box_max[0]), T(p->box_max[1]), T(p->box_max[2])};
@@ -346,18 +359,12 @@ template <typename T> void calc_balsara(const Consts<T>& C, const Parts<T>& P, R
     }
 }
 
-/* computeDerivatives.h:33-47 environmentAcceleration */
-template <typename T> inline void environmentAcceleration(const Consts<T>& C, V3<T> pos, V3<T> vel, V3<T>& acc)
+/* computeDerivatives.h:33-47 environmentAcceleration: the constant
+ * acceleration (:42-44).  CLOHESSY_WILTSHIRE (:36-40) is rejected by
+ * validate(): it adds a scalar to an f3_t through an mpUtils operator
+ * (v0.12.0, not vendored) whose meaning no available source fixes. */
+template <typename T> inline void environmentAcceleration(const Consts<T>& C, V3<T>& acc)
 {
-    if (C.has(GRASPH_CLOHESSY_WILTSHIRE)) {
-        /* a scalar added to every component: the literal expression */
-        const T a1 = T(3) * C.cw_n * C.cw_n * pos.x + T(2) * C.cw_n * vel.y;
-        acc.x += a1; acc.y += a1; acc.z += a1;
-        const T a2 = T(-2) * C.cw_n * vel.x;
-        acc.x += a2; acc.y += a2; acc.z += a2;
-        const T a3 = -C.cw_n * C.cw_n * pos.z;
-        acc.x += a3; acc.y += a3; acc.z += a3;
-    }
     if (C.has(GRASPH_CONSTANT_ACCELERATION)) acc += C.const_acc;
 }
 
@@ -443,7 +450,7 @@ template <typename T> void calc_acceleration(const Consts<T>& C, const Parts<T>&
                 if (xsph) xvel += T(2) * mj / (rhoi + rhoj) * (vj - vi) * Wspline(r, C.H, C.Wpref);
             }
         }
-        environmentAcceleration(C, pi, vi, acc);
+        environmentAcceleration(C, acc);
         P.s4(P.acc, i, acc);
         P.s4(P.xvel, i, xvel);
         if (P.maxvsig) P.maxvsig[i] = maxvsig;
@@ -551,6 +558,7 @@ template <typename T> int validate(const grasph_params* p)
     if (!p) return -1;
     if (p->dimension < 1 || p->dimension > 3) return -1;
     if ((p->flags & GRASPH_SOLIDS) && (p->flags & GRASPH_ARTIFICIAL_STRESS)) return -3;
+    if (p->flags & GRASPH_CLOHESSY_WILTSHIRE) return -3;
     if ((p->flags & GRASPH_PLASTICITY_MIESE) && (p->flags & GRASPH_PLASTICITY_MC)) return -1;
     return 0;
 }
@@ -568,8 +576,11 @@ template <typename T> Parts<T> parts_from(const oracle_particles_t* p)
     return r;
 }
 
+/* ts: the variable step's {currentTS, nextTS} (integration.h:127-128), or
+ * NULL.  With VARIABLE_TIMESTEP_LEAPFROG, calcAcceleration::do_after sets
+ * nextTS = max_timestep (computeDerivatives.h:305-308). */
 template <typename T> int pass(int which, const grasph_params* prm, const oracle_particles_t* p, const int64_t* s,
-                               int64_t ns)
+                               int64_t ns, T* ts)
 {
     int v = validate<T>(prm);
     if (v) return v;
@@ -578,22 +589,25 @@ template <typename T> int pass(int which, const grasph_params* prm, const oracle
     Range R{s, s ? ns : P.n};
     if (which == 0) calc_density(C, P, R);
     else if (which == 1) calc_balsara(C, P, R);
-    else calc_acceleration(C, P, R);
+    else {
+        calc_acceleration(C, P, R);
+        if (ts && C.has(GRASPH_VARIABLE_TIMESTEP)) ts[1] = C.ts_max;
+    }
     return 0;
 }
 
 /* computeDerivatives.h:315-328 */
-template <typename T> int derivs(const grasph_params* prm, const oracle_particles_t* p)
+template <typename T> int derivs(const grasph_params* prm, const oracle_particles_t* p, T* ts)
 {
     int v = validate<T>(prm);
     if (v) return v;
     const uint32_t f = prm->flags;
     if (f & GRASPH_SPH) {
-        if (!(f & GRASPH_INTEGRATE_DENSITY)) pass<T>(0, prm, p, nullptr, 0);
+        if (!(f & GRASPH_INTEGRATE_DENSITY)) pass<T>(0, prm, p, nullptr, 0, ts);
         if ((f & GRASPH_SOLIDS) || (f & GRASPH_INTEGRATE_DENSITY) || (f & GRASPH_BALSARA_SWITCH))
-            pass<T>(1, prm, p, nullptr, 0);
+            pass<T>(1, prm, p, nullptr, 0, ts);
     }
-    pass<T>(2, prm, p, nullptr, 0);
+    pass<T>(2, prm, p, nullptr, 0, ts);
     return 0;
 }
 
@@ -687,22 +701,27 @@ int oracle_get_threads(void)
 #endif
 }
 
-/* passes on a sample of i (NULL = all); j always runs over all particles */
+/* passes on a sample of i (NULL = all); j always runs over all particles;
+ * ts = {currentTS, nextTS} in the precision of the suffix or NULL */
 int oracle_calc_density_f32(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns)
-{ return pass<float>(0, prm, p, s, ns); }
+{ return pass<float>(0, prm, p, s, ns, nullptr); }
 int oracle_calc_density_f64(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns)
-{ return pass<double>(0, prm, p, s, ns); }
+{ return pass<double>(0, prm, p, s, ns, nullptr); }
 int oracle_calc_balsara_f32(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns)
-{ return pass<float>(1, prm, p, s, ns); }
+{ return pass<float>(1, prm, p, s, ns, nullptr); }
 int oracle_calc_balsara_f64(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns)
-{ return pass<double>(1, prm, p, s, ns); }
-int oracle_calc_acceleration_f32(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns)
-{ return pass<float>(2, prm, p, s, ns); }
-int oracle_calc_acceleration_f64(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns)
-{ return pass<double>(2, prm, p, s, ns); }
+{ return pass<double>(1, prm, p, s, ns, nullptr); }
+int oracle_calc_acceleration_f32(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns,
+                                 float* ts)
+{ return pass<float>(2, prm, p, s, ns, ts); }
+int oracle_calc_acceleration_f64(const grasph_params* prm, const oracle_particles_t* p, const int64_t* s, int64_t ns,
+                                 double* ts)
+{ return pass<double>(2, prm, p, s, ns, ts); }
 
-int oracle_compute_derivatives_f32(const grasph_params* prm, const oracle_particles_t* p) { return derivs<float>(prm, p); }
-int oracle_compute_derivatives_f64(const grasph_params* prm, const oracle_particles_t* p) { return derivs<double>(prm, p); }
+int oracle_compute_derivatives_f32(const grasph_params* prm, const oracle_particles_t* p, float* ts)
+{ return derivs<float>(prm, p, ts); }
+int oracle_compute_derivatives_f64(const grasph_params* prm, const oracle_particles_t* p, double* ts)
+{ return derivs<double>(prm, p, ts); }
 
 int oracle_fixed_ts_leapfrog_f32(const grasph_params* prm, const oracle_particles_t* p, int nfs)
 {

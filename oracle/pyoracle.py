@@ -35,13 +35,16 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER(OracleParticles)
         for prec in ("f32", "f64"):
-            for name in ("calc_density", "calc_balsara", "calc_acceleration"):
+            for name in ("calc_density", "calc_balsara"):
                 fn = getattr(L, "oracle_%s_%s" % (name, prec))
                 fn.restype = ctypes.c_int
                 fn.argtypes = [ctypes.c_void_p, P, ctypes.c_void_p, ctypes.c_int64]
+            fn = getattr(L, "oracle_calc_acceleration_" + prec)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_void_p, P, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
             fn = getattr(L, "oracle_compute_derivatives_" + prec)
             fn.restype = ctypes.c_int
-            fn.argtypes = [ctypes.c_void_p, P]
+            fn.argtypes = [ctypes.c_void_p, P, ctypes.c_void_p]
             fn = getattr(L, "oracle_fixed_ts_leapfrog_" + prec)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_void_p, P, ctypes.c_int]
@@ -137,16 +140,28 @@ def calc_balsara(prm, h, prec="f64", sample=None):
     return b.arrays
 
 
-def calc_acceleration(prm, h, prec="f64", sample=None):
+def _ts(state, prec):
+    if state is None:
+        return None
+    if state.prec != prec:
+        raise ValueError("time-step state precision %s, oracle precision %s" % (state.prec, prec))
+    return state.ts.ctypes.data
+
+
+def calc_acceleration(prm, h, prec="f64", sample=None, state=None):
+    """calcAcceleration; with the variable step, `state` (a TimestepState of
+    the same precision) gets next dt = max_timestep as do_after does."""
     b = _Bound(h, prec)
     sp, ns, keep = _sample(sample)
-    _check(getattr(lib(), "oracle_calc_acceleration_" + prec)(ctypes.byref(prm), ctypes.byref(b.view), sp, ns))
+    _check(getattr(lib(), "oracle_calc_acceleration_" + prec)(ctypes.byref(prm), ctypes.byref(b.view), sp, ns,
+                                                              _ts(state, prec)))
     return b.arrays
 
 
-def compute_derivatives(prm, h, prec="f64"):
+def compute_derivatives(prm, h, prec="f64", state=None):
     b = _Bound(h, prec)
-    _check(getattr(lib(), "oracle_compute_derivatives_" + prec)(ctypes.byref(prm), ctypes.byref(b.view)))
+    _check(getattr(lib(), "oracle_compute_derivatives_" + prec)(ctypes.byref(prm), ctypes.byref(b.view),
+                                                                _ts(state, prec)))
     return b.arrays
 
 

@@ -64,7 +64,7 @@ def test_matches_one_sided_batch_and_fp64(sym_batch_ctx, gravity_only):
             assert np.array_equal(a["dstress_dt"], r["dstress_dt"]), k
     o = O.compute_derivatives(params[1], insts[1], "f64")
     mx, nr = vec_errors(got[1]["acc"], o["acc"])
-    assert nr <= 1e-5 and mx <= 1e-4, (mx, nr)
+    assert nr <= 1e-5 and mx <= 1e-5, (mx, nr)
 
 
 def test_graph_steps_match_eager_and_repeat(sym_batch_ctx):

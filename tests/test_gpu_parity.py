@@ -53,8 +53,10 @@ def randomise(h, prm, seed, vscale=0.05, sscale=0.02):
 
 
 def check_vec(got, ref, tol=TOL, what="acc"):
+    """Per-particle (|a_ref| >= 1e-3 median) and norm-wise relative error, both <= tol."""
     mx, nr = vec_errors(got, ref)
-    assert nr <= tol and mx <= 10 * tol, "%s: max per-particle rel %.3g, norm-wise %.3g" % (what, mx, nr)
+    print("[parity] %s: max per-particle rel %.3g, norm-wise %.3g" % (what, mx, nr), flush=True)
+    assert nr <= tol and mx <= tol, "%s: max per-particle rel %.3g, norm-wise %.3g" % (what, mx, nr)
     return mx, nr
 
 
@@ -116,7 +118,7 @@ def test_liquid_derivatives(gpu_ctx):
     out = gpu_run(gpu_ctx, h, prm)
     ref = O.compute_derivatives(prm, h, "f64")
     mx, nr = scalar_errors(out["density"], ref["density"])
-    assert nr <= TOL and mx <= 10 * TOL, (mx, nr)
+    assert nr <= TOL and mx <= TOL, (mx, nr)
     check_vec(out["acc"], ref["acc"])
     # AV without the Balsara switch does not track the signal velocity
     assert np.all(out["maxvsig"] == 0) and np.all(ref["maxvsig"] == 0)
@@ -128,7 +130,7 @@ def test_solid_derivatives(gpu_ctx, gravity):
     out = gpu_run(gpu_ctx, h, prm)
     ref = O.compute_derivatives(prm, h, "f64")
     mx, nr = scalar_errors(out["density"], ref["density"])
-    assert nr <= TOL and mx <= 10 * TOL, ("density", mx, nr)
+    assert nr <= TOL and mx <= TOL, ("density", mx, nr)
     check_vec(out["acc"], ref["acc"])
     mx, nr = tensor_errors(out["dstress_dt"], ref["dstress_dt"])
     assert mx <= 10 * TOL and nr <= TOL, ("dstress_dt", mx, nr)
@@ -167,10 +169,10 @@ def test_liquid_extras(gpu_ctx):
 
 @pytest.mark.parametrize("case", ["solid", "gravity", "none"])
 def test_environment_forces(gpu_ctx, case):
-    """Clohessy-Wiltshire and constant acceleration on top of SPH + gravity
-    (added by the acceleration pass), on gravity alone (added where the
-    sweep stores its sums: j-split combine at this N) and with neither
-    (the sweep only finishes), against the FP64 oracle."""
+    """Constant acceleration on top of SPH + gravity (added by the
+    acceleration pass), on gravity alone (added where the sweep stores its
+    sums: j-split combine at this N) and with neither (the sweep only
+    finishes), against the FP64 oracle."""
     if case == "solid":
         h, prm = solid_case(2500, seed=23)
     else:
@@ -178,8 +180,7 @@ def test_environment_forces(gpu_ctx, case):
         h["vel"][:, :3] = np.random.default_rng(24).normal(scale=0.3, size=(5000, 3))
         if case == "none":
             prm.flags &= ~g.SELF_GRAVITY
-    prm.flags |= g.CLOHESSY_WILTSHIRE | g.CONSTANT_ACCELERATION
-    prm.cw_n = 0.7
+    prm.flags |= g.CONSTANT_ACCELERATION
     prm.constant_acceleration[:] = (0.1, -0.5, 0.25)
     out = gpu_run(gpu_ctx, h, prm)
     ref = O.compute_derivatives(prm, h, "f64")
@@ -321,6 +322,12 @@ def test_errors_are_reported(gpu_ctx):
     with pytest.raises(g.GrasphError) as e:
         gpu_ctx.compute_derivatives(d, bad)
     assert e.value.code == g.E_UNSUPPORTED
+    bad = prm.copy()
+    bad.flags |= g.CLOHESSY_WILTSHIRE  # meaning left open by an mpUtils operator: rejected
+    for call in (gpu_ctx.compute_derivatives, gpu_ctx.calc_acceleration, gpu_ctx.integrate):
+        with pytest.raises(g.GrasphError) as e:
+            call(d, bad)
+        assert e.value.code == g.E_UNSUPPORTED and "CLOHESSY" in str(e.value)
     v = d.view()
     v.dstress = None
     with pytest.raises(g.GrasphError) as e:

@@ -76,7 +76,7 @@ def test_sph_matches_fp64_and_one_sided_lists(gpu_ctx, sym_ctx, case):
     assert np.array_equal(out["density"], one["density"])
     ref = O.compute_derivatives(prm, h, "f64")
     mx, nr = scalar_errors(out["density"], ref["density"])
-    assert nr <= TOL and mx <= 10 * TOL, ("density", mx, nr)
+    assert nr <= TOL and mx <= TOL, ("density", mx, nr)
     check_vec(out["acc"], ref["acc"])
     mx, nr = vec_errors(out["acc"], one["acc"])
     assert nr <= 1e-6 and mx <= 1e-5, ("symmetric vs one-sided", mx, nr)

@@ -378,6 +378,167 @@ def test_leapfrog_stress_density_boundary():
     assert o["vel"][1, 0] == pytest.approx(-0.5)  # reflected with reflectiveness 0.5
 
 
+def test_mohr_coulomb_yield_by_hand():
+    """PLASTICITY_MC (integration.h:24-27, models.h:207-210): Y = tan(phi) p + c
+    with the Murnaghan pressure of the density after the drift; factor
+    Y^2 / (3 J2) below yield, unchanged above.  Three particles: compressed
+    (p = 1), at rest density (p = 0: only the cohesion), in tension (p = -1:
+    the sign of p enters Y, and Y^2 = 0.79^2 rather than 0.81^2)."""
+    prm = ics.solid_params(0.1, dt=0.01)  # rho0 = 1, K = 2, n = 1
+    prm.flags = (prm.flags | g.INTEGRATE_DENSITY | g.PLASTICITY_MC) & ~g.PLASTICITY_MIESE
+    prm.tanfr, prm.cohesion = 0.8, 0.01
+    h = make(np.array([[0.0, 0, 0], [3.0, 0, 0], [6.0, 0, 0]]), np.ones(3))
+    h["density"][:] = [1.4, 1.0, 0.5]
+    h["density_dt"][:] = [10.0, 0.0, 0.0]  # particle 0 reaches rho = 1.5 in the drift: p = 2 (1.5 - 1) = 1
+    h["dstress_dt"][3, :] = np.array([1.0, 0.005, 1.0]) / prm.fixed_timestep  # S_xy after the drift
+    o = O.leapfrog(prm, h, not_first_step=True)
+    assert o["density"][0] == pytest.approx(1.5, rel=1e-7)  # inputs stored in FP32
+    J2 = 1.0  # 1/2 (S_xy^2 + S_yx^2) with S_xy = 1
+    assert o["dstress"][3, 0] == pytest.approx(0.81 ** 2 / (3 * J2), rel=1e-6)
+    assert o["dstress"][3, 1] == pytest.approx(0.005, rel=1e-6)  # Y = c = 0.01: f = 4/3 > 1, unchanged
+    assert o["dstress"][3, 2] == pytest.approx(0.79 ** 2 / (3 * J2), rel=1e-6)
+    # without INTEGRATE_DENSITY the stored density sets the pressure
+    prm.flags &= ~g.INTEGRATE_DENSITY
+    o = O.leapfrog(prm, h, not_first_step=True)
+    assert o["dstress"][3, 0] == pytest.approx((0.8 * 2 * 0.4 + 0.01) ** 2 / 3, rel=1e-6)
+
+
+def _liquid_pair(rho_i, rho_j, r, H=0.1, arts=True, matexp=4.0, normalsep=0.03):
+    prm = ics.liquid_params(H, gravity=False, c=1.0)
+    prm.flags &= ~g.ARTIFICIAL_VISCOSITY
+    if arts:
+        prm.flags |= g.ARTIFICIAL_STRESS
+    prm.mateps, prm.matexp, prm.normalsep = 0.4, matexp, normalsep
+    h = make(np.array([[0.0, 0, 0], [r, 0, 0]]), np.array([1.0, 2.0]))
+    h["density"][:] = [rho_i, rho_j]
+    return prm, h
+
+
+def test_artificial_pressure_by_hand():
+    """Liquid ARTIFICIAL_STRESS (computeDerivatives.h:263-272, models.h:218-221):
+    the pair term adds f (R_i + R_j) with f = (W(r) / W(normalsep))^matexp and
+    R = -mateps sigma/rho^2 where sigma/rho^2 > 0 (tension), 0 otherwise.  The
+    change it makes to a_i is m_j f (R_i + R_j) dW(r)/r r_ij.  Inputs are
+    stored in FP32, hence rel 1e-6."""
+    r, H = 0.06, 0.1
+    Hf = float(np.float32(H))
+
+    def delta(rho_i, rho_j, rr=r, matexp=4.0, ns=0.03):
+        on = O.calc_acceleration(*_liquid_pair(rho_i, rho_j, rr, matexp=matexp, normalsep=ns))["acc"][0, :3]
+        off = O.calc_acceleration(*_liquid_pair(rho_i, rho_j, rr, arts=False))["acc"][0, :3]
+        return on - off
+
+    gradx = O.dW(r, Hf) / r * (-r)  # r_ij = x_i - x_j = -r along x
+    # both in tension: sigma = -c^2 (rho - rho0) > 0
+    sor_i, sor_j = 0.1 / 0.9 ** 2, 0.2 / 0.8 ** 2
+    f = (O.W(r, Hf) / O.W(float(np.float32(0.03)), Hf)) ** 4
+    d = delta(0.9, 0.8)
+    assert d[0] == pytest.approx(2.0 * f * (-0.4 * sor_i - 0.4 * sor_j) * gradx, rel=1e-6)
+    assert d[1] == 0 and d[2] == 0
+    # mixed: only the particle in tension contributes (the sign gate)
+    d = delta(0.9, 1.1)
+    assert d[0] == pytest.approx(2.0 * f * (-0.4 * sor_i) * gradx, rel=1e-6)
+    # both compressed: nothing
+    assert np.all(delta(1.1, 1.2) == 0)
+    # at r = normalsep, f = 1 whatever matexp is
+    ns = float(np.float32(r))
+    d4, d2 = delta(0.9, 0.8, ns=ns, matexp=4.0), delta(0.9, 0.8, ns=ns, matexp=2.0)
+    assert d4[0] == pytest.approx(d2[0], rel=1e-6)  # r in FP64, normalsep in FP32
+    assert d4[0] == pytest.approx(2.0 * (-0.4 * sor_i - 0.4 * sor_j) * gradx, rel=1e-6)
+
+
+def test_xsph_by_hand():
+    """XSPH (computeDerivatives.h:291-294): xvel_i = sum_j 2 m_j / (rho_i + rho_j)
+    (v_j - v_i) W(r); a pair by hand, zero for a uniform velocity field, and
+    sum_i m_i xvel_i = 0 (antisymmetric pair terms); the drift moves with
+    v + xsph_factor xvel (integration.h:98-99)."""
+    H = 0.1
+    Hf = float(np.float32(H))
+    prm = ics.liquid_params(H, gravity=False)
+    prm.flags = g.SPH | g.XSPH
+    h = make(np.array([[0.0, 0, 0], [0.06, 0, 0], [5.0, 0, 0]]), np.array([1.0, 2.0, 1.0]),
+             vel=np.array([[1.0, 0, 0], [0.0, 1.0, 0], [0.0, 0, 0]]))
+    h["density"][:] = [1.0, 3.0, 1.0]
+    o = O.calc_acceleration(prm, h)
+    w = O.W(float(np.float32(0.06)), Hf)
+    assert np.allclose(o["xvel"][0, :3], np.array([-1.0, 1.0, 0.0]) * (2 * 2.0 / 4.0) * w, rtol=1e-6)
+    assert np.allclose(o["xvel"][1, :3], np.array([1.0, -1.0, 0.0]) * (2 * 1.0 / 4.0) * w, rtol=1e-6)
+    assert np.all(o["xvel"][2, :3] == 0)
+    m = h["posm"][:, 3].astype(np.float64)
+    assert np.allclose((o["xvel"][:, :3] * m[:, None]).sum(0), 0, atol=1e-9 * w)
+    # uniform flow on a lattice: no correction
+    pos = cubic_lattice(6, 0.05)
+    hu = make(pos, np.full(len(pos), 0.05 ** 3), vel=np.tile([0.3, -0.2, 0.1], (len(pos), 1)))
+    hu["density"][:] = 1.0
+    prm.H = 0.125
+    assert np.all(O.calc_acceleration(prm, hu)["xvel"][:, :3] == 0)
+    # the drift: x += (v + xsph_factor xvel) dt
+    prm.H = H
+    prm.fixed_timestep = 0.01
+    prm.xsph_factor = 0.5
+    st = dict(h)
+    st["xvel"] = o["xvel"]
+    lf = O.leapfrog(prm, st, not_first_step=True)
+    dt = float(np.float32(0.01))  # as grasph_params holds it
+    expect = h["posm"][0, :3] + (h["vel"][0, :3] + 0.5 * o["xvel"][0, :3]) * dt
+    assert np.allclose(lf["posm"][0, :3], expect, rtol=0, atol=1e-12)
+
+
+def _liquid_rate_case(L, rho=2.0):
+    """liquid lattice with v = L x, rho = 2 and m = 2 d^3 (m/rho = volume)"""
+    d = 0.05
+    H = 2.5 * d
+    pos = cubic_lattice(14, d)
+    prm = ics.liquid_params(H, gravity=False, rho0=1.0)
+    n = len(pos)
+    h = make(pos, np.full(n, rho * d ** 3), vel=pos @ np.asarray(L).T)
+    h["density"][:] = rho
+    idx = interior(pos, H * 1.05)
+    return prm, h, idx
+
+
+def test_density_rate_is_minus_rho_div_v():
+    """INTEGRATE_DENSITY (computeDerivatives.h:136-138,152-155):
+    rho_dot = -rho div v; v = L x on a lattice gives div v = tr L, so
+    rho_dot = -2 tr L at rho = 2 (a dropped rho_i would give -tr L)."""
+    L = np.array([[0.3, 1.0, 0.0], [0.0, -0.1, 0.2], [0.4, 0.0, 0.5]])
+    for solid in (False, True):
+        prm, h, idx = _liquid_rate_case(L)
+        prm.flags |= g.INTEGRATE_DENSITY
+        if solid:
+            prm.flags |= g.SOLIDS
+            prm.BULK, prm.dBULKdP, prm.shear = 2.0, 1.0, 1.0
+        out = O.calc_balsara(prm, h, sample=idx)
+        assert np.allclose(out["density_dt"][idx], -2.0 * np.trace(L), rtol=0.02), solid
+
+
+def test_liquid_balsara_curl():
+    """Liquid BALSARA_SWITCH (computeDerivatives.h:133-135): curl v from
+    sum m_j/rho_j v_ij x grad W, div v from the same weights; on v = L x,
+    |curl| = |(L32 - L23, L13 - L31, L21 - L12)| and div = tr L, so
+    beta = |div| / (|div| + |curl| + 1e-4 c/H); pure rotation gives ~0,
+    pure expansion ~1."""
+    cases = [
+        np.array([[0.3, 1.0, 0.0], [0.0, -0.1, 0.2], [0.4, 0.0, 0.2]]),   # mixed, asymmetric
+        np.array([[0.0, -0.5, 0.2], [0.5, 0.0, -0.3], [-0.2, 0.3, 0.0]]),  # rigid rotation
+        np.eye(3) * 0.4,                                                   # expansion
+    ]
+    for k, L in enumerate(cases):
+        prm, h, idx = _liquid_rate_case(L)
+        prm.flags |= g.BALSARA_SWITCH
+        out = O.calc_balsara(prm, h, sample=idx)
+        div = np.trace(L)
+        curl = np.array([L[2, 1] - L[1, 2], L[0, 2] - L[2, 0], L[1, 0] - L[0, 1]])
+        b = abs(div) / (abs(div) + np.linalg.norm(curl) + 1e-4 * prm.SOUNDSPEED / prm.H)
+        got = out["balsara"][idx]
+        if k == 0:
+            assert np.allclose(got, b, rtol=0.03)
+        elif k == 1:
+            assert np.all(got < 1e-3)
+        else:
+            assert np.all(got > 0.99)
+
+
 # ---------------------------------------------------------------- variable step
 def _vts_params(H=0.05):
     prm = ics.plummer_params(H=H, dt=0.002)
@@ -396,62 +557,123 @@ def _vts_state(acc, maxvsig):
 
 
 def test_variable_step_criteria_by_hand():
-    """Parameters are FP32 (as in grasph_params), hence the 1e-6 tolerances."""
+    """getNewTimestep's criteria (integration.h:134,149-154) and
+    useNewTimestep's clamp (:170), each step starting from the next dt an
+    acceleration pass leaves (max_timestep; the reset itself is pinned by
+    test_acceleration_pass_resets_next_dt).  Parameters are FP32 (as in
+    grasph_params), hence the 1e-6 tolerances."""
     prm = _vts_params()
     st = O.TimestepState("f64")
     h = _vts_state([[3.0, 4.0, 0.0], [0.0, 0.0, 0.0]], [0.0, 2.0])
     O.variable_leapfrog(prm, h, st, not_first_step=False)
     assert (st.current, st.next) == pytest.approx((0.002, 0.002), rel=1e-7)  # first step: initial_timestep
-    # |a| = 5: sqrt(0.01 * 0.05 / 5) = 0.01; 0.09 * 0.05 / 2 = 0.00225; running min stays 0.002
-    O.variable_leapfrog(prm, h, st)
-    assert st.current == pytest.approx(0.002, rel=1e-7)
-    h = _vts_state([[300.0, 400.0, 0.0], [0.0, 0.0, 0.0]], [0.0, 2.0])  # sqrt(1e-6) = 0.001
-    O.variable_leapfrog(prm, h, st)
-    assert st.current == pytest.approx(0.001, rel=1e-6)
-    h = _vts_state([[0.0, 0.0, 0.0]], [100.0])  # 0.09 * 0.05 / 100 = 4.5e-5
-    O.variable_leapfrog(prm, h, st)
-    assert st.current == pytest.approx(4.5e-5, rel=1e-6)
-    h = _vts_state([[0.0, 0.0, 0.0]], [0.0])  # no criterion binds: dt never grows back
-    O.variable_leapfrog(prm, h, st)
-    assert st.current == pytest.approx(4.5e-5, rel=1e-6)
-    h = _vts_state([[5e8, 0.0, 0.0]], [0.0])  # sqrt(1e-12) = 1e-6 < min_timestep
-    O.variable_leapfrog(prm, h, st)
+
+    def step(acc, vsig):
+        st.ts[1] = np.float64(np.float32(prm.max_timestep))  # as the acceleration pass leaves it
+        O.variable_leapfrog(prm, _vts_state(acc, vsig), st)
+        return st.current
+
+    # |a| = 5: sqrt(0.01 * 0.05 / 5) = 0.01; 0.09 * 0.05 / 2 = 0.00225 binds
+    assert step([[3.0, 4.0, 0.0], [0.0, 0.0, 0.0]], [0.0, 2.0]) == pytest.approx(0.00225, rel=1e-6)
+    # |a| = 500: sqrt(1e-6) = 0.001 binds
+    assert step([[300.0, 400.0, 0.0], [0.0, 0.0, 0.0]], [0.0, 2.0]) == pytest.approx(0.001, rel=1e-6)
+    # 0.09 * 0.05 / 100 = 4.5e-5
+    assert step([[0.0, 0.0, 0.0]], [100.0]) == pytest.approx(4.5e-5, rel=1e-6)
+    # no criterion binds: max_timestep
+    assert step([[0.0, 0.0, 0.0]], [0.0]) == pytest.approx(0.02, rel=1e-6)
+    # sqrt(1e-12) = 1e-6 < min_timestep: the next dt keeps it, the step clamps
+    assert step([[5e8, 0.0, 0.0]], [0.0]) == pytest.approx(2e-5, rel=1e-6)
+    assert st.next == pytest.approx(1e-6, rel=1e-6)
+    # without an acceleration pass between two integrate calls the minimum carries over
+    O.variable_leapfrog(prm, _vts_state([[0.0, 0.0, 0.0]], [0.0]), st)
     assert st.next == pytest.approx(1e-6, rel=1e-6) and st.current == pytest.approx(2e-5, rel=1e-6)
 
 
-def test_variable_step_from_zero_state_uses_min_timestep():
-    """integrate(pb, true) on the very first call (as main.cu:257 does): the
-    zero-initialised next dt wins the minimum, so dt = min_timestep."""
-    prm = _vts_params()
-    for prec in ("f64", "f32"):
-        st = O.TimestepState(prec)
-        h = _vts_state([[1.0, 0.0, 0.0]], [1.0])
-        O.variable_leapfrog(prm, h, st, not_first_step=True)
-        assert st.next == 0.0
-        assert st.current == pytest.approx(2e-5, rel=1e-6)
+def _two_body(r, H=0.05):
+    h = make(np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]]), np.array([1.0, 1.0]))
+    h["density"][:] = 1.0
+    prm = _vts_params(H=H)
+    prm.flags |= g.SELF_GRAVITY
+    prm.flags &= ~g.VELOCITY_CRITERION
+    return h, prm
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_acceleration_pass_resets_next_dt(prec):
+    """calcAcceleration::do_after sets nextTS = max_timestep
+    (computeDerivatives.h:305-308) with the variable step, and only then;
+    the other passes leave it."""
+    h, prm = _two_body(0.1)
+    st = O.TimestepState(prec)
+    st.ts[:] = [0.25, 0.125]
+    O.calc_density(prm, h, prec)
+    O.calc_balsara(prm, h, prec)
+    assert list(st.ts) == [0.25, 0.125]
+    O.calc_acceleration(prm, h, prec, state=st)
+    assert st.current == 0.25 and st.next == pytest.approx(0.02, rel=1e-7)
+    st.ts[:] = [0.25, 0.125]
+    O.compute_derivatives(prm, h, prec, state=st)
+    assert st.next == pytest.approx(0.02, rel=1e-7)
+    fixed = prm.copy()
+    fixed.flags &= ~g.VARIABLE_TIMESTEP
+    st.ts[:] = [0.25, 0.125]
+    O.compute_derivatives(fixed, h, prec, state=st)
+    assert list(st.ts) == [0.25, 0.125]
+
+
+def test_main_loop_dt_is_min_of_max_and_criteria():
+    """computeDerivatives then integrate(pb, true) from the first step on, as
+    main.cu:256-257 (the state zero, as the reference's __device__ symbols
+    start): dt = min(max_timestep, sqrt(accel_accuracy H / |a|)) with the
+    two-body |a| = m r / (r^2 + H^2)^(3/2) by hand; it shrinks as the pair
+    closes and grows back, up to max_timestep, as it separates."""
+    H = 0.05
+    st = O.TimestepState("f64")
+    dts = []
+    for r in (0.1, 0.05, 1.0, 0.2):
+        h, prm = _two_body(r, H)
+        o = O.compute_derivatives(prm, h, "f64", state=st)
+        H32 = float(np.float32(H))  # grasph_params holds FP32 values
+        amag = r / (r * r + H32 * H32) ** 1.5
+        assert np.linalg.norm(o["acc"][1, :3]) == pytest.approx(amag, rel=1e-12)
+        O.variable_leapfrog(prm, o, st, not_first_step=True)
+        expect = min(float(np.float32(prm.max_timestep)), math.sqrt(float(np.float32(0.01)) * H32 / amag))
+        assert st.current == pytest.approx(expect, rel=1e-12), r
+        dts.append(st.current)
+    assert dts[1] < dts[0] < dts[3] < dts[2]
+    assert dts[2] == pytest.approx(0.02, rel=1e-7)  # max_timestep binds at r = 1
 
 
 def test_variable_step_exact_for_constant_acceleration():
     """Kick-drift-kick with synchronised half kicks integrates constant
-    acceleration exactly for any dt sequence: x = x0 + v0 t + a t^2 / 2."""
+    acceleration exactly for any dt sequence: x = x0 + v0 t + a t^2 / 2.
+    The acceleration is CONSTANT_ACCELERATION (the only force), each step
+    runs the oracle's derivative path (which resets the next dt) and then
+    integrate(pb, true); accel_accuracy changes per step, so dt shrinks and
+    grows (pins the half kick with the current dt and the drift with the new
+    one)."""
     prm = _vts_params(H=0.1)
+    prm.flags = g.VARIABLE_TIMESTEP | g.ACCELERATION_CRITERION | g.CONSTANT_ACCELERATION
     a = np.array([0.0, -2.0, 0.5])
+    prm.constant_acceleration[:] = a
     x0, v0 = np.array([0.3, 0.2, -0.1]), np.array([1.0, 0.5, 0.0])
     h = make(x0[None], np.array([1.0]), vel=v0[None], fields=("posm", "vel", "acc", "maxvsig"))
-    h["acc"][0, :3] = a
     x0, v0, a = (np.asarray(x, dtype=np.float32).astype(np.float64) for x in (x0, v0, a))  # as stored
     st = O.TimestepState("f64")
     t = 0.0
     dts = []
-    for k, vsig in enumerate([0.0, 2.0, 3.0, 5.0, 8.0, 8.0, 13.0]):
-        h["maxvsig"][:] = vsig  # shrinks dt step by step through the velocity criterion
+    for k, acc_acc in enumerate([1e-4, 4e-5, 1e-5, 6e-5, 2e-5, 1e-4, 1e-2]):
+        prm.accel_accuracy = acc_acc
+        h = O.compute_derivatives(prm, h, "f64", state=st)
         o = O.variable_leapfrog(prm, h, st, not_first_step=(k > 0))
         h["posm"], h["vel"] = o["posm"], o["vel"]
         t += st.current
         dts.append(st.current)
         assert np.allclose(h["posm"][0, :3], x0 + v0 * t + 0.5 * a * t * t, rtol=0, atol=1e-13)
         assert np.allclose(h["vel"][0, :3], v0 + a * (t - 0.5 * st.current), rtol=0, atol=1e-13)
-    assert len(set(dts)) >= 4 and all(x >= y for x, y in zip(dts, dts[1:]))
+    assert dts[0] == pytest.approx(0.002, rel=1e-7)  # integrate(pb, false): initial_timestep
+    assert dts[2] < dts[1] < dts[3]  # shrinks, grows back
+    assert dts[-1] == pytest.approx(0.02, rel=1e-7)  # max_timestep binds
 
 
 # ---------------------------------------------------------------- invariants
