@@ -374,6 +374,11 @@ public:
     {
         for (auto& s : storage_) std::fill(s.data.begin(), s.data.end(), 0);
     }
+    // the C-ABI arrays this buffer type holds, with their storage kinds
+    static std::vector<std::pair<std::string, grasph::Store>> storage_names()
+    {
+        return {{Args::name(), Args::kind}...};
+    }
 
     // loadParticle<Attrs...>(i): a particle with the requested attributes
     template <typename... P>

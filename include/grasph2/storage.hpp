@@ -152,9 +152,16 @@ void writeTextSnapshot(const std::string& filename, const HostParticleBuffer<Tag
     if (file.fail()) throw std::runtime_error("could not write output file " + filename);
 }
 
-// ResultStorageManager (src/ResultStorageManager.h:21-186): snapshots are
-// copied to host memory when queued and written by one worker thread; at
-// most maxJobs wait before printToFile blocks until the queue drains.
+// ResultStorageManager (src/ResultStorageManager.h:21-186): at most maxJobs
+// snapshots wait before printToFile blocks until the queue drains; one
+// worker thread writes them.  A device buffer is duplicated on the device
+// (src/ResultStorageManager.h:124-160): the arrays the snapshot type stores
+// are copied device-to-device on the null stream, where the library's
+// entry points are enqueued by the C++ layer, so the copy holds the state
+// after the work queued before printToFile and the simulation may go on at
+// once; the worker waits for that copy, downloads it on a stream of its own
+// and frees it.  Without device memory for the copy the state is
+// downloaded on the caller's thread instead (:148-158).
 template <typename HostDiscPT = HostParticleBuffer<HOST_POSM, HOST_VEL, HOST_DENSITY>>
 class ResultStorageManager {
 public:
@@ -178,15 +185,23 @@ public:
 
     template <typename... D> void printToFile(const DeviceParticleBuffer<D...>& particles, f1_t time)
     {
-        std::unique_ptr<HostDiscPT> h(new HostDiscPT());
-        *h = particles;  // synchronous download
-        enqueue(std::move(h), time);
+        Job j;
+        j.time = time;
+        j.dev.reset(new DeviceCopy);
+        if (!j.dev->make(particles.getDeviceReference())) {
+            j.dev.reset();
+            j.host.reset(new HostDiscPT());
+            *j.host = particles;  // no device memory for a copy: download now
+        }
+        enqueue(std::move(j));
     }
     template <typename... H> void printToFile(const HostParticleBuffer<H...>& particles, f1_t time)
     {
-        std::unique_ptr<HostDiscPT> h(new HostDiscPT(particles.size()));
-        for (size_t i = 0; i < particles.size(); ++i) h->storeParticle(i, particles.loadParticle(i));
-        enqueue(std::move(h), time);
+        Job j;
+        j.time = time;
+        j.host.reset(new HostDiscPT(particles.size()));
+        for (size_t i = 0; i < particles.size(); ++i) j.host->storeParticle(i, particles.loadParticle(i));
+        enqueue(std::move(j));
     }
     std::string getStartTime() const { return start_; }
     // the file a snapshot at `time` goes to (:161-163)
@@ -210,11 +225,78 @@ public:
     }
 
 private:
-    void enqueue(std::unique_ptr<HostDiscPT> h, f1_t time)
+    // a device-side duplicate of the arrays HostDiscPT stores
+    struct DeviceCopy {
+        int device = 0;
+        int64_t n = 0;
+        std::vector<std::pair<std::string, void*>> arrays;
+        hipEvent_t done = nullptr;  // recorded after the copies on the null stream
+        ~DeviceCopy()
+        {
+            (void)hipSetDevice(device);
+            for (auto& a : arrays) (void)hipFree(a.second);
+            if (done) (void)hipEventDestroy(done);
+        }
+        bool make(grasph_particles src)
+        {
+            n = src.n;
+            if (hipGetDevice(&device) != hipSuccess) return fail();
+            for (const auto& t : HostDiscPT::storage_names()) {
+                void* from = *detail::slot(src, t.first);
+                if (!from) continue;  // the source lacks it: the column stays zero
+                const size_t bytes = detail::bytes_of(t.second, (size_t)n);
+                void* to = nullptr;
+                if (hipMalloc(&to, bytes ? bytes : 1) != hipSuccess) return fail();
+                arrays.emplace_back(t.first, to);
+                if (hipMemcpyAsync(to, from, bytes, hipMemcpyDeviceToDevice, nullptr) != hipSuccess) return fail();
+            }
+            if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) return fail();
+            if (hipEventRecord(done, nullptr) != hipSuccess) return fail();
+            return true;
+        }
+        bool fail()
+        {
+            (void)hipGetLastError();
+            return false;
+        }
+        // on the worker thread: wait for the copies, download on a stream of its own
+        std::unique_ptr<HostDiscPT> download() const
+        {
+            hip_check(hipSetDevice(device), "hipSetDevice");
+            std::unique_ptr<HostDiscPT> h(new HostDiscPT((size_t)n));
+            hip_check(hipEventSynchronize(done), "hipEventSynchronize");
+            hipStream_t s = nullptr;
+            hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+            grasph_particles hv = h->view();
+            hipError_t e = hipSuccess;
+            for (const auto& a : arrays) {
+                void* to = *detail::slot(hv, a.first);
+                const size_t bytes = detail::bytes_of(kind_of(a.first), (size_t)n);
+                if (e == hipSuccess) e = hipMemcpyAsync(to, a.second, bytes, hipMemcpyDeviceToHost, s);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+            hip_check(e, "snapshot download");
+            return h;
+        }
+        static Store kind_of(const std::string& name)
+        {
+            for (const auto& t : HostDiscPT::storage_names())
+                if (t.first == name) return t.second;
+            throw std::logic_error("unknown storage " + name);
+        }
+    };
+    struct Job {
+        std::unique_ptr<HostDiscPT> host;
+        std::unique_ptr<DeviceCopy> dev;
+        f1_t time = 0;
+    };
+
+    void enqueue(Job j)
     {
         std::unique_lock<std::mutex> l(m_);
         done_.wait(l, [this] { return (int)q_.size() < max_; });  // queue full: wait (:113-124)
-        q_.emplace(std::move(h), time);
+        q_.push(std::move(j));
         cv_.notify_one();
     }
     void work()
@@ -229,7 +311,11 @@ private:
             l.unlock();
             std::string e;
             try {
-                writeTextSnapshot(filename(job.second), *job.first);
+                if (job.dev) {
+                    job.host = job.dev->download();
+                    job.dev.reset();
+                }
+                writeTextSnapshot(filename(job.time), *job.host);
             } catch (const std::exception& x) {
                 e = x.what();
             }
@@ -242,7 +328,7 @@ private:
 
     const std::string dir_, prefix_, start_;
     const int max_;
-    std::queue<std::pair<std::unique_ptr<HostDiscPT>, f1_t>> q_;
+    std::queue<Job> q_;
     std::mutex m_;
     std::condition_variable cv_, done_;
     bool stop_ = false, busy_ = false;
