@@ -69,6 +69,8 @@ def parse():
                     help="instances of the batched-ensemble measurement (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--loop-steps", type=int, default=2)
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="context option (grasph_set_option) for every context the run creates; repeatable")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run device-to-device copies of MIB MiB (the reference kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -149,6 +151,9 @@ def main():
     sph = bool(prm.flags & g.SPH)
 
     ctx = g.Context(0)
+    for kv in args.option:
+        key, val = kv.split("=", 1)
+        ctx.set_option(key, int(val))
     d = g.DeviceParticles.from_host(h)
     g.hip.synchronize()
 

@@ -515,6 +515,14 @@ int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_
             ProfScope ps(&c, fs, KC_FAR);
             if ((e = launch_sym_batch_far(L, fs)) != hipSuccess) return hip_fail(c, e, "batched far-field launch");
         }
+        if (fork) {
+            // the gravity inside the groups needs only the positions: after the
+            // far field on its stream, beside the neighbour passes on s
+            ProfScope ps(&c, fs, KC_SWEEP);
+            if ((e = launch_sym_batch_ingroup(L, fs)) != hipSuccess)
+                return hip_fail(c, e, "batched near-field gravity launch");
+            L.ingroup_done = true;
+        }
         if (fork && (e = hipEventRecord(b.ev_join, b.side)) != hipSuccess)
             return hip_fail(c, e, "batched far-field join event");
     }
@@ -526,7 +534,7 @@ int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_
 int sym_batch_end(Ctx& c, BatchState& b, SymBatchLaunch& L, hipStream_t s, bool forked, bool add_acc)
 {
     hipError_t e;
-    {
+    if (!L.ingroup_done) {
         // the gravity inside the groups last on s: only the combine needs it
         ProfScope ps(&c, s, KC_SWEEP);
         if ((e = launch_sym_batch_ingroup(L, s)) != hipSuccess)

@@ -241,6 +241,7 @@ struct SymBatchLaunch {
     int max_words;
     int* queue;           // one work counter
     int wgs;
+    bool ingroup_done;    // the in-group gravity was enqueued after the far field on its stream
     const float* umass;   // {uniform, mass} per instance
     float4* acc;
     int* nbr;

@@ -126,8 +126,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     CSum rho{0.0f, 0.0f};
     for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return xs[j]; },
                        [&](int, const float4 pj) {
-        const float r = sqrtf(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z));
-        rho.add(pj.w * w_spline(r, P.H, P.Wpref));
+        rho.add(pj.w * kernel_w_f64(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, P));
     });
     float r = rho.value();
     if ((P.flags & GRASPH_DEAL_WITH_NO_PARTNERS) && r < 0.001f) r = P.rho0 * 0.01f;  // :85-88
@@ -150,25 +149,26 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     const float4 vi = rec[t].vr;
     const float rhoi = vi.w;
 
-    float exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;  // strain rate (symmetric)
-    float rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
-    float divv = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    // Sums and terms in FP64 from the FP32 inputs: the velocity divergence
+    // and curl cancel strongly where bodies meet, and the Balsara factor
+    // built from them scales the artificial viscosity of the acceleration
+    // pass, where FP32 rounding here showed up amplified.
+    double exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;  // strain rate (symmetric)
+    double rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
+    double divv = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
 
     for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return PV{rec[j].pm, rec[j].vr}; },
                        [&](int, const PV& q) {
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = dist2_exact(rx, ry, rz);
-        const float r = sqrtf(r2);
-        const float dw = dw_spline(r, P.H, P.dWpref);
-        const float s = dw / r;
-        const float gx = s * rx, gy = s * ry, gz = s * rz;
+        const double s = dw_over_r_f64(rx, ry, rz, P);
+        const double gx = s * (double)rx, gy = s * (double)ry, gz = s * (double)rz;
         const float4 vj = q.vr;
-        const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-        const float mj = pj.w, rhoj = vj.w;
-        const float vol = mj / rhoj;  // one division per pair, shared by the terms below
+        const double vx = (double)vi.x - (double)vj.x, vy = (double)vi.y - (double)vj.y,
+                     vz = (double)vi.z - (double)vj.z;
+        const double vol = (double)pj.w / (double)vj.w;  // one division per pair, shared by the terms below
         if (SOLIDS) {  // models.h:225-248
-            const float tmp = -0.5f * vol;
+            const double tmp = -0.5 * vol;
             exx += tmp * (vx * gx + vx * gx);
             exy += tmp * (vx * gy + vy * gx);
             exz += tmp * (vx * gz + vz * gx);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
             rxz += tmp * (vx * gz - vz * gx);
             ryz += tmp * (vy * gz - vz * gy);
         } else if (BALS) {  // curl v, computeDerivatives.h:134
-            const float f = vol;
+            const double f = vol;
             cx += f * (vy * gz - vz * gy);
             cy += f * (vz * gx - vx * gz);
             cz += f * (vx * gy - vy * gx);
@@ -194,12 +194,12 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
                         dstress[m.plane6(3, i)], dstress[m.plane6(4, i)], dstress[m.plane6(5, i)]);
     float sdt[6] = {0, 0, 0, 0, 0, 0};
     if (SOLIDS) {
-        const Tens ed = sym_to_full(exx, eyy, ezz, exy, exz, eyz);
+        const Tens ed = sym_to_full((float)exx, (float)eyy, (float)ezz, (float)exy, (float)exz, (float)eyz);
         Tens rd;
         rd.m[0][0] = 0.0f; rd.m[1][1] = 0.0f; rd.m[2][2] = 0.0f;
-        rd.m[0][1] = rxy; rd.m[1][0] = -rxy;
-        rd.m[0][2] = rxz; rd.m[2][0] = -rxz;
-        rd.m[1][2] = ryz; rd.m[2][1] = -ryz;
+        rd.m[0][1] = (float)rxy; rd.m[1][0] = -(float)rxy;
+        rd.m[0][2] = (float)rxz; rd.m[2][0] = -(float)rxz;
+        rd.m[1][2] = (float)ryz; rd.m[2][1] = -(float)ryz;
         sdt[0] = dsdt_entry(ed, rd, S, P.shear, 0, 0);
         sdt[1] = dsdt_entry(ed, rd, S, P.shear, 1, 1);
         sdt[2] = dsdt_entry(ed, rd, S, P.shear, 2, 2);
@@ -207,14 +207,14 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
         sdt[4] = dsdt_entry(ed, rd, S, P.shear, 0, 2);
         sdt[5] = dsdt_entry(ed, rd, S, P.shear, 1, 2);
         // curl from the rotation rate, :150
-        cx = -2.0f * rd.m[1][2];
-        cy = -2.0f * rd.m[2][0];
-        cz = -2.0f * rd.m[0][1];
+        cx = -2.0 * ryz;
+        cy = 2.0 * rxz;
+        cz = -2.0 * rxy;
     }
     float bal = 0.0f;
-    if (BALS) bal = balsara_switch(divv, cx, cy, cz, P.c, P.H);
+    if (BALS) bal = balsara_switch_f64(divv, cx, cy, cz, P.c, P.H);
     if (balsara) balsara[i] = bal;
-    if (density_dt) density_dt[i] = INTD ? -rhoi * divv : 0.0f;
+    if (density_dt) density_dt[i] = INTD ? (float)(-(double)rhoi * divv) : 0.0f;
     if (dstress_dt) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) dstress_dt[m.plane6(c, i)] = sdt[c];
@@ -294,9 +294,8 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
         const float r = sqrtf(r2);
-        const float dw = dw_spline(r, P.H, P.dWpref);
-        const float s = dw / r;
-        const float gx = s * rx, gy = s * ry, gz = s * rz;
+        float gx, gy, gz;  // dW/r r_ij: FP64, rounded once (the pair terms cancel where the stresses are large)
+        kernel_gradient_f64(rx, ry, rz, P, gx, gy, gz);
         const float4 qa = q.ra, qb = q.rb;
         const float rhoj = qb.w;
         const float mj = pj.w;

@@ -25,6 +25,9 @@ struct DevParams {
     float cand;  // d2 = r2 + H2 threshold that admits every pair with r2 <= H2
     // variable-step leapfrog, settings/settings.h:56-62
     float ts_init, ts_min, ts_max, acc_acc, vel_acc;
+    // FP64 constants of the passes' kernel and gradient: 1/H, the W and dW
+    // prefactors and the dW prefactor over H (SPH only, H > 0)
+    double invH64, Wpref64, dWpref64, dWprefH64;
 };
 
 // Everything below rounds each product and sum as written, wherever it is
@@ -64,6 +67,43 @@ __host__ __device__ inline float dw_spline(float r, float h, float factor)
     const float q = r / h;
     return factor * ((q <= 1.0f) ? ((q < 0.5f) ? (3.0f * q * q - 2.0f * q) : -1.0f * (1.0f - q) * (1.0f - q))
                                  : 0.0f);
+}
+
+// The kernel gradient g = dW(r)/r * r_ij (src/sph/kernel.h:169-175 over r)
+// of a pair at the FP32 separation (rx, ry, rz), evaluated in FP64 and
+// rounded once per component.  Where a particle's pair terms cancel (the
+// compressed zone of a contact), the half-dozen FP32 roundings of this
+// factor, amplified by the cancellation, dominated the acceleration error
+// against FP64; rounded once they leave a fifth of it.  For q < 1/2 the
+// factor is prefactor (3 q - 2) / H (q / r = 1 / H), for q >= 1/2
+// -prefactor (1 - q)^2 / r; both pieces meet at q = 1/2.
+__device__ inline double dw_over_r_f64(float rx, float ry, float rz, const DevParams& P)
+{
+    const double x = rx, y = ry, z = rz;
+    const double r2 = (x * x + y * y) + z * z;
+    const double ir = rsqrt(r2);
+    const double q = r2 * ir * P.invH64;
+    const double om = 1.0 - q;
+    return (q < 0.5) ? P.dWprefH64 * (3.0 * q - 2.0) : -P.dWpref64 * (om * om) * ir;
+}
+__device__ inline void kernel_gradient_f64(float rx, float ry, float rz, const DevParams& P, float& gx, float& gy,
+                                           float& gz)
+{
+    const double s = dw_over_r_f64(rx, ry, rz, P);
+    gx = (float)(s * (double)rx);
+    gy = (float)(s * (double)ry);
+    gz = (float)(s * (double)rz);
+}
+// W(r) (src/sph/kernel.h:130-136) in FP64 at the FP32 separation, rounded
+// once: the density sum then carries about half an ulp per particle, which
+// the pressures of a strongly compressed zone would otherwise amplify
+__device__ inline float kernel_w_f64(float rx, float ry, float rz, const DevParams& P)
+{
+    const double x = rx, y = ry, z = rz;
+    const double q = sqrt((x * x + y * y) + z * z) * P.invH64;
+    const double om = 1.0 - q;
+    const double w = (q < 0.5) ? (6.0 * q * q * q - 6.0 * q * q + 1.0) : 2.0 * (om * om * om);
+    return (float)(P.Wpref64 * w);
 }
 
 // src/sph/eos.h:88-96
@@ -134,6 +174,14 @@ __device__ inline float balsara_switch(float divv, float cx, float cy, float cz,
     return absdiv / (absdiv + abscurl + 0.0001f * c / h);
 }
 
+// the same from FP64 sums, rounded once
+__device__ inline float balsara_switch_f64(double divv, double cx, double cy, double cz, float c, float h)
+{
+    const double absdiv = fabs(divv);
+    const double abscurl = sqrt(cx * cx + cy * cy + cz * cz);
+    return (float)(absdiv / (absdiv + abscurl + 0.0001 * (double)c / (double)h));
+}
+
 // Symmetric tensor, unique components xx yy zz xy xz yz.
 struct Sym6 {
     float xx, yy, zz, xy, xz, yz;
@@ -178,6 +226,12 @@ __host__ inline DevParams make_dev_params(const grasph_params* p)
     // d2 = fma-chain(r2) + H2 carries at most a few ulp of relative error, so
     // pairs with r2 <= H2 all satisfy d2 <= 2 H2 (1 + 2^-18).
     d.cand = 2.0f * d.H2 * (1.0f + 1.0f / 262144.0f);
+    const double h = p->H;
+    d.invH64 = h > 0.0 ? 1.0 / h : 0.0;
+    d.Wpref64 = d.dim == 1 ? 4.0 / (3.0 * h) : d.dim == 2 ? 40.0 / (7.0 * M_PI * h * h) : 8.0 / (M_PI * h * h * h);
+    d.dWpref64 = d.dim == 1 ? 24.0 / (3.0 * h * h) : d.dim == 2 ? 240.0 / (7.0 * M_PI * h * h * h)
+                                                                : 48.0 / (M_PI * h * h * h * h);
+    d.dWprefH64 = h > 0.0 ? d.dWpref64 / h : 0.0;
     return d;
 }
 

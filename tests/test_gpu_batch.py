@@ -13,7 +13,7 @@ import pytest
 import grasph2_amd as g
 from grasph2_amd import ics
 from oracle import pyoracle as O
-from tests._util import vec_errors
+from tests._util import vec_errors, vec_errors_floored
 
 pytestmark = pytest.mark.gpu
 
@@ -89,7 +89,8 @@ def test_batched_derivatives_match_single_instances(unsplit):
     assert_same(got, ref, FIELDS_D)
     o = O.compute_derivatives(params[3], insts[3], "f64")
     mx, nr = vec_errors(got[3]["acc"], o["acc"])
-    assert nr <= 1e-5 and mx <= 1e-5, (mx, nr)
+    fl = vec_errors_floored(got[3]["acc"], o["acc"])
+    assert nr <= 1e-5 and fl <= 1e-5 and mx <= 1e-4, (mx, fl, nr)
 
 
 def test_graph_steps_match_loop_and_eager(unsplit):
@@ -170,7 +171,8 @@ def test_split_single_instance_agrees_with_batch(gpu_ctx):
     ref = single_results(gpu_ctx, insts, params)
     for a, r in zip(got, ref):
         mx, nr = vec_errors(a["acc"], r["acc"])
-        assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
+        fl = vec_errors_floored(a["acc"], r["acc"])
+        assert nr <= 1e-6 and fl <= 1e-5 and mx <= 1e-4, (mx, fl, nr)
         assert np.array_equal(a["density"], r["density"])  # list passes: same neighbours, same order
 
 
