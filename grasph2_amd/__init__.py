@@ -149,6 +149,13 @@ class Context:
         return self._check(self._lib.grasph_compute_derivatives(self.handle, ctypes.byref(v), ctypes.byref(prm),
                                                                 self._s(stream)))
 
+    def compute_derivatives_noalloc(self, parts, prm, stream=None):
+        """grasph_compute_derivatives_noalloc: the same passes with every work
+        array cut from the library's static per-GPU scratch (no allocation)."""
+        v = parts.view() if isinstance(parts, DeviceParticles) else parts
+        return self._check(self._lib.grasph_compute_derivatives_noalloc(self.handle, ctypes.byref(v),
+                                                                        ctypes.byref(prm), self._s(stream)))
+
     def calc_density(self, parts, prm, stream=None):
         v = parts.view() if isinstance(parts, DeviceParticles) else parts
         return self._check(self._lib.grasph_calc_density(self.handle, ctypes.byref(v), ctypes.byref(prm),

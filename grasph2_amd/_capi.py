@@ -116,6 +116,7 @@ _SIGNATURES = {
     "grasph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "grasph_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "grasph_compute_derivatives": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Params), ctypes.c_void_p]),
+    "grasph_compute_derivatives_noalloc": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Params), ctypes.c_void_p]),
     "grasph_calc_density": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Params), ctypes.c_void_p]),
     "grasph_calc_balsara_density_dt_dstress_dt": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Params), ctypes.c_void_p]),
     "grasph_calc_acceleration": (ctypes.c_int, [ctypes.c_void_p, _P(Particles), _P(Params), ctypes.c_void_p]),

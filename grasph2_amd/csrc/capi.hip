@@ -4,10 +4,17 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 
 #include "internal.hpp"
+
+// Work arrays of grasph_compute_derivatives_noalloc: one copy per GPU, at
+// its own address on each.  Host code never uses this name as an address:
+// scratch_address() resolves it once per device and every kernel argument
+// is cut from that address.
+static __device__ __align__(256) unsigned char scratch[GRASPH_NOALLOC_SCRATCH_BYTES];
 
 namespace grasph {
 
@@ -63,11 +70,43 @@ void choose_slices(const Ctx& c, int64_t n, int ni, int* slices, int* slice_len)
 
 namespace {
 
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;       // per device: the array's address, once resolved
+std::vector<std::string> g_scratch_err;  // per device: why it could not be
+
+}  // namespace
+
+// The scratch array's address on `device`, which must be current: asked of
+// the runtime once per device, then kept.  Null (and err set) on failure.
+void* scratch_address(int device, std::string& err)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= device) {
+        g_scratch_dev.resize(device + 1, nullptr);
+        g_scratch_err.resize(device + 1);
+    }
+    if (!g_scratch_dev[device] && g_scratch_err[device].empty()) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            g_scratch_err[device] = std::string("hipGetSymbolAddress of the scratch array: ") + hipGetErrorString(e);
+            (void)hipGetLastError();
+        } else {
+            g_scratch_dev[device] = ptr;
+        }
+    }
+    err = g_scratch_err[device];
+    return g_scratch_dev[device];
+}
+
+namespace {
+
 int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_eval)
 {
     const int64_t n = p->n;
     if (c.reorder_interval <= 0) return 0;
     if (c.reorder_cap < n || !c.perm) {
+        if (c.no_alloc) return fail(c, GRASPH_E_NOMEM, "the Morton order would need a device allocation");
         void* bufs[] = {c.perm, c.keys, c.keys_alt, c.vals};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
@@ -83,6 +122,7 @@ int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_ev
         }
         c.reorder_cap = n;
     }
+    if (!c.bbox && c.no_alloc) return fail(c, GRASPH_E_NOMEM, "the bounding box would need a device allocation");
     if (!c.bbox && hipMalloc((void**)&c.bbox, 257 * 6 * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(c, GRASPH_E_NOMEM, "device allocation of the bounding box failed");
@@ -92,6 +132,7 @@ int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_ev
     if (stale) {
         const size_t need = reorder_tmp_bytes((int)n);
         if (need > c.sort_tmp_bytes || !c.sort_tmp) {
+            if (c.no_alloc) return fail(c, GRASPH_E_NOMEM, "sort storage would need a device allocation");
             if (c.sort_tmp) (void)hipFree(c.sort_tmp);
             c.sort_tmp = nullptr;
             c.sort_tmp_bytes = 0;
@@ -120,6 +161,7 @@ int ensure_nbr(Ctx& c, int64_t n, int slices)
     const int k = c.max_neighbours;
     const int64_t need_n = n * slices;
     if (c.nbr && c.nbr_cap_n >= need_n && c.nbr_cap_k >= k) return 0;
+    if (c.no_alloc) return fail(c, GRASPH_E_NOMEM, "the neighbour list would need a device allocation");
     if (c.nbr) (void)hipFree(c.nbr);
     if (c.nbr_cnt) (void)hipFree(c.nbr_cnt);
     c.nbr = nullptr; c.nbr_cnt = nullptr; c.nbr_cap_n = 0; c.nbr_cap_k = 0;
@@ -251,6 +293,8 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
     hipError_t e;
     ws = s;
     if (sym_has_far(S)) {
+        if (c.no_alloc && (!c.side || !c.ev_fork || !c.ev_join))
+            return fail(c, GRASPH_E_NOMEM, "the second stream would need creating");
         if (!c.side && (e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking)) != hipSuccess)
             return hip_fail(c, e, "second stream");
         if ((!c.ev_fork && (e = hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming)) != hipSuccess) ||
@@ -350,10 +394,8 @@ int check_integrate_arrays(Ctx& c, const grasph_particles* p, uint32_t f)
 
 namespace {
 
-int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream, int which)
+int derivatives(Ctx& c, const grasph_particles* p, const grasph_params* prm, void* stream, int which)
 {
-    if (!ctx) return GRASPH_E_INVALID;
-    Ctx& c = ctx->c;
     int r;
     if ((r = check_params(c, prm)) || (r = check_particles(c, p))) return r;
     if (p->n == 0) return 0;
@@ -493,7 +535,112 @@ int derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params*
     return 0;
 }
 
+// Consecutive 256-byte-aligned pieces of one region; with base null it
+// only measures (offsets from 0, no limit).
+struct Carve {
+    char* base;
+    size_t cap;
+    size_t used = 0;
+    template <typename T>
+    T* take(int64_t count)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + (size_t)count * sizeof(T);
+        return base && used <= cap ? reinterpret_cast<T*>(base + off) : nullptr;
+    }
+};
+
+// The work arrays computeDerivatives needs on the one-sided sweep with
+// `slices` j-slices, re-sorted every call: cut from `k` into `sh` (its
+// capacities set to exactly what the path asks of grow / ensure_*).
+void carve_derivative_arrays(Ctx& sh, Carve& k, int64_t n, bool lists, bool grav, int slices)
+{
+    if (lists) {
+        sh.perm = k.take<int>(n);
+        sh.keys = k.take<uint32_t>(n);
+        sh.keys_alt = k.take<uint32_t>(n);
+        sh.vals = k.take<int>(n);
+        sh.reorder_cap = n;
+        sh.sort_tmp_bytes = reorder_tmp_bytes((int)n);
+        sh.sort_tmp = k.take<unsigned char>((int64_t)std::max<size_t>(sh.sort_tmp_bytes, 1));
+        sh.bbox = k.take<float>(257 * 6);
+        sh.xs = k.take<float4>(n);
+        sh.xs_cap = n;
+        sh.rec = k.take<SlotRec>(n);
+        sh.rec_cap = n;
+        if (grav) {
+            sh.mass_flags_cap = (n + 255) / 256;
+            sh.mass_flags = k.take<int>(sh.mass_flags_cap);
+        }
+        sh.nbr_cap_n = n * slices;
+        sh.nbr_cap_k = sh.max_neighbours;
+        sh.nbr = k.take<int>(sh.nbr_cap_n * sh.max_neighbours);
+        sh.nbr_cnt = k.take<int>(sh.nbr_cap_n);
+    }
+    if (slices > 1 && grav) {
+        sh.part_cap = 2 * n * slices;
+        sh.part = k.take<float4>(sh.part_cap);
+    }
+}
+
 }  // namespace
+
+int derivatives_noalloc(Ctx& c, const grasph_particles* p, const grasph_params* prm, void* stream)
+{
+    int r;
+    if ((r = check_params(c, prm)) || (r = check_particles(c, p))) return r;
+    if (!c.scratch)
+        return fail(c, GRASPH_E_HIP, "no allocation-free scratch on device " + std::to_string(c.device) + ": " +
+                                         c.scratch_err);
+    if (p->n == 0) return 0;
+    const int64_t n = p->n;
+    const uint32_t f = prm->flags;
+    const bool lists = f & GRASPH_SPH, grav = f & GRASPH_SELF_GRAVITY;
+
+    // a stand-in context: this context's options and state ({current, next}
+    // dt, the mass record), the one-sided sweep on one stream, a fresh
+    // Morton order, buffers from the scratch array, no allocation allowed
+    Ctx sh;
+    sh.device = c.device;
+    sh.cus = c.cus;
+    sh.max_neighbours = c.max_neighbours;
+    sh.sweep_ni = c.sweep_ni;
+    sh.sweep_symmetric = 0;
+    sh.reorder_interval = 1;
+    sh.profile = false;
+    sh.no_alloc = true;
+    sh.ts = c.ts;
+    sh.umass = c.umass;
+
+    // the j-split: as many slices as the allocating path would take, fewer
+    // until the arrays fit
+    sh.sweep_slices = c.sweep_slices;
+    int slices = 1, len = (int)n;
+    if (grav || lists) choose_slices(sh, n, sh.sweep_ni, &slices, &len);
+    size_t need = 0;
+    for (;;) {
+        Carve m{nullptr, 0};
+        carve_derivative_arrays(sh, m, n, lists, grav, slices);
+        need = m.used;
+        if (need <= (size_t)GRASPH_NOALLOC_SCRATCH_BYTES || slices == 1) break;
+        sh.sweep_slices = slices - 1;
+        choose_slices(sh, n, sh.sweep_ni, &slices, &len);
+    }
+    if (need > (size_t)GRASPH_NOALLOC_SCRATCH_BYTES)
+        return fail(c, GRASPH_E_INVALID,
+                    "allocation-free derivatives of " + std::to_string(n) + " particles need " +
+                        std::to_string(need) + " bytes of scratch with these flags and max_neighbours = " +
+                        std::to_string(c.max_neighbours) + "; the scratch array holds " +
+                        std::to_string((long long)GRASPH_NOALLOC_SCRATCH_BYTES));
+    sh.sweep_slices = slices;
+    Carve k{static_cast<char*>(c.scratch), (size_t)GRASPH_NOALLOC_SCRATCH_BYTES};
+    carve_derivative_arrays(sh, k, n, lists, grav, slices);
+    r = derivatives(sh, p, prm, stream, 0);
+    if (r) c.err = sh.err;
+    c.noalloc_slices = slices;
+    return r;
+}
+
 }  // namespace grasph
 
 using grasph::Ctx;
@@ -592,6 +739,9 @@ int grasph_create(int device, grasph_ctx** out)
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->c.cus = cus;
+    // the allocation-free entry point's array on this device (a failure only
+    // disables that entry point, which then reports it)
+    ctx->c.scratch = grasph::scratch_address(device, ctx->c.scratch_err);
     (void)hipGetLastError();
     ctx->c.err = "no error";
     *out = ctx;
@@ -694,26 +844,32 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value)
     else if (k == "batch_waves_per_block") *value = c.batch_wpb;
     else if (k == "batch_blocks_per_cu") *value = c.batch_bpc;
     else if (k == "profile") *value = c.profile ? 1 : 0;
+    else if (k == "noalloc_slices") *value = c.noalloc_slices;
     else return GRASPH_E_INVALID;
     return 0;
 }
 
 int grasph_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
 {
-    return grasph::derivatives(ctx, p, prm, stream, 0);
+    return ctx ? grasph::derivatives(ctx->c, p, prm, stream, 0) : GRASPH_E_INVALID;
+}
+int grasph_compute_derivatives_noalloc(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm,
+                                       void* stream)
+{
+    return ctx ? grasph::derivatives_noalloc(ctx->c, p, prm, stream) : GRASPH_E_INVALID;
 }
 int grasph_calc_density(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
 {
-    return grasph::derivatives(ctx, p, prm, stream, 1);
+    return ctx ? grasph::derivatives(ctx->c, p, prm, stream, 1) : GRASPH_E_INVALID;
 }
 int grasph_calc_balsara_density_dt_dstress_dt(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm,
                                               void* stream)
 {
-    return grasph::derivatives(ctx, p, prm, stream, 2);
+    return ctx ? grasph::derivatives(ctx->c, p, prm, stream, 2) : GRASPH_E_INVALID;
 }
 int grasph_calc_acceleration(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, void* stream)
 {
-    return grasph::derivatives(ctx, p, prm, stream, 3);
+    return ctx ? grasph::derivatives(ctx->c, p, prm, stream, 3) : GRASPH_E_INVALID;
 }
 
 int grasph_integrate(grasph_ctx* ctx, const grasph_particles* p, const grasph_params* prm, int not_first_step,

@@ -117,6 +117,17 @@ struct Ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     struct BatchState* batch = nullptr;
+
+    // The allocation-free entry point runs the derivative path on a
+    // stand-in context whose buffers are cut from the device's static
+    // scratch array; there no_alloc turns every allocation the path would
+    // make into an error.  scratch: that array's device address on this
+    // context's device (null, with the reason in scratch_err, when the
+    // runtime could not resolve it).
+    bool no_alloc = false;
+    void* scratch = nullptr;
+    std::string scratch_err;
+    int noalloc_slices = 0;  // j-slices of the last allocation-free call
 };
 
 // Bracket a kernel launch with events when profiling is on.
@@ -480,6 +491,7 @@ template <typename T>
 int grow(Ctx& c, T*& ptr, int64_t& cap, int64_t need, const char* what)
 {
     if (need <= cap && ptr) return 0;
+    if (c.no_alloc) return fail(c, GRASPH_E_NOMEM, std::string(what) + " would need a device allocation");
     if (ptr) {
         (void)hipFree(ptr);
         ptr = nullptr;

@@ -40,6 +40,7 @@ def _lib():
         h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
         h.hipSetDevice.argtypes = [ctypes.c_int]
+        h.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     return _HIP
 
 
@@ -60,6 +61,13 @@ def set_device(d):
 
 def synchronize():
     check(_lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def mem_info():
+    """(free, total) bytes of device memory on the current device."""
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    check(_lib().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)), "hipMemGetInfo")
+    return free.value, total.value
 
 
 class DeviceArray:

@@ -170,7 +170,9 @@ int grasph_destroy(grasph_ctx* ctx);
  * path's bit for bit when that path is not j-split). */
 int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value);
 /* The current value of an option of grasph_set_option (the autotuner sets
- * some of them); GRASPH_E_INVALID for an unknown key. */
+ * some of them), or the read-only "noalloc_slices" (the j-split the last
+ * grasph_compute_derivatives_noalloc call on this context took);
+ * GRASPH_E_INVALID for an unknown key. */
 int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value);
 
 /* ---- the hot path ------------------------------------------------------- */
@@ -185,6 +187,29 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value);
  * calcAcceleration::do_after sets nextTS (:305-308). */
 int grasph_compute_derivatives(grasph_ctx* ctx, const grasph_particles* p,
                                const grasph_params* prm, void* stream);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same passes and results as
+ * grasph_compute_derivatives with options sweep_symmetric = 0 and
+ * reorder_interval = 1 (and sweep_slices as chosen below), computed
+ * without any device allocation: every work array (Morton order and sort
+ * storage, slot-ordered positions and records, neighbour lists, the
+ * j-split's partial sums) is cut from one static device array of
+ * GRASPH_NOALLOC_SCRATCH_BYTES per GPU, held from library load until the
+ * process ends.  Runs on `stream` alone (no second stream), re-sorts the
+ * Morton order on every call and records no timing events.  The j-split
+ * ("sweep_slices", automatic when 0) is reduced until the arrays fit; when
+ * even one slice does not fit, returns GRASPH_E_INVALID and says how many
+ * bytes the call needs.  With SPH and the default 160 neighbours per list a
+ * particle needs ~760 bytes (about 44000 particles); gravity alone needs
+ * none at one slice.  Every call on one GPU shares that GPU's array: calls
+ * that overlap in time on one GPU (several host threads, or streams that do
+ * not order them) race and give wrong results.  Nothing in the array needs
+ * to start at zero: each call writes every element before reading it. */
+#define GRASPH_NOALLOC_SCRATCH_BYTES (32LL * 1024 * 1024)
+int grasph_compute_derivatives_noalloc(grasph_ctx* ctx, const grasph_particles* p,
+                                       const grasph_params* prm, void* stream);
 
 /* The three passes individually, each standalone (each runs its own sweep):
  *   do_for_each_pair_fast<calcDensity>                   computeDerivatives.h:52-91
