@@ -191,7 +191,7 @@ def main():
     pairs = float(n) * (n - 1)
     value = pairs / (ms_step * 1e-3)
 
-    sym = n >= (SYM_AUTO_MIN_LISTS if sph else SYM_AUTO_MIN_GRAVITY)
+    sym = kt["far"][1] > 0  # the symmetric sweep ran (its far field was timed)
     # all kernels of the all-pairs sweep, summed per evaluation (with the symmetric sweep, intervals that overlap)
     sweep_ms = (kt["sweep"][0] + kt["far"][0]) / args.steps
     sweep_flops = FLOP_GRAVITY_PAIR * pairs

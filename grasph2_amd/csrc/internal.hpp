@@ -171,7 +171,8 @@ hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s)
 // arrays, in elements.
 struct SymGeom {
     int nb, groups, wgs, nwords;
-    int64_t nsb, units;   // super-blocks of 256 slots; far work units
+    int64_t nsb, units;   // super-blocks; far work units
+    int split;            // pieces per partner super-block in a far unit (1, 2 or 4)
     int64_t box_elems;    // float4: bounding box per block of 64
     int64_t bits_elems;   // uint32: near-block bit matrix (lists wanted)
     int64_t near_elems;   // float4: near-field partial per slot and share

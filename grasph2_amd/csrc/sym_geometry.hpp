@@ -21,28 +21,53 @@ constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefron
 constexpr int kGroup = 1024;            // slots per group
 constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)
 
-// far units (partner super-blocks) of row g: super-blocks [kGroupSB (g + 1), nsb)
-__device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb)
+// far units of row g: each partner super-block of [kGroupSB (g + 1), nsb)
+// cut into Q pieces of kSB / Q partners (Q = 1, 2 or 4; more pieces give
+// small particle counts enough units to fill the GPU)
+__device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb, int Q = 1)
 {
     const int64_t u = nsb - (int64_t)kGroupSB * (g + 1);
-    return u > 0 ? u : 0;
+    return u > 0 ? u * Q : 0;
 }
-// units before row g (g <= rows_with_units): sum over g' < g of (nsb - kGroupSB (g' + 1))
-__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb)
+// units before row g (g <= rows_with_units): sum over g' < g of row_units(g')
+__device__ __host__ inline int64_t row_prefix(int64_t g, int64_t nsb, int Q = 1)
 {
-    return g * nsb - (int64_t)kGroupSB * (g * (g + 1) / 2);
+    return Q * (g * nsb - (int64_t)kGroupSB * (g * (g + 1) / 2));
 }
 // rows with far work: nsb - kGroupSB (g + 1) >= 1
 __device__ __host__ inline int64_t rows_with_units(int64_t nsb) { return nsb >= 1 ? (nsb - 1) / kGroupSB : 0; }
-// the row holding unit u (0 <= u < row_prefix(rows_with_units(nsb), nsb)): the largest g with row_prefix(g) <= u
-__device__ __host__ inline int64_t row_of_unit(int64_t u, int64_t nsb)
+// the row holding unit u (0 <= u < row_prefix(rows_with_units(nsb), nsb, Q)): the largest g with row_prefix(g) <= u
+__device__ __host__ inline int64_t row_of_unit(int64_t u, int64_t nsb, int Q = 1)
 {
     int64_t lo = 0, hi = rows_with_units(nsb) - 1;
     while (lo < hi) {
         const int64_t mid = (lo + hi + 1) / 2;
-        if (row_prefix(mid, nsb) <= u) lo = mid; else hi = mid - 1;
+        if (row_prefix(mid, nsb, Q) <= u) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+// pieces per partner super-block for n particles and `wanted` far
+// workgroups (as many as run at once): the kernel lasts as long as its
+// fullest share, ceil(U / W) units of kSB / Q partners with W = min(wanted,
+// U); the Q of 1, 2, 4 that makes that shortest, the fewest pieces on a tie
+// (smaller units cost more per partner)
+__device__ __host__ inline int far_split(int64_t n, int64_t wanted)
+{
+    const int64_t nsb = (n + kSB - 1) / kSB;
+    const int64_t u1 = row_prefix(rows_with_units(nsb), nsb, 1);
+    if (u1 <= 0 || wanted <= 0) return 1;
+    int best = 1;
+    int64_t best_num = 0, best_den = 1;  // makespan as the fraction num / den super-blocks
+    for (int Q = 1; Q <= 4; Q *= 2) {
+        const int64_t U = u1 * Q, W = wanted < U ? wanted : U;
+        const int64_t num = (U + W - 1) / W;
+        if (Q == 1 || num * best_den < best_num * Q) {
+            best = Q;
+            best_num = num;
+            best_den = Q;
+        }
+    }
+    return best;
 }
 // the workgroup whose share [k U / W, (k + 1) U / W) holds unit u
 __device__ __host__ inline int64_t wg_of_unit(int64_t u, int64_t U, int64_t W) { return ((u + 1) * W - 1) / U; }

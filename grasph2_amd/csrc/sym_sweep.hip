@@ -392,11 +392,12 @@ __device__ __forceinline__ int stage_index_rot(const int lane)
 // One partner super-block [b0, b0 + kSB) against the wavefront's slots;
 // the reduced partner side of partner p, component c goes to my_stage[3 p + c].
 template <bool UM, bool SOFT, bool BOUNDED>
-__device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, const int n, const int b0, FarOwn& o,
-                                                const float H2, const int lane, const int stg, float* my_stage)
+__device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, const int n, const int b0,
+                                                const int nchunks, FarOwn& o, const float H2, const int lane,
+                                                const int stg, float* my_stage)
 {
     float4 A = load_partner_row<UM>(xs, b0, n, lane);
-    for (int q = 0; q < kSB / kRow; ++q) {
+    for (int q = 0; q < nchunks; ++q) {
         const float4 B = load_partner_row<UM>(xs, b0 + kRow * (q + 1), n, lane);  // unused after the last
         my_stage[3 * kRow * q + stg] =
             far_chunk_rot<UM, SOFT>(bound_partner_row<BOUNDED>(A, b0 + kRow * q, n, lane), o, H2);
@@ -440,23 +441,25 @@ __device__ __forceinline__ void far_flush(float4* __restrict__ apart, const size
     }
 }
 
-// One far unit of row g: partner super-block J against the workgroup's
-// 1024 slots.  The own side is folded into the compensated sums; the
-// partner side, summed over the wavefronts in wavefront order, is stored
-// as row g's partial of each partner (bpart: the instance's partials).
-// Called by the whole workgroup (two barriers).
-template <bool UM, bool SOFT>
-__device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t nsb,
-                                         const int64_t g, const int J, FarOwn& o, const bool wave_on,
+// One far unit of row g: piece `piece` of Q (kSB / Q partners) of partner
+// super-block J against the workgroup's 1024 slots.  The own side is folded
+// into the compensated sums; the partner side, summed over the wavefronts in
+// wavefront order, is stored as row g's partial of each partner (bpart: the
+// instance's partials).  Called by the whole workgroup (two barriers).
+template <bool UM, bool SOFT, int QC = 0>
+__device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t g, const int J,
+                                         const int piece, const int Qr, FarOwn& o, const bool wave_on,
                                          float* __restrict__ bpart, const float H2, const float m, const float* stage,
                                          float* my_stage, const int lane, const int tid, const int stg)
 {
+    const int Q = QC ? QC : Qr;  // QC: the split known at compile time (1 for the largest counts)
+    const int len = kSB / Q, b0 = kSB * J + len * piece, nchunks = len / kRow;
     if (!wave_on) {
-        for (int q = 0; q < kSB / kRow; ++q) my_stage[3 * kRow * q + stg] = 0.0f;
-    } else if ((n % kSB) != 0 && J == nsb - 1) {
-        far_segment_rot<UM, SOFT, true>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+        for (int q = 0; q < nchunks; ++q) my_stage[3 * kRow * q + stg] = 0.0f;
+    } else if (b0 + len > n) {
+        far_segment_rot<UM, SOFT, true>(xs, n, b0, nchunks, o, H2, lane, stg, my_stage);
     } else {
-        far_segment_rot<UM, SOFT, false>(xs, n, kSB * J, o, H2, lane, stg, my_stage);
+        far_segment_rot<UM, SOFT, false>(xs, n, b0, nchunks, o, H2, lane, stg, my_stage);
     }
     // own side: plain sums over the segment, then compensated
 #pragma unroll
@@ -474,14 +477,13 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
     }
     __syncthreads();
     // partner side: the wavefronts' values in wavefront order
-#pragma unroll
-    for (int r = 0; r < kStageWave / kFarThreads; ++r) {
+    for (int r = 0; r < 3 * len / kFarThreads; ++r) {
         const int idx = tid + kFarThreads * r;  // 3 partner + component
         float v = 0.0f;
 #pragma unroll
         for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kStageWave + idx];
         const int b = idx / 3, comp = idx - 3 * b;
-        const int B = kSB * J + b;
+        const int B = b0 + b;
         if (B < n) {
             const int64_t h = B / kGroup;
             const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
@@ -493,33 +495,35 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
 
 // Far field of one instance.  Workgroup k of W takes the units
 // [k U / W, (k + 1) U / W) of the rows g (group g against the later
-// super-blocks) laid end to end; a row's own-side sums leave as fragment
-// g + k.
-template <bool UM, bool SOFT>
+// super-blocks, each cut into Q pieces) laid end to end; a row's own-side
+// sums leave as fragment g + k.
+template <bool UM, bool SOFT, int QC>
 __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
-                                         const int64_t U, const int W, float4* __restrict__ apart,
+                                         const int64_t U, const int W, const int Qr, float4* __restrict__ apart,
                                          float* __restrict__ bpart, const float H2, const float m, float* stage,
                                          float* sums)
 {
+    const int Q = QC ? QC : Qr;
     const int k = blockIdx.x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
     const int64_t u0 = (int64_t)k * U / W, u1 = (int64_t)(k + 1) * U / W;
     if (u0 >= u1) return;  // the whole workgroup leaves: no barrier is reached
-    int64_t g = row_of_unit(u0, nsb), s = u0 - row_prefix(g, nsb);
+    int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
     const int stg = stage_index_rot(lane);
     float* const my_stage = stage + (size_t)w * kStageWave;
     FarOwn o;
     o.sums = sums + (size_t)w * kFarSums;
     bool wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
     for (int64_t u = u0; u < u1; ++u) {
-        if (s == row_units(g, nsb)) {
+        if (s == row_units(g, nsb, Q)) {
             far_flush(apart, (size_t)(g + k), w, lane, o);
             ++g;
             s = 0;
             wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
         }
-        const int J = (int)(kGroupSB * (g + 1) + s);  // partner super-block of this unit
-        far_unit<UM, SOFT>(xs, n, nsb, g, J, o, wave_on, bpart, H2, m, stage, my_stage, lane, tid, stg);
+        const int J = (int)(kGroupSB * (g + 1) + s / Q);  // partner super-block of this unit
+        far_unit<UM, SOFT, QC>(xs, n, g, J, (int)(s % Q), Q, o, wave_on, bpart, H2, m, stage, my_stage, lane, tid,
+                               stg);
         ++s;
     }
     far_flush(apart, (size_t)(g + k), w, lane, o);
@@ -528,15 +532,24 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
 template <bool SOFT>
 __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __restrict__ xs, const int n,
                                                             const int64_t nsb, const int64_t U, const int W,
-                                                            float4* __restrict__ apart, float* __restrict__ bpart,
-                                                            const float H2, const float* __restrict__ umass)
+                                                            const int Q, float4* __restrict__ apart,
+                                                            float* __restrict__ bpart, const float H2,
+                                                            const float* __restrict__ umass)
 {
     __shared__ float stage[kGroupSB * kStageWave];
     __shared__ float sums[kGroupSB * kFarSums];
-    if (umass[0] != 0.0f)
-        far_work<true, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, umass[1], stage, sums);
-    else
-        far_work<false, SOFT>(xs, n, nsb, U, W, apart, bpart, H2, 1.0f, stage, sums);
+    const bool um = umass[0] != 0.0f;
+    if (Q == 1) {
+        if (um)
+            far_work<true, SOFT, 1>(xs, n, nsb, U, W, 1, apart, bpart, H2, umass[1], stage, sums);
+        else
+            far_work<false, SOFT, 1>(xs, n, nsb, U, W, 1, apart, bpart, H2, 1.0f, stage, sums);
+    } else {
+        if (um)
+            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, umass[1], stage, sums);
+        else
+            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, 1.0f, stage, sums);
+    }
 }
 
 // Per slot, in a fixed order: the near partials (share order), the far own-side fragments
@@ -545,7 +558,7 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
 // acceleration is finished as the one-range sweep does.
 __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
     const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
-    const int n, const int nshares, const int64_t nsb, const int64_t U, const int W, const int grav,
+    const int n, const int nshares, const int64_t nsb, const int64_t U, const int W, const int Q, const int grav,
     const int* __restrict__ perm,
     const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const int add_acc,
     const float4* __restrict__ vel, float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams P)
@@ -562,10 +575,27 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
         }
         const int64_t g = t / kGroup, local = t - g * kGroup;
         if (U > 0 && g < rows_with_units(nsb)) {
-            const int64_t k0 = wg_of_unit(row_prefix(g, nsb), U, W);
-            const int64_t k1 = wg_of_unit(row_prefix(g + 1, nsb) - 1, U, W);
-            for (int64_t k = k0; k <= k1; ++k) {
-                if (k * U / W == (k + 1) * U / W) continue;  // no share (never with W <= U)
+            // one fragment per workgroup that touched the row (every workgroup
+            // has a share: W <= U); eight fragments' loads in flight before
+            // their ordered adds (rows of small counts span many workgroups)
+            const int64_t k0 = wg_of_unit(row_prefix(g, nsb, Q), U, W);
+            const int64_t k1 = wg_of_unit(row_prefix(g + 1, nsb, Q) - 1, U, W);
+            int64_t k = k0;
+            for (; k + 8 <= k1 + 1; k += 8) {
+                float4 f[8], fc[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const size_t e = 2 * ((size_t)(g + k + u) * kGroup + local);
+                    f[u] = apart[e];
+                    fc[u] = apart[e + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    x.add(f[u].x); y.add(f[u].y); z.add(f[u].z);
+                    x.c += fc[u].x; y.c += fc[u].y; z.c += fc[u].z;
+                }
+            }
+            for (; k <= k1; ++k) {
                 const size_t e = 2 * ((size_t)(g + k) * kGroup + local);
                 const float4 f = apart[e], fc = apart[e + 1];
                 x.add(f.x); y.add(f.y); z.add(f.z);
@@ -734,10 +764,11 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
         for (int s = 0; s < R.units; ++s) {
             const int J = kGroupSB * (R.g + 1) + s;
             if (um)
-                far_unit<true, SOFT>(xk, S.n, S.nsb, R.g, J, o, wave_on, bk, H2, m, stage, my_stage, lane, tid, stg);
+                far_unit<true, SOFT, 1>(xk, S.n, R.g, J, 0, 1, o, wave_on, bk, H2, m, stage, my_stage, lane, tid,
+                                        stg);
             else
-                far_unit<false, SOFT>(xk, S.n, S.nsb, R.g, J, o, wave_on, bk, H2, 1.0f, stage, my_stage, lane, tid,
-                                      stg);
+                far_unit<false, SOFT, 1>(xk, S.n, R.g, J, 0, 1, o, wave_on, bk, H2, 1.0f, stage, my_stage, lane,
+                                         tid, stg);
         }
         far_flush(apart, (size_t)(S.row0 + R.g), w, lane, o);
     }
@@ -835,7 +866,8 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
     G.nb = (int)((n + 63) / 64);
     G.nsb = (n + kSB - 1) / kSB;
     G.groups = (int)((G.nsb + kGroupSB - 1) / kGroupSB);
-    G.units = row_prefix(rows_with_units(G.nsb), G.nsb);
+    G.split = far_split(n, wgs);
+    G.units = row_prefix(rows_with_units(G.nsb), G.nsb, G.split);
     G.wgs = far_workgroups(wgs, G.units);
     G.nwords = (G.nb + 31) / 32;
     G.box_elems = 2 * (int64_t)G.nb;
@@ -883,10 +915,10 @@ hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s)
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
     if (P.H2 > 0.0f)
         hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
-                           L.apart, L.bpart, P.H2, L.umass);
+                           G.split, L.apart, L.bpart, P.H2, L.umass);
     else
         hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
-                           L.apart, L.bpart, P.H2, L.umass);
+                           G.split, L.apart, L.bpart, P.H2, L.umass);
     return hipGetLastError();
 }
 
@@ -896,7 +928,7 @@ hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_
     if (L.n <= 0 || !(L.grav || L.finish)) return hipSuccess;
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
     hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part, L.apart,
-                       L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
+                       L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, G.split, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
                        L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
     return hipGetLastError();
 }

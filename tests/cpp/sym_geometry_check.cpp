@@ -1,9 +1,9 @@
 // Host check of the symmetric sweep's work geometry
 // (grasph2_amd/csrc/sym_geometry.hpp), no GPU: replays on the host how the
 // far kernel's workgroups walk their shares of the rows (as far_work in
-// sym_sweep.hip does) and how the combine reads the partials back
-// (sym_combine_kernel), for many particle counts and workgroup counts, and
-// checks that
+// sym_sweep.hip does, with partner super-blocks cut into Q = 1, 2 or 4
+// pieces) and how the combine reads the partials back (sym_combine_kernel),
+// for many particle counts and workgroup counts, and checks that
 //  - every pair of super-blocks in different groups is computed exactly once
 //    and no other pair is;
 //  - every own-side fragment and partner-side partial index is in bounds of
@@ -28,45 +28,46 @@ static int fail(const char* what, int64_t n, int64_t W, int64_t a = -1, int64_t 
     return 1;
 }
 
-static int check(int64_t n, int64_t wanted)
+static int check(int64_t n, int64_t wanted, int Q)
 {
     const int64_t nsb = (n + kSB - 1) / kSB;
     const int64_t G = (nsb + kGroupSB - 1) / kGroupSB;
     const int64_t npos = rows_with_units(nsb);
-    const int64_t U = row_prefix(npos, nsb);
+    const int64_t U = row_prefix(npos, nsb, Q);
     int64_t sum = 0;
-    for (int64_t g = 0; g < G; ++g) sum += row_units(g, nsb);
+    for (int64_t g = 0; g < G; ++g) sum += row_units(g, nsb, Q);
     if (sum != U) return fail("unit total", n, wanted, sum, U);
     if (U == 0) return 0;  // no far field: the near field covers the one group
     const int64_t W = far_workgroups(wanted, U);
     const int64_t apart_elems = 2 * (int64_t)kGroup * (G + wanted);
     const int64_t bpart_elems = 3 * bpart_group_base(G);
 
-    std::vector<char> seen((size_t)(nsb * nsb), 0);
+    std::vector<char> seen((size_t)(nsb * nsb * Q), 0);
     std::vector<char> bw((size_t)(bpart_elems / 3), 0);
     std::set<std::pair<int64_t, int64_t>> frag;  // (row, workgroup)
     for (int64_t k = 0; k < W; ++k) {
         const int64_t u0 = k * U / W, u1 = (k + 1) * U / W;
         if (u0 >= u1) return fail("empty share", n, W, k);
-        int64_t g = row_of_unit(u0, nsb), s = u0 - row_prefix(g, nsb);
-        if (s < 0 || s >= row_units(g, nsb)) return fail("row search", n, W, u0, g);
+        int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
+        if (s < 0 || s >= row_units(g, nsb, Q)) return fail("row search", n, W, u0, g);
         for (int64_t u = u0; u < u1; ++u) {
-            if (s == row_units(g, nsb)) {
+            if (s == row_units(g, nsb, Q)) {
                 frag.insert({g, k});
                 ++g;
                 s = 0;
                 if (g >= npos) return fail("row past the last", n, W, g);
             }
-            const int64_t J = kGroupSB * (g + 1) + s;
+            const int64_t J = kGroupSB * (g + 1) + s / Q, piece = s % Q;
             if (J >= nsb) return fail("partner super-block", n, W, g, J);
             for (int w = 0; w < kGroupSB; ++w) {
                 const int64_t sbA = g * kGroupSB + w;
                 if (sbA >= nsb) continue;
-                char& c = seen[(size_t)(sbA * nsb + J)];
+                char& c = seen[(size_t)((sbA * nsb + J) * Q + piece)];
                 if (c) return fail("pair twice", n, W, sbA, J);
                 c = 1;
             }
-            for (int64_t B = kSB * J; B < kSB * (J + 1) && B < n; ++B) {
+            const int64_t len = kSB / Q, b0 = kSB * J + len * piece;
+            for (int64_t B = b0; B < b0 + len && B < n; ++B) {
                 const int64_t h = B / kGroup;
                 const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
                 if (x < 0 || 3 * x + 2 >= bpart_elems) return fail("partial bounds", n, W, g, B);
@@ -79,14 +80,17 @@ static int check(int64_t n, int64_t wanted)
     }
     for (int64_t I = 0; I < nsb; ++I)
         for (int64_t J = 0; J < nsb; ++J)
-            if ((I / kGroupSB < J / kGroupSB) != (seen[(size_t)(I * nsb + J)] != 0)) return fail("coverage", n, W, I, J);
+            for (int p = 0; p < Q; ++p)
+                if ((I / kGroupSB < J / kGroupSB) != (seen[(size_t)((I * nsb + J) * Q + p)] != 0))
+                    return fail("coverage", n, W, I, J);
     for (const auto& f : frag)
         if (2 * ((f.first + f.second + 1) * (int64_t)kGroup) > apart_elems) return fail("fragment bounds", n, W, f.first, f.second);
     // what the combine reads
     size_t reads = 0;
     for (int64_t g = 0; g < G; ++g) {
         if (g < npos) {
-            const int64_t k0 = wg_of_unit(row_prefix(g, nsb), U, W), k1 = wg_of_unit(row_prefix(g + 1, nsb) - 1, U, W);
+            const int64_t k0 = wg_of_unit(row_prefix(g, nsb, Q), U, W);
+            const int64_t k1 = wg_of_unit(row_prefix(g + 1, nsb, Q) - 1, U, W);
             for (int64_t k = k0; k <= k1; ++k) {
                 if (k * U / W == (k + 1) * U / W) continue;
                 if (!frag.count({g, k})) return fail("fragment read, not written", n, W, g, k);
@@ -128,16 +132,26 @@ int main()
     const int64_t ws[] = {1, 2, 3, 7, 12, 64, 512, 1024, 100000};
     int cases = 0;
     for (int64_t n : ns)
-        for (int64_t w : ws) {
-            if (check(n, w)) return 1;
-            ++cases;
-        }
+        for (int64_t w : ws)
+            for (int Q : {1, 2, 4}) {
+                if (check(n, w, Q)) return 1;
+                ++cases;
+            }
     srand(7);
     for (int i = 0; i < 60; ++i) {
         const int64_t n = 1 + rand() % 150000, w = 1 + rand() % 2048;
-        if (check(n, w)) return 1;
+        if (check(n, w, far_split(n, w))) return 1;
         ++cases;
     }
+    // the split chosen for a count is the one its workgroup count gives back
+    for (int64_t n : ns)
+        for (int64_t w : ws) {
+            const int Q = far_split(n, w);
+            const int64_t nsb = (n + kSB - 1) / kSB;
+            const int64_t U = row_prefix(rows_with_units(nsb), nsb, Q);
+            if (U > 0 && far_split(n, far_workgroups(w, U)) != Q) return fail("split not stable", n, w, Q);
+            ++cases;
+        }
     std::vector<int64_t> sizes;
     for (int i = 0; i < 200; ++i) sizes.push_back(rand() % 17000);
     if (check_batch(sizes)) return 1;

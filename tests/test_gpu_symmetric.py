@@ -25,7 +25,10 @@ def sym_ctx():
     ctx.close()
 
 
-@pytest.mark.parametrize("n", [1, 255, 1000, 5000, 20000, 20037])
+# far units of 128 partners (each partner super-block cut into 4) for 5000
+# and 25000, of 256 for 20000 and 20037; 5000, 20037 and 25000 end in a
+# ragged piece
+@pytest.mark.parametrize("n", [1, 255, 1000, 5000, 20000, 20037, 25000])
 def test_gravity_matches_fp64(sym_ctx, n):
     print("symmetric gravity n=%d" % n, flush=True)
     h, prm = ics.plummer(n, seed=n)
