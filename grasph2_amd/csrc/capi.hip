@@ -155,6 +155,62 @@ int ensure_order(Ctx& c, const grasph_particles* p, hipStream_t s, bool count_ev
 
 const int* order_of(Ctx& c) { return c.reorder_interval > 0 ? c.perm : nullptr; }
 
+// The list capacity an earlier evaluation asked for: once the copy of its
+// largest neighbour count has landed (queried, never waited for), a count
+// above max_neighbours raises it to 1.25 times that count, in steps of 32,
+// at most 4096 and at most 16 GiB of list planes.  Overflowing particles
+// take the exact all-partner scan meanwhile, so the results do not change.
+// whether work enqueued on s is being captured into a graph (then the
+// event query and the copy would break the capture)
+bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+void grow_list_capacity(Ctx& c, int64_t n, int slices, hipStream_t s)
+{
+    if (!c.list_max_pending || c.capturing || stream_capturing(s)) return;
+    const hipError_t q = hipEventQuery(c.list_max_ev);
+    if (q != hipSuccess) {
+        (void)hipGetLastError();  // not ready (or unknown): look again next time
+        return;
+    }
+    c.list_max_pending = false;
+    const int m = *c.list_max_host;
+    if (!c.max_neighbours_grow || m <= c.max_neighbours) return;
+    int64_t k = ((int64_t)m + m / 4 + 31) / 32 * 32;
+    const int64_t cap = (int64_t(16) << 30) / (4 * std::max<int64_t>(1, n) * std::max(1, slices));
+    k = std::min<int64_t>(std::min<int64_t>(k, 4096), cap / 32 * 32);
+    if (k > c.max_neighbours) c.max_neighbours = (int)k;
+}
+
+// After the lists of an evaluation are final: their largest count, reduced
+// on ws and copied to pinned host memory behind an event (not while the
+// caller's stream is being captured into a graph, nor without growth).
+int watch_list_counts(Ctx& c, int64_t n, hipStream_t s, hipStream_t ws)
+{
+    if (!c.max_neighbours_grow || c.no_alloc || c.capturing || stream_capturing(s)) return 0;
+    hipError_t e;
+    if (!c.list_max_dev && (e = hipMalloc((void**)&c.list_max_dev, sizeof(int))) != hipSuccess)
+        return hip_fail(c, e, "neighbour-count maximum");
+    if (!c.list_max_host && (e = hipHostMalloc((void**)&c.list_max_host, sizeof(int), hipHostMallocDefault)) != hipSuccess)
+        return hip_fail(c, e, "pinned neighbour-count maximum");
+    if (!c.list_max_ev && (e = hipEventCreateWithFlags(&c.list_max_ev, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(c, e, "neighbour-count event");
+    ProfScope ps(&c, ws, KC_OTHER);
+    if ((e = launch_count_max(c.nbr_cnt, (int)n, c.list_max_dev, ws)) != hipSuccess ||
+        (e = hipMemcpyAsync(c.list_max_host, c.list_max_dev, sizeof(int), hipMemcpyDeviceToHost, ws)) != hipSuccess ||
+        (e = hipEventRecord(c.list_max_ev, ws)) != hipSuccess)
+        return hip_fail(c, e, "neighbour-count watch");
+    c.list_max_pending = true;
+    return 0;
+}
+
 // lists for `slices` slices: slices * kmax * n entries and slices * n counts
 int ensure_nbr(Ctx& c, int64_t n, int slices)
 {
@@ -423,6 +479,7 @@ int derivatives(Ctx& c, const grasph_particles* p, const grasph_params* prm, voi
     if ((sweep_grav || need_list) && !sym) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
     if (sym) slices = sym_near_shares(c, n);  // the near field's shares take the slices' list planes
     if (need_list) {
+        grow_list_capacity(c, n, slices, s);
         if ((r = ensure_nbr(c, n, slices))) return r;
         c.last_list_n = n;
         c.last_list_slices = slices;
@@ -486,6 +543,7 @@ int derivatives(Ctx& c, const grasph_particles* p, const grasph_params* prm, voi
         if (e != hipSuccess) return hip_fail(c, e, "list merge launch");
         c.last_list_slices = 1;
     }
+    if (need_list && (r = watch_list_counts(c, n, s, ws))) return r;
 
     NbrLaunch N = nbr_base(c, p);
     N.perm = perm;
@@ -768,6 +826,9 @@ int grasph_destroy(grasph_ctx* ctx)
     for (auto e : c.ev_free) (void)hipEventDestroy(e);
     if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
     if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+    if (c.list_max_ev) (void)hipEventDestroy(c.list_max_ev);
+    if (c.list_max_dev) (void)hipFree(c.list_max_dev);
+    if (c.list_max_host) (void)hipHostFree(c.list_max_host);
     if (c.side) (void)hipStreamDestroy(c.side);
     delete ctx;
     return 0;
@@ -782,6 +843,9 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     if (k == "max_neighbours") {
         if (value < 1 || value > 4096) return grasph::fail(c, GRASPH_E_INVALID, "max_neighbours must be in [1, 4096]");
         c.max_neighbours = (int)value;
+    } else if (k == "max_neighbours_grow") {
+        if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "max_neighbours_grow must be 0 or 1");
+        c.max_neighbours_grow = (int)value;
     } else if (k == "reorder_interval") {
         if (value < 0) return grasph::fail(c, GRASPH_E_INVALID, "reorder_interval must be >= 0");
         c.reorder_interval = (int)(value > 1000000000 ? 1000000000 : value);
@@ -832,6 +896,7 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value)
     const Ctx& c = ctx->c;
     const std::string k(key);
     if (k == "max_neighbours") *value = c.max_neighbours;
+    else if (k == "max_neighbours_grow") *value = c.max_neighbours_grow;
     else if (k == "reorder_interval") *value = c.reorder_interval;
     else if (k == "sweep_i_per_lane") *value = c.sweep_ni;
     else if (k == "sweep_slices") *value = c.sweep_slices;

@@ -29,6 +29,14 @@ struct Ctx {
 
     // options
     int max_neighbours = 160;
+    // grow max_neighbours after an evaluation whose lists overflowed: its
+    // largest count is reduced on the device and copied to pinned host
+    // memory; a later call reads it once the copy has landed (no wait)
+    int max_neighbours_grow = 1;
+    int* list_max_dev = nullptr;
+    int* list_max_host = nullptr;
+    hipEvent_t list_max_ev = nullptr;
+    bool list_max_pending = false;
     int reorder_interval = 8;  // particles move little in 8 steps: the order only affects speed
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
@@ -278,6 +286,8 @@ inline bool sym_batch_has_far(const SymBatchLaunch& L) { return L.ntotal > 0 && 
 // Fold the per-slice lists of a j-split sweep into one list per particle
 // (slice 0's arrays; a count above kmax means "scan the instance").
 hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s);
+// out[0] = the largest of cnt[0, n) (one workgroup)
+hipError_t launch_count_max(const int* cnt, int n, int* out, hipStream_t s);
 
 // Per-slot record of the neighbour passes (64 bytes, slot order): position
 // and mass; velocity and density; the acceleration pass's j-side terms

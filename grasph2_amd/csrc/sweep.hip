@@ -286,6 +286,24 @@ hipError_t launch_b(const BatchSweepLaunch& L, const DevParams& P, hipStream_t s
     return hipGetLastError();
 }
 
+// The largest neighbour count of a list (plane 0 after the merge: totals,
+// overflowed ones included), one workgroup: every thread strides the
+// counts, then a tree in LDS; the result is written, never accumulated, so
+// nothing needs clearing.
+__global__ __launch_bounds__(1024) void count_max_kernel(const int* __restrict__ cnt, const int n, int* __restrict__ out)
+{
+    __shared__ int red[1024];
+    int m = 0;
+    for (int t = threadIdx.x; t < n; t += 1024) m = max(m, cnt[t]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
 }  // namespace
 
 hipError_t launch_sweep(const SweepLaunch& L, const DevParams& P, hipStream_t s)
@@ -321,6 +339,13 @@ hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, 
     if (n <= 0 || nslices <= 1) return hipSuccess;
     hipLaunchKernelGGL(merge_lists_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbr, cnt, n, kmax,
                        nslices);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_max(const int* cnt, int n, int* out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(count_max_kernel, dim3(1), dim3(1024), 0, s, cnt, n, out);
     return hipGetLastError();
 }
 

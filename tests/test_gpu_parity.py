@@ -258,11 +258,13 @@ def test_order_and_list_capacity_do_not_change_results(gpu_ctx):
         gpu_ctx.set_option("reorder_interval", 0)  # identity order
         ident = gpu_run(gpu_ctx, h, prm)
         gpu_ctx.set_option("reorder_interval", 1)
+        gpu_ctx.set_option("max_neighbours_grow", 0)  # keep the small capacity
         gpu_ctx.set_option("max_neighbours", 8)  # every list overflows: all-j fallback
         small = gpu_run(gpu_ctx, h, prm)
     finally:
         gpu_ctx.set_option("reorder_interval", 8)  # the defaults
         gpu_ctx.set_option("max_neighbours", 160)
+        gpu_ctx.set_option("max_neighbours_grow", 1)
     for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
         assert np.array_equal(base[f], small[f]), f
     mx, nr = vec_errors(ident["acc"], base["acc"])
@@ -275,6 +277,32 @@ def test_order_and_list_capacity_do_not_change_results(gpu_ctx):
 
 
 # ------------------------------------------------------------------ integrator
+def test_list_capacity_grows_after_overflow(gpu_ctx):
+    """An evaluation whose lists overflow (max_neighbours 16 against ~65
+    neighbours) takes the exact scan; its largest count reaches the host
+    behind an event, and the next evaluation raises max_neighbours to 1.25
+    times it (steps of 32).  Both evaluations equal the one with ample
+    capacity bit for bit."""
+    h, prm = solid_case(3000, seed=43)
+    base = gpu_run(gpu_ctx, h, prm)
+    ctx = g.Context(0)
+    try:
+        ctx.set_option("max_neighbours", 16)
+        first = gpu_run(ctx, h, prm)
+        pairs, mxn, over = ctx.neighbour_stats()
+        assert over > 0 and mxn > 16, (pairs, mxn, over)
+        assert ctx.get_option("max_neighbours") == 16
+        second = gpu_run(ctx, h, prm)  # gpu_run synchronised: the count has landed
+        k = ctx.get_option("max_neighbours")
+        pairs2, mxn2, over2 = ctx.neighbour_stats()
+        print("capacity 16 -> %d (largest count %d)" % (k, mxn), flush=True)
+        assert k % 32 == 0 and k >= mxn + mxn // 4 and over2 == 0 and mxn2 == mxn and pairs2 == pairs
+        for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
+            assert np.array_equal(first[f], base[f]) and np.array_equal(second[f], base[f]), f
+    finally:
+        ctx.close()
+
+
 def test_leapfrog_fp64_drift(gpu_ctx):
     h, prm = solid_case(2000, seed=51)
     prm.flags |= g.INTEGRATE_DENSITY

@@ -104,10 +104,12 @@ def test_list_overflow_scan_is_bitwise_equal(sym_ctx):
     h, prm = solid_case(n=9000, seed=14)
     base = gpu_run(sym_ctx, h, prm)
     try:
+        sym_ctx.set_option("max_neighbours_grow", 0)  # keep the small capacity
         sym_ctx.set_option("max_neighbours", 8)
         small = gpu_run(sym_ctx, h, prm)
     finally:
         sym_ctx.set_option("max_neighbours", 160)
+        sym_ctx.set_option("max_neighbours_grow", 1)
     for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
         assert np.array_equal(base[f], small[f]), f
 
