@@ -8,7 +8,7 @@
 // hold neighbour candidates), and the list passes' gathers of neighbour data
 // hit the same cache lines.  The order changes the summation order of the
 // (compensated) sums, not which pairs are summed.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "internal.hpp"
 
@@ -194,8 +194,8 @@ hipError_t launch_gather(const int* perm, int n, const float4* posm, const float
 size_t batch_order_tmp_bytes(int ntotal, int key_bits)
 {
     size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr,
-                                             (int*)nullptr, ntotal, 0, key_bits, (hipStream_t)0);
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr,
+                                    (int*)nullptr, ntotal, 0u, (unsigned)key_bits, (hipStream_t)0);
     return bytes;
 }
 
@@ -208,15 +208,15 @@ hipError_t launch_batch_order(const BatchOrder& B, hipStream_t s)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t bytes = B.sort_tmp_bytes;
-    return hipcub::DeviceRadixSort::SortPairs(B.sort_tmp, bytes, B.keys, B.keys_alt, B.vals, B.perm, B.ntotal, 0,
-                                              B.key_bits, s);
+    return rocprim::radix_sort_pairs(B.sort_tmp, bytes, B.keys, B.keys_alt, B.vals, B.perm, B.ntotal, 0u,
+                                     (unsigned)B.key_bits, s);
 }
 
 size_t reorder_tmp_bytes(int n)
 {
     size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
-                                             (int*)nullptr, n, 0, 30, (hipStream_t)0);
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
+                                    (int*)nullptr, n, 0u, 30u, (hipStream_t)0);
     return bytes;
 }
 
@@ -232,7 +232,7 @@ hipError_t launch_reorder(const float4* posm, int n, Ctx& c, hipStream_t s)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t bytes = c.sort_tmp_bytes;
-    return hipcub::DeviceRadixSort::SortPairs(c.sort_tmp, bytes, c.keys, c.keys_alt, c.vals, c.perm, n, 0, 30, s);
+    return rocprim::radix_sort_pairs(c.sort_tmp, bytes, c.keys, c.keys_alt, c.vals, c.perm, n, 0u, 30u, s);
 }
 
 }  // namespace grasph
