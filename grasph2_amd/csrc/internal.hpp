@@ -233,7 +233,7 @@ struct SymInst {
     int64_t bpart0;     // first float of its partner-side partials
 };
 struct SymRow {
-    int inst, g, units, pad;  // far row g of instance inst: units partner super-blocks
+    int inst, g, piece, units;  // piece `piece` of far row g of instance inst: units partner groups
 };
 struct SymBatchTables {
     std::vector<SymInst> insts;  // every instance; those below min_n have n = 0

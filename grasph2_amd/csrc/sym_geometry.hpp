@@ -1,8 +1,8 @@
 // sym_geometry.hpp -- how the symmetric sweep (sym_sweep.hip) cuts its work:
-// slots in super-blocks of 512 (one far-field wavefront, 8 slots per lane)
-// and groups of 1024 (one far-field workgroup of 2 wavefronts).  The far field's rows are
-// group g against the super-blocks of the later groups; one unit is one
-// partner super-block.  Host and device code use these functions, and
+// slots in groups of 1024 (one far-field wavefront, 16 slots per lane; a
+// super-block is the slots of one far wavefront, here the whole group).  The
+// far field's rows are group g against the later groups; one unit is one
+// partner super-block, or a piece of one.  Host and device code use these functions, and
 // tests/cpp/sym_geometry_check.cpp checks them on the host (every far pair
 // exactly once, indices in bounds, every partial that is read was written).
 #pragma once
@@ -14,16 +14,17 @@
 namespace grasph {
 
 // own slots per lane in the far field: the partner-side reduction and the
-// partners' DPP moves of a chunk are shared by the lane's slots (8: ~216
-// VGPRs, 2 wavefronts per SIMD; 4 was measured 7% slower)
-constexpr int kNI = 8;
+// partners' DPP moves of a chunk are shared by the lane's slots (16: one
+// wavefront owns a group of 1024, ~210 VGPRs, 2 wavefronts per SIMD; 4 was
+// measured 7% slower than 8)
+constexpr int kNI = 16;
 constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefront's slots)
 constexpr int kGroup = 1024;            // slots per group
 constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)
 
 // far units of row g: each partner super-block of [kGroupSB (g + 1), nsb)
-// cut into Q pieces of kSB / Q partners (Q = 1, 2 or 4; more pieces give
-// small particle counts enough units to fill the GPU)
+// cut into Q pieces of kSB / Q partners (Q = 1, 2, 4, 8 or 16; more pieces
+// give small particle counts enough units to fill the GPU)
 __device__ __host__ inline int64_t row_units(int64_t g, int64_t nsb, int Q = 1)
 {
     const int64_t u = nsb - (int64_t)kGroupSB * (g + 1);
@@ -49,19 +50,22 @@ __device__ __host__ inline int64_t row_of_unit(int64_t u, int64_t nsb, int Q = 1
 // pieces per partner super-block for n particles and `wanted` far
 // workgroups (as many as run at once): the kernel lasts as long as its
 // fullest share, ceil(U / W) units of kSB / Q partners with W = min(wanted,
-// U); the Q of 1, 2, 4 that makes that shortest, the fewest pieces on a tie
-// (smaller units cost more per partner)
+// U); the Q of 2, 4, 8, 16 that makes that shortest, the fewest pieces on
+// a tie (smaller units cost more per partner)
+// a piece has at most 512 partners (a slot's plain FP32 own-side sum takes
+// at most that many terms) and at least 64 (4 chunks of 16)
+constexpr int kMinSplit = 2, kMaxSplit = 16;
 __device__ __host__ inline int far_split(int64_t n, int64_t wanted)
 {
     const int64_t nsb = (n + kSB - 1) / kSB;
     const int64_t u1 = row_prefix(rows_with_units(nsb), nsb, 1);
-    if (u1 <= 0 || wanted <= 0) return 1;
-    int best = 1;
+    if (u1 <= 0 || wanted <= 0) return kMinSplit;
+    int best = kMinSplit;
     int64_t best_num = 0, best_den = 1;  // makespan as the fraction num / den super-blocks
-    for (int Q = 1; Q <= 4; Q *= 2) {
+    for (int Q = kMinSplit; Q <= kMaxSplit; Q *= 2) {
         const int64_t U = u1 * Q, W = wanted < U ? wanted : U;
         const int64_t num = (U + W - 1) / W;
-        if (Q == 1 || num * best_den < best_num * Q) {
+        if (Q == kMinSplit || num * best_den < best_num * Q) {
             best = Q;
             best_num = num;
             best_den = Q;
@@ -77,6 +81,18 @@ __host__ inline int far_workgroups(int64_t wanted, int64_t units)
 {
     const int64_t w = wanted < units ? wanted : units;
     return (int)(w > 1 ? w : 1);
+}
+// A batch's far rows go to its wavefronts whole or in pieces: row g of an
+// instance (row_units(g, nsb) partner groups) is cut into
+// batch_row_pieces(units) pieces of consecutive partner groups, piece h
+// starting at batch_piece_begin(units, h); each piece leaves its own-side
+// sums as its own fragment (kBatchPieces fragments per row), so the longest
+// rows do not set the kernel's length.
+constexpr int kBatchPieces = 4;
+__device__ __host__ inline int batch_row_pieces(int64_t units) { return units < kBatchPieces ? (int)units : kBatchPieces; }
+__device__ __host__ inline int64_t batch_piece_begin(int64_t units, int h)
+{
+    return h * units / batch_row_pieces(units);
 }
 // first element of the partner-side partials of group h (the h groups before it contribute)
 __device__ __host__ inline int64_t bpart_group_base(int64_t h) { return (int64_t)kGroup * (h * (h - 1) / 2); }

@@ -3,8 +3,7 @@
 //
 // Same results as the sweep of sweep.hip (softened gravity of
 // calcAcceleration, src/computeDerivatives.h:229-235, plus the neighbour
-// lists of :72,122,238), organised by super-blocks of 512 consecutive slots
-// and groups of 2 super-blocks (1024 slots):
+// lists of :72,122,238), organised by groups of 1024 consecutive slots:
 //  - near field, one wavefront per block of 64 slots, partner blocks in
 //    increasing slot order through the candidate sweep of sweep_core.hpp:
 //    the blocks of its own group with gravity, and (when lists are wanted)
@@ -13,19 +12,17 @@
 //    block (the box distance, rounded like the pair distance, never exceeds
 //    it), so the lists come out complete and in slot order, exactly as the
 //    one-range sweep writes them;
-//  - far field: every super-block pair of different groups (I < J) is
-//    computed once, for both sides (Newton's third law).  A workgroup of 2
-//    wavefronts owns a group, a wavefront one super-block (8 slots per
-//    lane).  The later groups' particles come 16 at a time, one per lane of
-//    each 16-lane DPP row; rotation k (row_ror:k) pairs a lane's slots with
-//    the partner k lanes away in its row.  A lane sums its own slots'
-//    gravity in registers, and the partner side of each rotation over its
-//    8 slots in one accumulator per rotation, so nothing crosses lanes
-//    until the 16 x 16 x 8 pairs of the chunk are done; 15 DPP adds per
-//    component then gather each partner's rotations in its own lane, two
-//    swaps add the four rows, and the values are staged in LDS, summed
-//    over the 2 wavefronts in wavefront order and stored as one partial per
-//    (group, partner);
+//  - far field: every pair of slots in different groups is computed once,
+//    for both sides (Newton's third law).  One wavefront owns a group (16
+//    slots per lane).  The later groups' particles come 16 at a time, one
+//    per lane of each 16-lane DPP row; rotation k (row_ror:k) pairs a lane's
+//    slots with the partner k lanes away in its row.  A lane sums its own
+//    slots' gravity in registers, and the partner side of each rotation
+//    over its 16 slots in one accumulator per rotation, so nothing crosses
+//    lanes until the 16 x 16 x 16 pairs of the chunk are done; 15 DPP adds
+//    per component then gather each partner's rotations in its own lane,
+//    two swaps add the four rows, and the values are stored as the
+//    (group, partner) partial straight away;
 //  - a combine kernel adds, per slot and in a fixed order, the near sum,
 //    the far own-side fragments and the far partner-side partials
 //    (compensated), then finishes the acceleration as the one-range sweep.
@@ -33,7 +30,7 @@
 // the own side and 3 FMA for the partner side (2 more mul with unequal
 // masses), plus, per chunk and lane, 45 DPP moves of the partners' x, y, z
 // (a DPP move with several uses is not folded into them) and 51 for the
-// reduction, over 128 pairs: ~15.8 VALU instructions per wavefront and
+// reduction, over 256 pairs: ~15.4 VALU instructions per wavefront and
 // partner for two interactions per lane and slot, against ~12 for one
 // interaction in the one-sided sweep.
 // Every sum is in a fixed order, so results do not vary from run to run.
@@ -47,8 +44,8 @@
 namespace grasph {
 namespace {
 
-constexpr int kFarThreads = 64 * kGroupSB;
-constexpr int kStageWave = 3 * kSB;  // a wavefront's staged partner side of one unit: 3 components per partner
+constexpr int kFarThreads = 64;  // one wavefront owns a group
+static_assert(kGroupSB == 1 && kSB == kGroup, "the far field assumes one wavefront per group");
 
 // ------------------------------------------------------------------ cross-lane reduction
 
@@ -251,16 +248,17 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
     near_blocks<GRAV, NBR, SOFT, false>(xs, 0, n, nb, nwords, bits, box, I, sl, ns, lane, cnt, part, P, R, 1.0f);
 }
 
-// Own slots of a lane in the far field: position, mass, plain gravity
-// sums, and where its compensated sums of the row are kept: in LDS (sums,
-// the wavefront's kFarSums floats, element (6 q + e) * 64 + lane: e = 0-2
-// sum x y z, 3-5 their corrections), touched once per unit, so they take no
-// registers from the pair loop.
-constexpr int kFarSums = kNI * 6 * 64;
+// Own slots of a lane in the far field: position, mass and the plain
+// gravity sums of the current unit.  The row's compensated sums live in the
+// workgroup's own-side fragment of the row (slot s: frag[2 s] sums x y z,
+// frag[2 s + 1] their corrections), read and written once per unit -- a
+// unit is kNI slots x up to kSB partners per lane, so this costs nothing
+// next to the pair loop and keeps them out of registers and LDS.
 struct FarOwn {
     float x[kNI], y[kNI], z[kNI], m[kNI];
     float ax[kNI], ay[kNI], az[kNI];
-    float* sums;
+    float4* frag;
+    bool fresh;  // no unit of the row folded into the fragment yet: the first writes it
 };
 
 // ------------------------------------------------------------------ rotated far segment
@@ -379,42 +377,49 @@ __device__ __forceinline__ float4 bound_partner_row(const float4 v, const int b,
     return v;
 }
 
-// stage index of this lane's value from far_chunk_rot: 3 partner + component
-// (rows 1 and 3 hold the same z, bit for bit, and store it to the same
-// place: every lane stores, so the store needs no branch, behind which the
-// compiler would sink the chunk's own-side sums)
-__device__ __forceinline__ int stage_index_rot(const int lane)
+// component of the partner side this lane holds after far_chunk_rot: row 0
+// x, row 2 y, rows 1 and 3 z (the same value, bit for bit, stored to the
+// same place: every lane stores, so an unbounded chunk's store needs no
+// branch, behind which the compiler would sink the chunk's own-side sums)
+__device__ __forceinline__ int partner_comp_rot(const int lane)
 {
-    const int row = lane >> 4, p = lane & (kRow - 1);
-    return 3 * p + (row == 0 ? 0 : row == 2 ? 1 : 2);
+    const int row = lane >> 4;
+    return row == 0 ? 0 : row == 2 ? 1 : 2;
 }
 
-// One partner super-block [b0, b0 + kSB) against the wavefront's slots;
-// the reduced partner side of partner p, component c goes to my_stage[3 p + c].
+// Partners [b0, b0 + 16 nchunks) of group J against the wavefront's slots;
+// each chunk's partner side (summed over the whole group: one wavefront
+// owns it) goes straight to row g's partials bp (bp[3 p + c]: partner p of
+// group J, component c; UM: times the common mass m).  The store has no
+// condition: partners past n (only in the instance's last group, whose
+// partials are allocated whole) get values nobody reads, and a store behind
+// a branch makes the compiler spill the chunk's registers.
 template <bool UM, bool SOFT, bool BOUNDED>
 __device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, const int n, const int b0,
-                                                const int nchunks, FarOwn& o, const float H2, const int lane,
-                                                const int stg, float* my_stage)
+                                                const int nchunks, const int J, FarOwn& o, const float H2,
+                                                const int lane, float* __restrict__ bp, const float m)
 {
+    const int comp = partner_comp_rot(lane);
+    const int p0 = b0 - kGroup * J + (lane & (kRow - 1));  // this lane's partner within group J, chunk 0
     float4 A = load_partner_row<UM>(xs, b0, n, lane);
     for (int q = 0; q < nchunks; ++q) {
         const float4 B = load_partner_row<UM>(xs, b0 + kRow * (q + 1), n, lane);  // unused after the last
-        my_stage[3 * kRow * q + stg] =
-            far_chunk_rot<UM, SOFT>(bound_partner_row<BOUNDED>(A, b0 + kRow * q, n, lane), o, H2);
+        const float v = far_chunk_rot<UM, SOFT>(bound_partner_row<BOUNDED>(A, b0 + kRow * q, n, lane), o, H2);
+        const int p = p0 + kRow * q;
+        bp[3 * p + comp] = UM ? v * m : v;  // p past n: in the last group's partials, never read
         A = B;
     }
 }
 
-// The own slots of wavefront w in row g (slots 1024 g + kSB w + 64 q + lane,
-// q < kNI) of an instance of n particles at xs; slots past n sit far away and
-// massless.  Returns whether the wavefront has any slot.
-__device__ __forceinline__ bool far_load_row(const float4* __restrict__ xs, const int n, const int64_t nsb,
-                                             const int64_t g, const int w, const int lane, FarOwn& o)
+// The own slots of row g (slots 1024 g + 64 q + lane, q < kNI) of an
+// instance of n particles at xs (slots past n sit far away and massless),
+// and the row's fragment of workgroup k: frag (fresh until the first unit).
+__device__ __forceinline__ void far_load_row(const float4* __restrict__ xs, const int n, const int64_t g,
+                                             const int lane, float4* frag, FarOwn& o)
 {
-    const int sbA = (int)(g * kGroupSB) + w;
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
-        const int t = kSB * sbA + 64 * q + lane;
+        const int t = kSB * (int)g + 64 * q + lane;
         o.x[q] = o.y[q] = o.z[q] = -3.0e18f;
         o.m[q] = 0.0f;
         if (t < n) {
@@ -422,111 +427,85 @@ __device__ __forceinline__ bool far_load_row(const float4* __restrict__ xs, cons
             o.x[q] = p.x; o.y[q] = p.y; o.z[q] = p.z; o.m[q] = p.w;
         }
         o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
-#pragma unroll
-        for (int e = 0; e < 6; ++e) o.sums[(6 * q + e) * 64 + lane] = 0.0f;
     }
-    return sbA < nsb;
+    o.frag = frag;
+    o.fresh = true;
 }
 
-// own-side compensated sums of the row into fragment f (1024 slots, two float4 each)
-__device__ __forceinline__ void far_flush(float4* __restrict__ apart, const size_t f, const int w, const int lane,
-                                          const FarOwn& o)
+// The lane's plain own-side sums folded (compensated) into the row's
+// fragment, then cleared.
+template <bool UM>
+__device__ __forceinline__ void far_fold_own(FarOwn& o, const float m, const int lane)
 {
+    // the fragment's address passes through an empty asm statement, so the
+    // compiler cannot keep the fragment in registers across a row's units
+    // (96 more live values, which the pair loop cannot spare)
+    float4* frag = o.frag;
+    __asm__ volatile("" : "+v"(frag));
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
-        const size_t e = 2 * (f * kGroup + kSB * w + 64 * q + lane);
-        const float* v = o.sums + 6 * 64 * q + lane;
-        apart[e] = make_float4(v[0], v[64], v[128], 0.0f);
-        apart[e + 1] = make_float4(v[192], v[256], v[320], 0.0f);
-    }
-}
-
-// One far unit of row g: piece `piece` of Q (kSB / Q partners) of partner
-// super-block J against the workgroup's 1024 slots.  The own side is folded
-// into the compensated sums; the partner side, summed over the wavefronts in
-// wavefront order, is stored as row g's partial of each partner (bpart: the
-// instance's partials).  Called by the whole workgroup (two barriers).
-template <bool UM, bool SOFT, int QC = 0>
-__device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t g, const int J,
-                                         const int piece, const int Qr, FarOwn& o, const bool wave_on,
-                                         float* __restrict__ bpart, const float H2, const float m, const float* stage,
-                                         float* my_stage, const int lane, const int tid, const int stg)
-{
-    const int Q = QC ? QC : Qr;  // QC: the split known at compile time (1 for the largest counts)
-    const int len = kSB / Q, b0 = kSB * J + len * piece, nchunks = len / kRow;
-    if (!wave_on) {
-        for (int q = 0; q < nchunks; ++q) my_stage[3 * kRow * q + stg] = 0.0f;
-    } else if (b0 + len > n) {
-        far_segment_rot<UM, SOFT, true>(xs, n, b0, nchunks, o, H2, lane, stg, my_stage);
-    } else {
-        far_segment_rot<UM, SOFT, false>(xs, n, b0, nchunks, o, H2, lane, stg, my_stage);
-    }
-    // own side: plain sums over the segment, then compensated
-#pragma unroll
-    for (int q = 0; q < kNI; ++q) {
-        const float a[3] = {o.ax[q], o.ay[q], o.az[q]};
-        float* v = o.sums + 6 * 64 * q + lane;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            CSum t{v[64 * c], v[64 * (c + 3)]};
-            t.add(UM ? a[c] * m : a[c]);
-            v[64 * c] = t.s;
-            v[64 * (c + 3)] = t.c;
+        const int slot = 64 * q + lane;
+        float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), cv = sv;
+        if (!o.fresh) {
+            sv = frag[2 * slot];
+            cv = frag[2 * slot + 1];
         }
+        CSum x{sv.x, cv.x}, y{sv.y, cv.y}, z{sv.z, cv.z};
+        x.add(UM ? o.ax[q] * m : o.ax[q]);
+        y.add(UM ? o.ay[q] * m : o.ay[q]);
+        z.add(UM ? o.az[q] * m : o.az[q]);
+        frag[2 * slot] = make_float4(x.s, y.s, z.s, 0.0f);
+        frag[2 * slot + 1] = make_float4(x.c, y.c, z.c, 0.0f);
         o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
     }
-    __syncthreads();
-    // partner side: the wavefronts' values in wavefront order
-    for (int r = 0; r < 3 * len / kFarThreads; ++r) {
-        const int idx = tid + kFarThreads * r;  // 3 partner + component
-        float v = 0.0f;
-#pragma unroll
-        for (int ww = 0; ww < kGroupSB; ++ww) v += stage[(size_t)ww * kStageWave + idx];
-        const int b = idx / 3, comp = idx - 3 * b;
-        const int B = b0 + b;
-        if (B < n) {
-            const int64_t h = B / kGroup;
-            const int64_t x = bpart_group_base(h) + g * kGroup + (B - h * kGroup);
-            bpart[3 * x + comp] = UM ? v * m : v;
-        }
-    }
-    __syncthreads();
+    o.fresh = false;
+}
+
+// One far unit of row g: piece `piece` of Q (kSB / Q partners, Q >= 2) of
+// partner group J against the wavefront's 1024 slots.  The partner side is
+// stored per chunk as row g's partials; the own side is folded into the
+// row's fragment (compensated) at the end, so a slot adds at most 512 terms
+// in plain FP32 before they join its compensated sum.
+template <bool UM, bool SOFT, int QC = 0>
+__device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t g, const int J,
+                                         const int piece, const int Qr, FarOwn& o, float* __restrict__ bpart,
+                                         const float H2, const float m, const int lane)
+{
+    const int Q = QC ? QC : Qr;  // QC: the split known at compile time (2 for the largest counts)
+    const int len = kSB / Q, b0 = kSB * J + len * piece, nchunks = len / kRow;
+    float* const bp = bpart + 3 * (bpart_group_base(J) + g * kGroup);
+    if (b0 + len > n)
+        far_segment_rot<UM, SOFT, true>(xs, n, b0, nchunks, J, o, H2, lane, bp, m);
+    else
+        far_segment_rot<UM, SOFT, false>(xs, n, b0, nchunks, J, o, H2, lane, bp, m);
+    far_fold_own<UM>(o, m, lane);
 }
 
 // Far field of one instance.  Workgroup k of W takes the units
 // [k U / W, (k + 1) U / W) of the rows g (group g against the later
-// super-blocks, each cut into Q pieces) laid end to end; a row's own-side
-// sums leave as fragment g + k.
+// groups, each cut into Q pieces) laid end to end; its own-side sums of row
+// g go to fragment g + k.
 template <bool UM, bool SOFT, int QC>
 __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
                                          const int64_t U, const int W, const int Qr, float4* __restrict__ apart,
-                                         float* __restrict__ bpart, const float H2, const float m, float* stage,
-                                         float* sums)
+                                         float* __restrict__ bpart, const float H2, const float m)
 {
     const int Q = QC ? QC : Qr;
-    const int k = blockIdx.x;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
+    const int k = blockIdx.x, lane = threadIdx.x;
     const int64_t u0 = (int64_t)k * U / W, u1 = (int64_t)(k + 1) * U / W;
-    if (u0 >= u1) return;  // the whole workgroup leaves: no barrier is reached
+    if (u0 >= u1) return;
     int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
-    const int stg = stage_index_rot(lane);
-    float* const my_stage = stage + (size_t)w * kStageWave;
     FarOwn o;
-    o.sums = sums + (size_t)w * kFarSums;
-    bool wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
+    far_load_row(xs, n, g, lane, apart + 2 * (size_t)(g + k) * kGroup, o);
     for (int64_t u = u0; u < u1; ++u) {
         if (s == row_units(g, nsb, Q)) {
-            far_flush(apart, (size_t)(g + k), w, lane, o);
             ++g;
             s = 0;
-            wave_on = far_load_row(xs, n, nsb, g, w, lane, o);
+            far_load_row(xs, n, g, lane, apart + 2 * (size_t)(g + k) * kGroup, o);
         }
-        const int J = (int)(kGroupSB * (g + 1) + s / Q);  // partner super-block of this unit
-        far_unit<UM, SOFT, QC>(xs, n, g, J, (int)(s % Q), Q, o, wave_on, bpart, H2, m, stage, my_stage, lane, tid,
-                               stg);
+        far_unit<UM, SOFT, QC>(xs, n, g, (int)(g + 1 + s / Q), (int)(s % Q), Q, o, bpart, H2, m, lane);
         ++s;
     }
-    far_flush(apart, (size_t)(g + k), w, lane, o);
 }
 
 template <bool SOFT>
@@ -536,19 +515,17 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
                                                             float* __restrict__ bpart, const float H2,
                                                             const float* __restrict__ umass)
 {
-    __shared__ float stage[kGroupSB * kStageWave];
-    __shared__ float sums[kGroupSB * kFarSums];
     const bool um = umass[0] != 0.0f;
-    if (Q == 1) {
+    if (Q == 2) {
         if (um)
-            far_work<true, SOFT, 1>(xs, n, nsb, U, W, 1, apart, bpart, H2, umass[1], stage, sums);
+            far_work<true, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, umass[1]);
         else
-            far_work<false, SOFT, 1>(xs, n, nsb, U, W, 1, apart, bpart, H2, 1.0f, stage, sums);
+            far_work<false, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, 1.0f);
     } else {
         if (um)
-            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, umass[1], stage, sums);
+            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, umass[1]);
         else
-            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, 1.0f, stage, sums);
+            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, 1.0f);
     }
 }
 
@@ -731,6 +708,18 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
                                         1.0f);
 }
 
+// one piece of a far row of an instance of the batch: group R.g against
+// R.units partner groups from j0 on, two pieces of 512 partners each
+template <bool UM, bool SOFT>
+__device__ __forceinline__ void batch_far_row(const float4* __restrict__ xk, const int n, const SymRow R, const int j0,
+                                              float4* frag, float* __restrict__ bk, const float H2, const float m,
+                                              const int lane)
+{
+    FarOwn o;
+    far_load_row(xk, n, R.g, lane, frag, o);
+    for (int s = 0; s < 2 * R.units; ++s) far_unit<UM, SOFT, 2>(xk, n, R.g, j0 + (s >> 1), s & 1, 2, o, bk, H2, m, lane);
+}
+
 // Persistent far field over the batch's rows; every wavefront leaves once
 // the counter passes nrows.
 template <bool SOFT>
@@ -739,17 +728,11 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
     const SymRow* __restrict__ rows, const int nrows, int* __restrict__ queue, float4* __restrict__ apart,
     float* __restrict__ bpart, const float* __restrict__ umass)
 {
-    __shared__ float stage[kGroupSB * kStageWave];
-    __shared__ float sums[kGroupSB * kFarSums];
-    __shared__ int row_sh;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, tid = threadIdx.x;
-    const int stg = stage_index_rot(lane);
-    float* const my_stage = stage + (size_t)w * kStageWave;
+    const int lane = threadIdx.x;
     for (;;) {
-        if (tid == 0) row_sh = atomicAdd(queue, 1);
-        __syncthreads();
-        const int r = row_sh;
-        __syncthreads();  // all have read it before it is written again
+        int r = 0;
+        if (lane == 0) r = atomicAdd(queue, 1);
+        r = __builtin_amdgcn_readfirstlane(__shfl(r, 0, 64));
         if (r >= nrows) break;
         const SymRow R = rows[r];
         const SymInst S = sinst[R.inst];
@@ -758,19 +741,12 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
         float* const bk = bpart + S.bpart0;
         const bool um = umass[2 * R.inst] != 0.0f;
         const float m = umass[2 * R.inst + 1];
-        FarOwn o;
-        o.sums = sums + (size_t)w * kFarSums;
-        const bool wave_on = far_load_row(xk, S.n, S.nsb, R.g, w, lane, o);
-        for (int s = 0; s < R.units; ++s) {
-            const int J = kGroupSB * (R.g + 1) + s;
-            if (um)
-                far_unit<true, SOFT, 1>(xk, S.n, R.g, J, 0, 1, o, wave_on, bk, H2, m, stage, my_stage, lane, tid,
-                                        stg);
-            else
-                far_unit<false, SOFT, 1>(xk, S.n, R.g, J, 0, 1, o, wave_on, bk, H2, 1.0f, stage, my_stage, lane,
-                                         tid, stg);
-        }
-        far_flush(apart, (size_t)(S.row0 + R.g), w, lane, o);
+        const int j0 = R.g + 1 + (int)batch_piece_begin(row_units(R.g, S.nsb), R.piece);
+        float4* const frag = apart + 2 * ((size_t)kBatchPieces * (S.row0 + R.g) + R.piece) * kGroup;
+        if (um)
+            batch_far_row<true, SOFT>(xk, S.n, R, j0, frag, bk, H2, m, lane);
+        else
+            batch_far_row<false, SOFT>(xk, S.n, R, j0, frag, bk, H2, 1.0f, lane);
     }
 }
 
@@ -797,10 +773,13 @@ __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
         const int local = t - S.off;
         const int g = local / kGroup, lg = local - g * kGroup;
         if (g < S.nrows) {
-            const size_t e = 2 * ((size_t)(S.row0 + g) * kGroup + lg);
-            const float4 f = apart[e], fc = apart[e + 1];
-            x.add(f.x); y.add(f.y); z.add(f.z);
-            x.c += fc.x; y.c += fc.y; z.c += fc.z;
+            const int pieces = batch_row_pieces(row_units(g, S.nsb));
+            for (int h = 0; h < pieces; ++h) {
+                const size_t e = 2 * (((size_t)kBatchPieces * (S.row0 + g) + h) * kGroup + lg);
+                const float4 f = apart[e], fc = apart[e + 1];
+                x.add(f.x); y.add(f.y); z.add(f.z);
+                x.c += fc.x; y.c += fc.y; z.c += fc.z;
+            }
         }
         const float* bk = bpart + S.bpart0;
         const int64_t base = bpart_group_base(g) + lg;
@@ -956,7 +935,11 @@ SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_n
         S.row0 = row0;
         S.bits0 = bits0;
         S.bpart0 = bpart0;
-        for (int g = 0; g < S.nrows; ++g) T.rows.push_back(SymRow{k, g, (int)row_units(g, S.nsb), 0});
+        for (int g = 0; g < S.nrows; ++g) {
+            const int64_t u = row_units(g, S.nsb);
+            for (int h = 0; h < batch_row_pieces(u); ++h)
+                T.rows.push_back(SymRow{k, g, h, (int)(batch_piece_begin(u, h + 1) - batch_piece_begin(u, h))});
+        }
         T.max_words = std::max(T.max_words, S.nb * S.nwords);
         box0 += S.nb;
         row0 += S.nrows;
@@ -967,7 +950,7 @@ SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_n
     std::stable_sort(T.rows.begin(), T.rows.end(), [](const SymRow& a, const SymRow& b) { return a.units > b.units; });
     T.box_elems = 2 * (int64_t)box0;
     T.bits_elems = bits0;
-    T.apart_elems = 2 * (int64_t)kGroup * row0;
+    T.apart_elems = 2 * (int64_t)kGroup * kBatchPieces * row0;
     T.bpart_elems = bpart0;
     return T;
 }

@@ -1,8 +1,8 @@
 // Host check of the symmetric sweep's work geometry
 // (grasph2_amd/csrc/sym_geometry.hpp), no GPU: replays on the host how the
 // far kernel's workgroups walk their shares of the rows (as far_work in
-// sym_sweep.hip does, with partner super-blocks cut into Q = 1, 2 or 4
-// pieces) and how the combine reads the partials back (sym_combine_kernel),
+// sym_sweep.hip does, with partner super-blocks cut into Q = 1, 2, 4, 8
+// or 16 pieces) and how the combine reads the partials back (sym_combine_kernel),
 // for many particle counts and workgroup counts, and checks that
 //  - every pair of super-blocks in different groups is computed exactly once
 //    and no other pair is;
@@ -10,7 +10,8 @@
 //    the arrays sym_geometry() sizes, and no partial is written twice;
 //  - every fragment and partial the combine reads was written, and every
 //    fragment written is read.
-// Also the batch form: the rows table covers each (instance, row) once.
+// Also the batch form: the rows table covers each (instance, row) once, and
+// each row's pieces cover its partner groups once.
 // Prints "ok <cases>" and exits 0, or prints the first failure and exits 1.
 #include <cstdio>
 #include <cstdlib>
@@ -115,8 +116,19 @@ static int check_batch(const std::vector<int64_t>& sizes)
         const int64_t n = sizes[k], nsb = (n + kSB - 1) / kSB, G = (nsb + kGroupSB - 1) / kGroupSB;
         const int64_t nrows = rows_with_units(nsb);
         for (int64_t g = 0; g < nrows; ++g) {
-            if (row_units(g, nsb) <= 0) return fail("batch empty row", n, (int64_t)k, g);
+            const int64_t u = row_units(g, nsb);
+            if (u <= 0) return fail("batch empty row", n, (int64_t)k, g);
             if (!rows.insert({(int64_t)k, g}).second) return fail("batch row twice", n, (int64_t)k, g);
+            // the pieces: consecutive, non-empty, covering the row's partner groups once
+            const int P = batch_row_pieces(u);
+            if (P < 1 || P > kBatchPieces) return fail("batch pieces", n, (int64_t)k, g);
+            int64_t next = 0;
+            for (int h = 0; h < P; ++h) {
+                const int64_t b = batch_piece_begin(u, h), e = batch_piece_begin(u, h + 1);
+                if (b != next || e <= b) return fail("batch piece range", n, (int64_t)k, g, h);
+                next = e;
+            }
+            if (next != u) return fail("batch pieces cover", n, (int64_t)k, g);
         }
         if (row_units(nrows, nsb) != 0) return fail("batch row missing", n, (int64_t)k, nrows);
         row0 += nrows;
@@ -133,7 +145,7 @@ int main()
     int cases = 0;
     for (int64_t n : ns)
         for (int64_t w : ws)
-            for (int Q : {1, 2, 4}) {
+            for (int Q : {1, 2, 4, 8, 16}) {
                 if (check(n, w, Q)) return 1;
                 ++cases;
             }
