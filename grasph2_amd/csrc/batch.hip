@@ -483,7 +483,7 @@ int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_
     L.nrows = b.n_srows;
     L.max_words = b.sym_max_words;
     L.queue = b.d_queue + 1;
-    L.wgs = sym_default_workgroups(device_cus(c));
+    L.wgs = c.sym_workgroups > 0 ? c.sym_workgroups : sym_default_workgroups(device_cus(c));
     L.acc = (float4*)p->acc;
     L.nbr = b.d_nbr;
     L.nbr_cnt = b.d_cnt;

@@ -83,7 +83,9 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
         uy = v.y + P.xsph * xv.y;
         uz = v.z + P.xsph * xv.z;
     }
-    float4 p = posm[i];
+    // with FP64 masters the FP32 mirror is only written (x y z; the mass in
+    // w stays), so posm is not read
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
     double X = 0, Y = 0, Z = 0;
     if (pos64) {
         X = pos64[m.plane3(0, i)] + (double)ux * (double)dt;
@@ -91,6 +93,7 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
         Z = pos64[m.plane3(2, i)] + (double)uz * (double)dt;
         p.x = (float)X; p.y = (float)Y; p.z = (float)Z;
     } else {
+        p = posm[i];
         p.x = p.x + ux * dt;
         p.y = p.y + uy * dt;
         p.z = p.z + uz * dt;
@@ -138,7 +141,10 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(float4* __restrict__ p
     if (pos64) {
         pos64[m.plane3(0, i)] = X; pos64[m.plane3(1, i)] = Y; pos64[m.plane3(2, i)] = Z;
     }
-    posm[i] = p;
+    if (pos64)
+        *reinterpret_cast<float3*>(posm + i) = make_float3(p.x, p.y, p.z);
+    else
+        posm[i] = p;
     vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 

@@ -249,16 +249,21 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
 }
 
 // Own slots of a lane in the far field: position, mass and the plain
-// gravity sums of the current unit.  The row's compensated sums live in the
-// workgroup's own-side fragment of the row (slot s: frag[2 s] sums x y z,
-// frag[2 s + 1] their corrections), read and written once per unit -- a
-// unit is kNI slots x up to kSB partners per lane, so this costs nothing
-// next to the pair loop and keeps them out of registers and LDS.
+// gravity sums of the current unit.  At the end of a unit those go into
+// the wavefront's LDS sums (lsum[1024 c + slot], component c: plain sums of
+// at most kFoldsPerFlush units), and every kFoldsPerFlush units and at the
+// end of the row the LDS sums go into the row's own-side fragment in global
+// memory (slot s: frag[2 s] sums x y z, frag[2 s + 1] their corrections),
+// compensated.  Neither level takes registers from the pair loop, and the
+// fragment is read and written about once per row.
+constexpr int kFoldsPerFlush = 16;
 struct FarOwn {
     float x[kNI], y[kNI], z[kNI], m[kNI];
     float ax[kNI], ay[kNI], az[kNI];
+    float* lsum;
     float4* frag;
-    bool fresh;  // no unit of the row folded into the fragment yet: the first writes it
+    int folds;   // units in lsum since the last flush (0: lsum holds nothing)
+    bool fresh;  // nothing flushed into the fragment yet: the first flush writes it
 };
 
 // ------------------------------------------------------------------ rotated far segment
@@ -413,9 +418,10 @@ __device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, c
 
 // The own slots of row g (slots 1024 g + 64 q + lane, q < kNI) of an
 // instance of n particles at xs (slots past n sit far away and massless),
-// and the row's fragment of workgroup k: frag (fresh until the first unit).
+// the wavefront's LDS sums lsum, and the row's fragment of workgroup k:
+// frag (fresh until the first flush).
 __device__ __forceinline__ void far_load_row(const float4* __restrict__ xs, const int n, const int64_t g,
-                                             const int lane, float4* frag, FarOwn& o)
+                                             const int lane, float* lsum, float4* frag, FarOwn& o)
 {
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
@@ -428,44 +434,75 @@ __device__ __forceinline__ void far_load_row(const float4* __restrict__ xs, cons
         }
         o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
     }
+    o.lsum = lsum;
     o.frag = frag;
+    o.folds = 0;
     o.fresh = true;
 }
 
-// The lane's plain own-side sums folded (compensated) into the row's
-// fragment, then cleared.
+// The lane's slot of own slot q, through an empty asm statement: indexed
+// by it, the LDS sums and the fragment are never kept in registers across
+// a row's units by the compiler (96 more live values, which the pair loop
+// cannot spare).
+__device__ __forceinline__ int own_lane(const int lane)
+{
+    int l = lane;
+    __asm__ volatile("" : "+v"(l));
+    return l;
+}
+
+// The LDS sums of the lane's slots added (compensated) into the row's
+// fragment.
+__device__ __forceinline__ void far_flush_own(FarOwn& o, const int lane)
+{
+    if (o.folds == 0) return;
+    const int l = own_lane(lane);
+#pragma unroll
+    for (int q = 0; q < kNI; ++q) {
+        const int slot = 64 * q + l;
+        float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), cv = sv;
+        if (!o.fresh) {
+            sv = o.frag[2 * slot];
+            cv = o.frag[2 * slot + 1];
+        }
+        CSum x{sv.x, cv.x}, y{sv.y, cv.y}, z{sv.z, cv.z};
+        x.add(o.lsum[slot]);
+        y.add(o.lsum[kGroup + slot]);
+        z.add(o.lsum[2 * kGroup + slot]);
+        o.frag[2 * slot] = make_float4(x.s, y.s, z.s, 0.0f);
+        o.frag[2 * slot + 1] = make_float4(x.c, y.c, z.c, 0.0f);
+    }
+    o.fresh = false;
+    o.folds = 0;
+}
+
+// The lane's plain own-side sums of a unit added to its LDS sums, then
+// cleared; every kFoldsPerFlush units the LDS sums go to the fragment.
 template <bool UM>
 __device__ __forceinline__ void far_fold_own(FarOwn& o, const float m, const int lane)
 {
-    // the fragment's address passes through an empty asm statement, so the
-    // compiler cannot keep the fragment in registers across a row's units
-    // (96 more live values, which the pair loop cannot spare)
-    float4* frag = o.frag;
-    __asm__ volatile("" : "+v"(frag));
+    const int l = own_lane(lane);
+    const bool first = o.folds == 0;
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
-        const int slot = 64 * q + lane;
-        float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), cv = sv;
-        if (!o.fresh) {
-            sv = frag[2 * slot];
-            cv = frag[2 * slot + 1];
-        }
-        CSum x{sv.x, cv.x}, y{sv.y, cv.y}, z{sv.z, cv.z};
-        x.add(UM ? o.ax[q] * m : o.ax[q]);
-        y.add(UM ? o.ay[q] * m : o.ay[q]);
-        z.add(UM ? o.az[q] * m : o.az[q]);
-        frag[2 * slot] = make_float4(x.s, y.s, z.s, 0.0f);
-        frag[2 * slot + 1] = make_float4(x.c, y.c, z.c, 0.0f);
+        const int slot = 64 * q + l;
+        const float ax = UM ? o.ax[q] * m : o.ax[q];
+        const float ay = UM ? o.ay[q] * m : o.ay[q];
+        const float az = UM ? o.az[q] * m : o.az[q];
+        o.lsum[slot] = first ? ax : o.lsum[slot] + ax;
+        o.lsum[kGroup + slot] = first ? ay : o.lsum[kGroup + slot] + ay;
+        o.lsum[2 * kGroup + slot] = first ? az : o.lsum[2 * kGroup + slot] + az;
         o.ax[q] = o.ay[q] = o.az[q] = 0.0f;
     }
-    o.fresh = false;
+    if (++o.folds == kFoldsPerFlush) far_flush_own(o, lane);
 }
 
 // One far unit of row g: piece `piece` of Q (kSB / Q partners, Q >= 2) of
 // partner group J against the wavefront's 1024 slots.  The partner side is
-// stored per chunk as row g's partials; the own side is folded into the
-// row's fragment (compensated) at the end, so a slot adds at most 512 terms
-// in plain FP32 before they join its compensated sum.
+// stored per chunk as row g's partials; the own side is folded at the end,
+// so a slot adds at most 512 terms in plain FP32 before they join its LDS
+// sum (and at most kFoldsPerFlush such sums before they join the
+// compensated fragment).
 template <bool UM, bool SOFT, int QC = 0>
 __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t g, const int J,
                                          const int piece, const int Qr, FarOwn& o, float* __restrict__ bpart,
@@ -488,7 +525,7 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
 template <bool UM, bool SOFT, int QC>
 __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
                                          const int64_t U, const int W, const int Qr, float4* __restrict__ apart,
-                                         float* __restrict__ bpart, const float H2, const float m)
+                                         float* __restrict__ bpart, const float H2, const float m, float* lsum)
 {
     const int Q = QC ? QC : Qr;
     const int k = blockIdx.x, lane = threadIdx.x;
@@ -496,16 +533,18 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
     if (u0 >= u1) return;
     int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
     FarOwn o;
-    far_load_row(xs, n, g, lane, apart + 2 * (size_t)(g + k) * kGroup, o);
+    far_load_row(xs, n, g, lane, lsum, apart + 2 * (size_t)(g + k) * kGroup, o);
     for (int64_t u = u0; u < u1; ++u) {
         if (s == row_units(g, nsb, Q)) {
+            far_flush_own(o, lane);
             ++g;
             s = 0;
-            far_load_row(xs, n, g, lane, apart + 2 * (size_t)(g + k) * kGroup, o);
+            far_load_row(xs, n, g, lane, lsum, apart + 2 * (size_t)(g + k) * kGroup, o);
         }
         far_unit<UM, SOFT, QC>(xs, n, g, (int)(g + 1 + s / Q), (int)(s % Q), Q, o, bpart, H2, m, lane);
         ++s;
     }
+    far_flush_own(o, lane);
 }
 
 template <bool SOFT>
@@ -515,17 +554,18 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
                                                             float* __restrict__ bpart, const float H2,
                                                             const float* __restrict__ umass)
 {
+    __shared__ float lsum[3 * kGroup];
     const bool um = umass[0] != 0.0f;
     if (Q == 2) {
         if (um)
-            far_work<true, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, umass[1]);
+            far_work<true, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, umass[1], lsum);
         else
-            far_work<false, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, 1.0f);
+            far_work<false, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, 1.0f, lsum);
     } else {
         if (um)
-            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, umass[1]);
+            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, umass[1], lsum);
         else
-            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, 1.0f);
+            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, 1.0f, lsum);
     }
 }
 
@@ -712,12 +752,13 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
 // R.units partner groups from j0 on, two pieces of 512 partners each
 template <bool UM, bool SOFT>
 __device__ __forceinline__ void batch_far_row(const float4* __restrict__ xk, const int n, const SymRow R, const int j0,
-                                              float4* frag, float* __restrict__ bk, const float H2, const float m,
-                                              const int lane)
+                                              float* lsum, float4* frag, float* __restrict__ bk, const float H2,
+                                              const float m, const int lane)
 {
     FarOwn o;
-    far_load_row(xk, n, R.g, lane, frag, o);
+    far_load_row(xk, n, R.g, lane, lsum, frag, o);
     for (int s = 0; s < 2 * R.units; ++s) far_unit<UM, SOFT, 2>(xk, n, R.g, j0 + (s >> 1), s & 1, 2, o, bk, H2, m, lane);
+    far_flush_own(o, lane);
 }
 
 // Persistent far field over the batch's rows; every wavefront leaves once
@@ -728,6 +769,7 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
     const SymRow* __restrict__ rows, const int nrows, int* __restrict__ queue, float4* __restrict__ apart,
     float* __restrict__ bpart, const float* __restrict__ umass)
 {
+    __shared__ float lsum[3 * kGroup];
     const int lane = threadIdx.x;
     for (;;) {
         int r = 0;
@@ -744,9 +786,9 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
         const int j0 = R.g + 1 + (int)batch_piece_begin(row_units(R.g, S.nsb), R.piece);
         float4* const frag = apart + 2 * ((size_t)kBatchPieces * (S.row0 + R.g) + R.piece) * kGroup;
         if (um)
-            batch_far_row<true, SOFT>(xk, S.n, R, j0, frag, bk, H2, m, lane);
+            batch_far_row<true, SOFT>(xk, S.n, R, j0, lsum, frag, bk, H2, m, lane);
         else
-            batch_far_row<false, SOFT>(xk, S.n, R, j0, frag, bk, H2, 1.0f, lane);
+            batch_far_row<false, SOFT>(xk, S.n, R, j0, lsum, frag, bk, H2, 1.0f, lane);
     }
 }
 

@@ -164,7 +164,8 @@ int grasph_destroy(grasph_ctx* ctx);
  * 16384 particles on when neighbour lists are built, from 32768 for gravity
  * alone, the default; results agree to FP32 rounding, the
  * neighbour lists are identical), "sym_workgroups" (workgroups of the
- * symmetric sweep's far-field kernel, 0 = two wavefronts per SIMD),
+ * symmetric sweep's far-field kernel, single instances and batches, 0 = two
+ * wavefronts per SIMD),
  * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
  * 0 = chosen from the occupancy), "sym_list_priority" (its near-field list
  * kernel takes issue priority over the far field running beside it: 1 = yes,
