@@ -348,6 +348,19 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
     S.bpart = c.sym_bpart;
     hipError_t e;
     ws = s;
+    if (sym_has_far(S) && !S.want_nbr) {
+        // gravity alone: nothing would run beside the far field but the
+        // in-group gravity, which, launched second, waits for the far field's
+        // workgroups to leave; both in order on s, the in-group gravity first
+        {
+            ProfScope ps(&c, s, KC_SWEEP);
+            if ((e = launch_sym_ingroup(S, P, s)) != hipSuccess) return hip_fail(c, e, "near-field gravity launch");
+            S.ingroup_done = true;
+        }
+        ProfScope ps(&c, s, KC_FAR);
+        if ((e = launch_sym_far(S, P, s)) != hipSuccess) return hip_fail(c, e, "far-field launch");
+        return 0;
+    }
     if (sym_has_far(S)) {
         if (c.no_alloc && (!c.side || !c.ev_fork || !c.ev_join))
             return fail(c, GRASPH_E_NOMEM, "the second stream would need creating");
@@ -379,7 +392,7 @@ int sym_sweep_end(Ctx& c, SymLaunch& S, const DevParams& P, hipStream_t s, hipSt
 {
     hipError_t e;
     S.add_acc = add_acc;
-    {
+    if (!S.ingroup_done) {
         // the gravity inside the groups last: only the combine needs it
         ProfScope ps(&c, ws, KC_SWEEP);
         if ((e = launch_sym_ingroup(S, P, ws)) != hipSuccess) return hip_fail(c, e, "near-field gravity launch");

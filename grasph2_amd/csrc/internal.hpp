@@ -197,6 +197,7 @@ struct SymLaunch {
     bool finish;        // as SweepLaunch::finish
     bool add_acc;       // acc already holds the SPH terms: the combine adds the gravity to them
     bool list_prio;     // the lists' wavefronts take issue priority over the far field's beside them
+    bool ingroup_done;  // the in-group gravity is already enqueued
     int nshares;        // near field cut into this many shares per block (own list planes, merged after)
     float4* acc;
     int* nbr;           // a list per slot and share
