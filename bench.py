@@ -46,10 +46,15 @@ SYM_AUTO_MIN_LISTS = 16384    # with neighbour lists (SPH on)
 SYM_AUTO_MIN_GRAVITY = 32768  # gravity alone
 SYM_GROUP = 1024  # slots per group of the symmetric sweep; pairs inside a group are the near field's
 FLOP_GRAVITY_PAIR = 20
-FLOP_DENSITY_NBR = 9
-FLOP_STRAIN_NBR = 63
-FLOP_ACCEL_NBR_SOLID = 74
-FLOP_ACCEL_NBR_LIQUID = 54
+# Per neighbour pair, the FLOPs of the passes' formulas (FMA 2; add, sub, mul 1; sqrt, rsqrt, div 1;
+# compensation 0), data-dependent branches at their typical share: W's inner branch (q < 1/2, 1/8 of
+# the neighbours by volume) costs 4 more than the outer; the artificial viscosity of approaching
+# pairs (about half) 11 more than that of receding ones.
+FLOP_DENSITY_NBR = 17.5        # r 3, r^2 5, sqrt, q, 1 - q, W 3 (7 inside), prefactor, m_j W, sum
+FLOP_STRAIN_NBR_SOLID = 75     # r 3, dW/r 12, g 3, v 3, m_j / rho_j, -vol / 2, 9 rate entries x 5, div v 7
+FLOP_STRAIN_NBR_LIQUID = 44    # the curl (15) instead of the rate tensors (46); 29 with div v alone
+FLOP_ACCEL_NBR_SOLID = 72.5    # r 3, r^2 5, sqrt, gradient 15, stress 6 + 18, v 3, AV 6 + 11 / 2 + 7, sums 3
+FLOP_ACCEL_NBR_LIQUID = 55.5   # the stress term 7 instead of 24
 INTEGRATOR_BYTES_SOLID = 168   # x64 24 + v 12 + a 12 + S 24 + dS 24 read; x64 24 + xf 12 + v 12 + S 24 written
 INTEGRATOR_BYTES_NBODY = 96    # x64 24 + v 12 + a 12 read; x64 24 + xf 12 + v 12 written
 
@@ -212,8 +217,16 @@ def main():
     integ_bytes = n * (INTEGRATOR_BYTES_SOLID if solid else INTEGRATOR_BYTES_NBODY)
     step_flops = sweep_flops
     if sph:
-        step_flops += nb_pairs * (FLOP_DENSITY_NBR + FLOP_STRAIN_NBR +
-                                  (FLOP_ACCEL_NBR_SOLID if solid else FLOP_ACCEL_NBR_LIQUID))
+        fl = prm.flags
+        dens = 0 if fl & g.INTEGRATE_DENSITY else FLOP_DENSITY_NBR
+        strain = 0
+        if solid:
+            strain = FLOP_STRAIN_NBR_SOLID
+        elif fl & g.BALSARA_SWITCH:
+            strain = FLOP_STRAIN_NBR_LIQUID
+        elif fl & g.INTEGRATE_DENSITY:
+            strain = FLOP_STRAIN_NBR_LIQUID - 15
+        step_flops += nb_pairs * (dens + strain + (FLOP_ACCEL_NBR_SOLID if solid else FLOP_ACCEL_NBR_LIQUID))
 
     traffic = None
     # HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
