@@ -14,8 +14,8 @@ roofline: the dominant kernel, FP32 vector ALU bound, at 20 algorithmic
 FLOPs per ordered gravity pair (FMA = 2, rsqrt = 1), divided by its mean
 duration from HIP events recorded around every launch on the stream it
 runs on, over a second pass of as many steps (the timed pass records no
-events: they lengthen a step by ~2%).  From 16384 particles with SPH, 32768
-for gravity alone (the symmetric sweep)
+events: they lengthen a step by ~2%).  From 16384 particles (the symmetric
+sweep)
 that kernel is the far field (far_kernel): the ordered pairs of particles in
 different groups of 1024 slots, N (N - 1) minus the in-group pairs, 99% of
 the pairs at N = 131072; the near field, the list merge and the SPH
@@ -43,7 +43,7 @@ PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (AMD published)
 # from this many particles on, gravity between groups of 1024 is evaluated once per pair for both sides.
 # The roofline stays algorithmic either way: 20 FLOP per ordered pair.
 SYM_AUTO_MIN_LISTS = 16384    # with neighbour lists (SPH on)
-SYM_AUTO_MIN_GRAVITY = 32768  # gravity alone
+SYM_AUTO_MIN_GRAVITY = 16384  # gravity alone
 SYM_GROUP = 1024  # slots per group of the symmetric sweep; pairs inside a group are the near field's
 FLOP_GRAVITY_PAIR = 20
 # Per neighbour pair, the FLOPs of the passes' formulas (FMA 2; add, sub, mul 1; sqrt, rsqrt, div 1;

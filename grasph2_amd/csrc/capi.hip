@@ -277,16 +277,13 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
 // later particle): 6 n^2 / 1024 bytes.  Past this size the j-split sweep
 // runs instead.
 constexpr int64_t kSymMaxPartialBytes = int64_t(12) << 30;
-// From this size on the symmetric sweep is the faster one (measured: 9%
-// slower at 16k particles, 3% faster at 65k, 10-16% at 131k).
 // The symmetric sweep's default thresholds (option sweep_symmetric = 2),
-// measured on the collision and Plummer workloads: with neighbour lists the
-// lists and the SPH passes run beside its far field, and it wins from 16384
-// particles (32768: 0.41 against 0.59 ms per step); gravity alone has
-// nothing to overlap, and its coarse far-field units leave the GPU part
-// idle below 32768 (24576: 0.29 against 0.23 ms).
+// measured step times against the one-sided sweep: with neighbour lists
+// (collision) 0.319 against 0.322 ms at 16384 particles, 0.269 against
+// 0.221 at 8192; gravity alone (Plummer) 0.106 against 0.116 ms at 16384,
+// 0.216 against 0.222 at 24576, 0.268 against 0.369 at 32768.
 constexpr int64_t kSymAutoMinLists = 16384;
-constexpr int64_t kSymAutoMinGravity = 32768;
+constexpr int64_t kSymAutoMinGravity = 16384;
 
 int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : sym_default_workgroups(c.cus); }
 

@@ -40,7 +40,7 @@ struct Ctx {
     int reorder_interval = 8;  // particles move little in 8 steps: the order only affects speed
     int sweep_ni = 1;  // i particles per lane in the single-instance sweep
     int sweep_slices = 0;  // j-split of the single-instance sweep (0: chosen from the occupancy)
-    int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 16384 particles with SPH, 32768 for gravity alone
+    int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 16384 particles
     int sym_workgroups = 0;   // workgroups of its far kernel (0: 2 wavefronts per SIMD)
     int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
     int sym_list_priority = 2;  // its near-field lists ahead of the far field: 0 no, 1 yes, 2 below 98304 particles

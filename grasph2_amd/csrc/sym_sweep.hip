@@ -572,7 +572,12 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
 // Per slot, in a fixed order: the near partials (share order), the far own-side fragments
 // (workgroup order), the far partner-side partials (group order, negated:
 // they were summed as seen from the other side), compensated; then the
-// acceleration is finished as the one-range sweep does.
+// acceleration is finished as the one-range sweep does.  With PARTS > 1
+// (small counts: the slots alone would not fill the GPU, and a slot's row
+// has up to hundreds of fragments), PARTS threads share a slot: part p
+// takes every PARTS-th term of each list from the p-th on, and part 0 adds
+// the parts' compensated sums in part order.
+template <int PARTS>
 __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
     const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
     const int n, const int nshares, const int64_t nsb, const int64_t U, const int W, const int Q, const int grav,
@@ -580,11 +585,12 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
     const float4* __restrict__ xs, float4* __restrict__ acc, const int finish, const int add_acc,
     const float4* __restrict__ vel, float4* __restrict__ xvel, float* __restrict__ maxvsig, const DevParams P)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
+    constexpr int kSlots = kBlock / PARTS;  // slots per block
+    const int p = (int)threadIdx.x / kSlots;
+    const int t = blockIdx.x * kSlots + ((int)threadIdx.x - p * kSlots);
     CSum x{0.0f, 0.0f}, y{0.0f, 0.0f}, z{0.0f, 0.0f};
-    if (grav) {
-        for (int sl = 0; sl < nshares; ++sl) {
+    if (grav && t < n) {
+        for (int sl = p; sl < nshares; sl += PARTS) {
             const size_t e = 2 * ((size_t)sl * n + t);
             const float4 a = near_part[e], c = near_part[e + 1];
             x.add(a.x); y.add(a.y); z.add(a.z);
@@ -594,15 +600,15 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
         if (U > 0 && g < rows_with_units(nsb)) {
             // one fragment per workgroup that touched the row (every workgroup
             // has a share: W <= U); eight fragments' loads in flight before
-            // their ordered adds (rows of small counts span many workgroups)
+            // their ordered adds
             const int64_t k0 = wg_of_unit(row_prefix(g, nsb, Q), U, W);
             const int64_t k1 = wg_of_unit(row_prefix(g + 1, nsb, Q) - 1, U, W);
-            int64_t k = k0;
-            for (; k + 8 <= k1 + 1; k += 8) {
+            int64_t k = k0 + p;
+            for (; k + 7 * PARTS <= k1; k += 8 * PARTS) {
                 float4 f[8], fc[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    const size_t e = 2 * ((size_t)(g + k + u) * kGroup + local);
+                    const size_t e = 2 * ((size_t)(g + k + u * PARTS) * kGroup + local);
                     f[u] = apart[e];
                     fc[u] = apart[e + 1];
                 }
@@ -612,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
                     x.c += fc[u].x; y.c += fc[u].y; z.c += fc[u].z;
                 }
             }
-            for (; k <= k1; ++k) {
+            for (; k <= k1; k += PARTS) {
                 const size_t e = 2 * ((size_t)(g + k) * kGroup + local);
                 const float4 f = apart[e], fc = apart[e + 1];
                 x.add(f.x); y.add(f.y); z.add(f.z);
@@ -622,12 +628,12 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
         const int64_t base = bpart_group_base(g) + local;
         const int ng = (int)g;
         // eight groups' loads in flight before their (ordered) adds
-        int gp = 0;
-        for (; gp + 8 <= ng; gp += 8) {
+        int gp = p;
+        for (; gp + 7 * PARTS < ng; gp += 8 * PARTS) {
             float v[8][3];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const size_t e = 3 * (size_t)(base + (int64_t)(gp + u) * kGroup);
+                const size_t e = 3 * (size_t)(base + (int64_t)(gp + u * PARTS) * kGroup);
                 v[u][0] = bpart[e]; v[u][1] = bpart[e + 1]; v[u][2] = bpart[e + 2];
             }
 #pragma unroll
@@ -635,11 +641,29 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
                 x.add(-v[u][0]); y.add(-v[u][1]); z.add(-v[u][2]);
             }
         }
-        for (; gp < ng; ++gp) {
+        for (; gp < ng; gp += PARTS) {
             const size_t e = 3 * (size_t)(base + (int64_t)gp * kGroup);
             x.add(-bpart[e]); y.add(-bpart[e + 1]); z.add(-bpart[e + 2]);
         }
     }
+    if constexpr (PARTS > 1) {
+        __shared__ float red[6][kBlock];  // every thread reaches the barrier
+        if (p > 0) {
+            red[0][threadIdx.x] = x.s; red[1][threadIdx.x] = x.c;
+            red[2][threadIdx.x] = y.s; red[3][threadIdx.x] = y.c;
+            red[4][threadIdx.x] = z.s; red[5][threadIdx.x] = z.c;
+        }
+        __syncthreads();
+        if (p > 0) return;
+#pragma unroll
+        for (int q = 1; q < PARTS; ++q) {
+            const int o = (int)threadIdx.x + q * kSlots;
+            x.add(red[0][o]); x.c += red[1][o];
+            y.add(red[2][o]); y.c += red[3][o];
+            z.add(red[4][o]); z.c += red[5][o];
+        }
+    }
+    if (t >= n) return;
     const int i = perm ? perm[t] : t;
     if (add_acc) {  // the SPH terms (and environment), summed before the far field was done
         const float4 a = acc[i];
@@ -948,9 +972,20 @@ hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_
 {
     if (L.n <= 0 || !(L.grav || L.finish)) return hipSuccess;
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
-    hipLaunchKernelGGL(sym_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part, L.apart,
-                       L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, G.split, L.grav ? 1 : 0, L.perm, L.xs, L.acc,
-                       L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.vel, L.xvel, L.maxvsig, P);
+    // threads per slot: about 2^17 threads or more in all
+    const int parts = L.n >= 98304 ? 1 : L.n >= 49152 ? 2 : L.n >= 24576 ? 4 : 8;
+#define GRASPH_COMBINE(PARTS)                                                                                         \
+    hipLaunchKernelGGL((sym_combine_kernel<PARTS>), dim3((L.n + kBlock / PARTS - 1) / (kBlock / PARTS)), dim3(kBlock), \
+                       0, s, L.near_part, L.apart, L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, G.split,            \
+                       L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.vel, L.xvel,       \
+                       L.maxvsig, P)
+    switch (parts) {
+    case 1: GRASPH_COMBINE(1); break;
+    case 2: GRASPH_COMBINE(2); break;
+    case 4: GRASPH_COMBINE(4); break;
+    default: GRASPH_COMBINE(8); break;
+    }
+#undef GRASPH_COMBINE
     return hipGetLastError();
 }
 

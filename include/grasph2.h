@@ -161,8 +161,7 @@ int grasph_destroy(grasph_ctx* ctx);
  * shape), "sweep_symmetric" (single
  * instance: 1 = gravity between groups of 1024 particles evaluated once per
  * pair for both sides, 0 = the one-sided sweep, 2 = the symmetric sweep from
- * 16384 particles on when neighbour lists are built, from 32768 for gravity
- * alone, the default; results agree to FP32 rounding, the
+ * 16384 particles on, the default; results agree to FP32 rounding, the
  * neighbour lists are identical), "sym_workgroups" (workgroups of the
  * symmetric sweep's far-field kernel, single instances and batches, 0 = two
  * wavefronts per SIMD),
