@@ -78,6 +78,34 @@ __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, 
                                       const float (&yi)[NI], const float (&zi)[NI], Acc<NI>& a, const int slot,
                                       const float H2, const float cand, const Rare& R)
 {
+    if constexpr (NBR && !GRAV && !POT) {
+        // Lists alone: the exact r^2 on the fast path (two instructions more
+        // per pair than the softened d2), so a candidate costs only the test
+        // and the append -- in the near field of small dense instances
+        // nearly every chunk holds one.  Same test, same pairs, same order
+        // as the rare path below.
+        float r2[NI][kChunk];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c)
+#pragma unroll
+            for (int q = 0; q < NI; ++q) r2[q][c] = dist2_exact(xi[q] - pj[c].x, yi[q] - pj[c].y, zi[q] - pj[c].z);
+        float mn = r2[0][0];
+#pragma unroll
+        for (int q = 0; q < NI; ++q)
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c) mn = fminf(mn, r2[q][c]);
+        if (mn <= R.h2) {
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c)
+#pragma unroll
+                for (int q = 0; q < NI; ++q)
+                    if (is_neighbour(r2[q][c], R.h2)) {
+                        if (a.cnt[q] < R.kmax) R.nbr[(size_t)a.cnt[q] * R.nstride + slot + 64 * q] = jb + c;
+                        ++a.cnt[q];
+                    }
+        }
+        return;
+    }
     constexpr bool BRANCH = NBR || (POT && SOFT);
     float d2[NI][kChunk];
 #pragma unroll
