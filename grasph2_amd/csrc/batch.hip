@@ -266,7 +266,7 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
     if (std::memcmp(b.tuned_ni, c.batch_ni, sizeof b.tuned_ni) != 0 || !b.d_items || b.items_sym_min != want_sym_min)
         if ((r = build_items(c, b))) return r;
     // lists and records
-    const int64_t need = (int64_t)c.max_neighbours * b.ntotal;
+    const int64_t need = (int64_t)list_plane(b.ntotal, c.max_neighbours);
     if (!b.d_nbr || b.nbr_cap < need || b.kmax != c.max_neighbours) {
         free_ptr(b.d_nbr);
         free_ptr(b.d_cnt);

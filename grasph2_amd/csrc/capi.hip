@@ -211,11 +211,12 @@ int watch_list_counts(Ctx& c, int64_t n, hipStream_t s, hipStream_t ws)
     return 0;
 }
 
-// lists for `slices` slices: slices * kmax * n entries and slices * n counts
+// lists for `slices` slices: slices planes of list_plane(n, kmax) entries
+// (slots rounded up to tiles of 64) and slices * n counts
 int ensure_nbr(Ctx& c, int64_t n, int slices)
 {
     const int k = c.max_neighbours;
-    const int64_t need_n = n * slices;
+    const int64_t need_n = (n + 63) / 64 * 64 * slices;
     if (c.nbr && c.nbr_cap_n >= need_n && c.nbr_cap_k >= k) return 0;
     if (c.no_alloc) return fail(c, GRASPH_E_NOMEM, "the neighbour list would need a device allocation");
     if (c.nbr) (void)hipFree(c.nbr);
@@ -640,7 +641,7 @@ void carve_derivative_arrays(Ctx& sh, Carve& k, int64_t n, bool lists, bool grav
             sh.mass_flags_cap = (n + 255) / 256;
             sh.mass_flags = k.take<int>(sh.mass_flags_cap);
         }
-        sh.nbr_cap_n = n * slices;
+        sh.nbr_cap_n = (n + 63) / 64 * 64 * slices;
         sh.nbr_cap_k = sh.max_neighbours;
         sh.nbr = k.take<int>(sh.nbr_cap_n * sh.max_neighbours);
         sh.nbr_cnt = k.take<int>(sh.nbr_cap_n);

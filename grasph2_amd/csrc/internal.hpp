@@ -11,6 +11,19 @@
 
 namespace grasph {
 
+// Neighbour lists come in tiles of 64 slots: entry k of slot t lies at
+// list_entry(t, k, kmax) of its plane, so a wavefront's k-th entries are one
+// 256-byte line and a wavefront's whole list is contiguous (a plane laid out
+// as [k][slot] puts each entry of a long list in another part of a
+// gigabyte-sized array, and the page-table walks made merging and walking
+// long lists several times slower).  A plane holds list_plane(n, kmax)
+// entries.
+__host__ __device__ inline size_t list_entry(int t, int k, int kmax)
+{
+    return ((size_t)(t >> 6) * (size_t)kmax + (size_t)k) * 64 + (size_t)(t & 63);
+}
+__host__ __device__ inline size_t list_plane(int64_t n, int kmax) { return (size_t)((n + 63) / 64) * 64 * (size_t)kmax; }
+
 enum KernelClass {
     KC_SWEEP = 0,
     KC_DENSITY = 1,
@@ -74,7 +87,8 @@ struct Ctx {
     float* bbox = nullptr;  // [nblocks][6] partial bounds then final [6]
     int64_t reorder_cap = 0;
 
-    // neighbour list: nbr[k * nbr_stride + slot], nbr_cnt[slot]
+    // neighbour lists: one plane per slice (list_plane entries; entry k of
+    // slot t at list_entry(t, k, kmax)), counts nbr_cnt[plane * n + slot]
     int* nbr = nullptr;
     int* nbr_cnt = nullptr;
     int64_t nbr_cap_n = 0;

@@ -16,8 +16,8 @@
 // rec[j] = {x y z m, vx vy vz rho, sigma/rho^2 (6) balsara rho}, in slot
 // order: one neighbour costs one contiguous gather, and a particle's
 // neighbours' records are close together in memory.  Outputs go to
-// particle i of the caller's arrays.  The list is stored slot-major,
-// nbr[k * nstride + t], so a wavefront's loads of its k-th neighbours are
+// particle i of the caller's arrays.  The list is stored in tiles of 64
+// slots (list_entry), so a wavefront's loads of its k-th neighbours are
 // contiguous, and holds slot indices in increasing order.  A particle whose
 // count exceeds the list capacity falls back to a scan over its instance's
 // slots with the exact neighbour test, so a full list never loses partners.
@@ -85,13 +85,13 @@ __device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ xs
     const int cnt = Ls.cnt[t];
     if (cnt <= kmax) {
         if (cnt <= 0) return;
-        const int* list = Ls.nbr + t;
+        const int* list = Ls.nbr + list_entry(t, 0, kmax);
         int j0 = list[0];
-        int j1 = (cnt > 1) ? list[nstride] : j0;
+        int j1 = (cnt > 1) ? list[64] : j0;
         auto d0 = load(j0);
         for (int k = 0; k < cnt; ++k) {
             const auto d1 = load(j1);
-            const int j2 = (k + 2 < cnt) ? list[(size_t)(k + 2) * nstride] : j1;
+            const int j2 = (k + 2 < cnt) ? list[64 * (size_t)(k + 2)] : j1;
             body(j0, d0);
             j0 = j1;
             d0 = d1;

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(const float4* __restrict_
     const int j0 = sl * slice_len;
     int j1 = min(n, j0 + slice_len);
     if (!(GRAV || NBR || POT)) j1 = j0;  // nothing pairwise to do: skip the j loop
-    const Rare R = make_rare(nbr + (size_t)sl * kmax * n, (size_t)n, kmax, P.H2);
+    const Rare R = make_rare(nbr + (size_t)sl * list_plane(n, kmax), kmax, P.H2);
     float4* const pp = part ? part + 2 * (size_t)sl * n : nullptr;
     if constexpr (GRAV && !POT) {
         if (umass && umass[0] != 0.0f) {  // {uniform, mass} of the partners, wave-uniform
@@ -93,38 +93,74 @@ __global__ __launch_bounds__(1024) void mass_check_kernel(const float4* __restri
 
 // Concatenate the per-slice neighbour lists of a j-split sweep into slice
 // 0's list, in slice order (so slot indices stay increasing), and leave the
-// total in slice 0's count.  A total above kmax (some slice overflowed, or
-// the slices together do) marks the particle for the exact-test scan of its
-// whole instance.  Each thread moves only its own slot's column.
-__global__ __launch_bounds__(kBlock) void merge_lists_kernel(int* __restrict__ nbr, int* __restrict__ cnt,
-                                                             const int n, const int kmax, const int nslices)
+// total in slice 0's count (merge_counts_kernel, afterwards).  A total above
+// kmax (some slice overflowed, or the slices together do) marks the particle
+// for the exact-test scan of its whole instance; its list is left as it is.
+// One wavefront per tile of 64 slots and per gridDim.y-th of the rows (one
+// for the default capacity, up to 8 for long lists):
+// lane t fills row r of its merged list (r past its slice-0 entries) from
+// the slice whose entries land there, so every store is one whole 256-byte
+// row of the tile (stores of single entries into rows that other
+// wavefronts fill too left partial lines in several caches; one thread per
+// slot copying slice after slice took 2.6 ms for 1e8 entries).  Reads come
+// from slices >= 1 and writes go to rows of slice 0 past its own entries,
+// so nothing is read after it is overwritten.
+template <int kMax>  // slices, rounded up to a power of two (at most 32)
+__global__ __launch_bounds__(64) void merge_lists_kernel(int* __restrict__ nbr, const int* __restrict__ cnt,
+                                                         const int n, const int kmax, const int nslices)
+{
+    const int lane = threadIdx.x, t = blockIdx.x * 64 + lane, set = blockIdx.y, nsets = gridDim.y;
+    const size_t ns = (size_t)n;
+    int start[kMax + 1];  // rows of slice s: [start[s], start[s + 1])
+    int c0 = 0, total = 0;
+    if (t < n) {
+        c0 = cnt[t];
+        total = c0;
+    }
+    start[0] = 0;
+    start[1] = c0;
+#pragma unroll
+    for (int s = 1; s < kMax; ++s) {
+        if (s < nslices && t < n) total += cnt[(size_t)s * ns + t];
+        start[s + 1] = total;
+    }
+    const bool fits = t < n && total <= kmax;
+    int r_lo = fits ? c0 : INT32_MAX, r_hi = fits ? total : 0;
+    // wave-uniform bounds of the rows to fill
+    for (int o = 32; o > 0; o >>= 1) {
+        r_lo = min(r_lo, __shfl_xor(r_lo, o, 64));
+        r_hi = max(r_hi, __shfl_xor(r_hi, o, 64));
+    }
+    const size_t plane = list_plane(n, kmax);
+    int* const dst = nbr + list_entry(t, 0, kmax);  // lane t's column of its tile, row 0
+    for (int r0 = r_lo - r_lo % nsets + set; r0 < r_hi; r0 += 8 * nsets) {
+        int v[8];
+        bool w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int r = r0 + u * nsets;
+            w[u] = fits && r >= c0 && r < total;
+            v[u] = 0;
+            if (w[u]) {
+                int sl = 1;
+#pragma unroll
+                for (int s = 2; s < kMax; ++s)
+                    if (s < nslices && r >= start[s]) sl = s;
+                v[u] = nbr[(size_t)sl * plane + list_entry(t, r - start[sl], kmax)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (w[u]) dst[64 * (size_t)(r0 + u * nsets)] = v[u];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void merge_counts_kernel(int* __restrict__ cnt, const int n, const int nslices)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
-    const size_t ns = (size_t)n;
-    // all counts first (independent loads), then the copies
-    constexpr int kMax = 16;
-    int c[kMax];
-#pragma unroll
-    for (int s = 0; s < kMax; ++s) c[s] = (s < nslices) ? cnt[(size_t)s * ns + t] : 0;
-    int total = c[0];
-    for (int s = 1; s < nslices; ++s) {
-        const int cs = (s < kMax) ? c[s] : cnt[(size_t)s * ns + t];
-        if (total + cs <= kmax) {
-            const int* src = nbr + (size_t)s * kmax * ns;
-            int* dst = nbr + (size_t)total * ns;
-            // 8 loads in flight, then their stores (the two regions never overlap)
-            for (int k = 0; k < cs; k += 8) {
-                int v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = (k + u < cs) ? src[(size_t)(k + u) * ns + t] : 0;
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (k + u < cs) dst[(size_t)(k + u) * ns + t] = v[u];
-            }
-        }
-        total += cs;  // once above kmax it stays above: the list is then ignored
-    }
+    int total = 0;
+    for (int s = 0; s < nslices; ++s) total += cnt[(size_t)s * n + t];
     cnt[t] = total;
 }
 
@@ -198,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void batch_sweep_kernel(const float4* __res
                                                              const float* __restrict__ umass)
 {
     const int lane = threadIdx.x & 63;
-    const Rare R0 = make_rare(nbr, (size_t)ntotal, kmax, 0.0f);
+    const Rare R0 = make_rare(nbr, kmax, 0.0f);
     for (;;) {
         int w = 0;
         if (lane == 0) w = atomicAdd(queue, 1);
@@ -337,7 +373,19 @@ hipError_t launch_mass_final(const float4* posm, const int* mass_flags, int nfla
 hipError_t launch_merge_lists(int* nbr, int* cnt, int n, int kmax, int nslices, hipStream_t s)
 {
     if (n <= 0 || nslices <= 1) return hipSuccess;
-    hipLaunchKernelGGL(merge_lists_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbr, cnt, n, kmax,
+    if (nslices > 32) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)std::min(8, std::max(1, kmax / 256)));
+    if (nslices <= 2)
+        hipLaunchKernelGGL(merge_lists_kernel<2>, grid, dim3(64), 0, s, nbr, cnt, n, kmax, nslices);
+    else if (nslices <= 4)
+        hipLaunchKernelGGL(merge_lists_kernel<4>, grid, dim3(64), 0, s, nbr, cnt, n, kmax, nslices);
+    else if (nslices <= 8)
+        hipLaunchKernelGGL(merge_lists_kernel<8>, grid, dim3(64), 0, s, nbr, cnt, n, kmax, nslices);
+    else if (nslices <= 16)
+        hipLaunchKernelGGL(merge_lists_kernel<16>, grid, dim3(64), 0, s, nbr, cnt, n, kmax, nslices);
+    else
+        hipLaunchKernelGGL(merge_lists_kernel<32>, grid, dim3(64), 0, s, nbr, cnt, n, kmax, nslices);
+    hipLaunchKernelGGL(merge_counts_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cnt, n,
                        nslices);
     return hipGetLastError();
 }

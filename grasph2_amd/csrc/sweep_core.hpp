@@ -25,8 +25,7 @@ struct Acc {
 // SGPRs would, under the SGPR pressure of the j buffers, be re-read from
 // the kernel arguments, and that wait would also stall on the j prefetch.
 struct Rare {
-    int* nbr;
-    size_t nstride;
+    int* nbr;  // the list plane
     int kmax;
     float h2;
 };
@@ -43,11 +42,10 @@ __device__ __forceinline__ uint64_t vgpr_u64(uint64_t x)
     const uint32_t lo = (uint32_t)vgpr_i32((int)(uint32_t)x), hi = (uint32_t)vgpr_i32((int)(uint32_t)(x >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ Rare make_rare(int* nbr, size_t nstride, int kmax, float h2)
+__device__ __forceinline__ Rare make_rare(int* nbr, int kmax, float h2)
 {
     Rare r;
     r.nbr = (int*)vgpr_u64((uint64_t)nbr);
-    r.nstride = (size_t)vgpr_u64((uint64_t)nstride);
     r.kmax = vgpr_i32(kmax);
     r.h2 = vgpr_f32(h2);
     return r;
@@ -64,7 +62,7 @@ __device__ __forceinline__ void neighbour_pair(const float4 pj, const int j, con
     const float r2 = dist2_exact(xi - pj.x, yi - pj.y, zi - pj.z);
     if (POT && r2 == 0.0f) phi -= pj.w * __builtin_amdgcn_rsqf(d2);  // the self pair was added on the fast path
     if (NBR && is_neighbour(r2, R.h2)) {
-        if (cnt < R.kmax) R.nbr[(size_t)cnt * R.nstride + slot] = j;
+        if (cnt < R.kmax) R.nbr[list_entry(slot, cnt, R.kmax)] = j;
         ++cnt;
     }
 }
@@ -100,7 +98,7 @@ __device__ __forceinline__ void chunk(const float4 (&pj)[kChunk], const int jb, 
 #pragma unroll
                 for (int q = 0; q < NI; ++q)
                     if (is_neighbour(r2[q][c], R.h2)) {
-                        if (a.cnt[q] < R.kmax) R.nbr[(size_t)a.cnt[q] * R.nstride + slot + 64 * q] = jb + c;
+                        if (a.cnt[q] < R.kmax) R.nbr[list_entry(slot + 64 * q, a.cnt[q], R.kmax)] = jb + c;
                         ++a.cnt[q];
                     }
         }

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
     if (I >= nb) return;
     // share blockIdx.y of gridDim.y: its list, count and partial planes
     const int sl = blockIdx.y, ns = gridDim.y;
-    const Rare R = make_rare(nbr + (size_t)sl * kmax * n, (size_t)n, kmax, P.H2);
+    const Rare R = make_rare(nbr + (size_t)sl * list_plane(n, kmax), kmax, P.H2);
     int* const cnt = nbr_cnt + (size_t)sl * n;
     float4* const part = near_part + 2 * (size_t)sl * n;
     if constexpr (GRAV) {
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
     const SymInst S = sinst[k];
     const int I = (first - S.off) >> 6;
     const DevParams P = prm[k];
-    const Rare R = make_rare(nbr, (size_t)ntotal, kmax, P.H2);
+    const Rare R = make_rare(nbr, kmax, P.H2);
     const uint32_t* b = NBR ? bits + S.bits0 : nullptr;
     const float4* bx = NBR ? box + 2 * (size_t)S.box0 : nullptr;
     if constexpr (GRAV) {
