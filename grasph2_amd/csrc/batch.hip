@@ -266,8 +266,9 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
     if (std::memcmp(b.tuned_ni, c.batch_ni, sizeof b.tuned_ni) != 0 || !b.d_items || b.items_sym_min != want_sym_min)
         if ((r = build_items(c, b))) return r;
     // lists and records
-    const int64_t need = (int64_t)list_plane(b.ntotal, c.max_neighbours);
-    if (!b.d_nbr || b.nbr_cap < need || b.kmax != c.max_neighbours) {
+    // the capacity set on the context, not one its single-instance calls grew to
+    const int64_t need = (int64_t)list_plane(b.ntotal, c.max_neighbours_set);
+    if (!b.d_nbr || b.nbr_cap < need || b.kmax != c.max_neighbours_set) {
         free_ptr(b.d_nbr);
         free_ptr(b.d_cnt);
         b.d_nbr = nullptr;
@@ -280,7 +281,7 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
             return fail(c, GRASPH_E_NOMEM, "device allocation of the batch neighbour lists failed");
         }
         b.nbr_cap = need;
-        b.kmax = c.max_neighbours;
+        b.kmax = c.max_neighbours_set;
     }
     if (b.rec_cap < b.ntotal || !b.d_rec) {
         drop_graph(b);
@@ -874,7 +875,7 @@ int grasph_batch_step_loop(grasph_ctx* ctx, const grasph_particles* p, const gra
             if (grasph_create(c.device, &B.loop[k]) != 0) return grasph::fail(c, GRASPH_E_HIP, grasph_last_error(nullptr));
         }
         Ctx& s = B.loop[k]->c;
-        s.max_neighbours = c.max_neighbours;
+        s.max_neighbours = s.max_neighbours_set = c.max_neighbours_set;
         if (s.reorder_interval != c.reorder_interval) {
             s.reorder_interval = c.reorder_interval;
             s.perm_n = 0;

@@ -853,7 +853,7 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     const std::string k(key);
     if (k == "max_neighbours") {
         if (value < 1 || value > 4096) return grasph::fail(c, GRASPH_E_INVALID, "max_neighbours must be in [1, 4096]");
-        c.max_neighbours = (int)value;
+        c.max_neighbours = c.max_neighbours_set = (int)value;
     } else if (k == "max_neighbours_grow") {
         if (value != 0 && value != 1) return grasph::fail(c, GRASPH_E_INVALID, "max_neighbours_grow must be 0 or 1");
         c.max_neighbours_grow = (int)value;

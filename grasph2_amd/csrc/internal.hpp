@@ -41,7 +41,8 @@ struct Ctx {
     std::string err;
 
     // options
-    int max_neighbours = 160;
+    int max_neighbours = 160;      // list capacity of this context's single-instance evaluations (grows)
+    int max_neighbours_set = 160;  // the capacity last set: batches and their launch-loop contexts start from it
     // grow max_neighbours after an evaluation whose lists overflowed: its
     // largest count is reduced on the device and copied to pinned host
     // memory; a later call reads it once the copy has landed (no wait)

@@ -73,10 +73,14 @@ struct Lists {
 // Walk slot t's neighbours: the list, or when it overflowed, a scan of the
 // instance's slots with the exact test (same partners, same order).
 // load(j) fetches what body(j, data) needs; the walk is software-pipelined
-// (partner k+1's data and k+2's index are in flight while k is computed),
-// so a lane keeps two gathers outstanding instead of waiting on each.  The
-// look-ahead past the end reloads the last partner (always in range).
-template <typename L, typename F>
+// D deep: while partner e is computed, the data of partners e+1 .. e+D-1
+// and the indices up to e+2D-1 are in flight (slot u of the rings holds
+// the partners e with e mod D = u).  The look-ahead past the end reloads
+// the last partner (always in range).  Deeper rings hide more of the gather
+// latency where few wavefronts share a SIMD (one instance: 2048 wavefronts
+// at 131072 particles), shallower ones keep the registers low where many
+// do (batches).
+template <int D, typename L, typename F>
 __device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ xs, const Lists& Ls,
                                                    const size_t nstride, const int t, const int kmax,
                                                    const float4 pi, const SlotMap& m, const float H2, L&& load,
@@ -86,16 +90,24 @@ __device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ xs
     if (cnt <= kmax) {
         if (cnt <= 0) return;
         const int* list = Ls.nbr + list_entry(t, 0, kmax);
-        int j0 = list[0];
-        int j1 = (cnt > 1) ? list[64] : j0;
-        auto d0 = load(j0);
-        for (int k = 0; k < cnt; ++k) {
-            const auto d1 = load(j1);
-            const int j2 = (k + 2 < cnt) ? list[64 * (size_t)(k + 2)] : j1;
-            body(j0, d0);
-            j0 = j1;
-            d0 = d1;
-            j1 = j2;
+        const int last = cnt - 1;
+        int jcur[D], jnx[D];
+        decltype(load(0)) dd[D];
+#pragma unroll
+        for (int u = 0; u < D; ++u) jcur[u] = list[64 * (size_t)min(u, last)];
+#pragma unroll
+        for (int u = 0; u < D; ++u) jnx[u] = list[64 * (size_t)min(u + D, last)];
+#pragma unroll
+        for (int u = 0; u < D; ++u) dd[u] = load(jcur[u]);
+        for (int k = 0; k < cnt; k += D) {
+#pragma unroll
+            for (int u = 0; u < D; ++u) {
+                const int e = k + u;
+                if (e < cnt) body(jcur[u], dd[u]);
+                jcur[u] = jnx[u];
+                dd[u] = load(jcur[u]);  // partner e + D (its index came D partners ago)
+                jnx[u] = list[64 * (size_t)min(e + 2 * D, last)];
+            }
         }
     } else {
         for (int j = m.off; j < m.off + m.n; ++j) {
@@ -124,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
     const int i = perm ? perm[t] : t;
     const float4 pi = xs[t];  // positions from the compact slot-ordered array (16 B per partner)
     CSum rho{0.0f, 0.0f};
-    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return xs[j]; },
+    for_each_neighbour<BATCH ? 2 : 4>(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return xs[j]; },
                        [&](int, const float4 pj) {
         rho.add(pj.w * kernel_w_f64(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, P));
     });
@@ -157,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     double rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     double divv = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
 
-    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return PV{rec[j].pm, rec[j].vr}; },
+    for_each_neighbour<BATCH ? 2 : 4>(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return PV{rec[j].pm, rec[j].vr}; },
                        [&](int, const PV& q) {
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    for_each_neighbour(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return rec[j]; },
+    for_each_neighbour<BATCH ? 2 : 4>(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return rec[j]; },
                        [&](int, const SlotRec& q) {
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;

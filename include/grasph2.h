@@ -147,10 +147,11 @@ int grasph_destroy(grasph_ctx* ctx);
 /* Tuning knobs.  Keys: "max_neighbours" (list capacity per particle,
  * default 160; a particle with more neighbours takes an exact scan of all
  * partners, with the same results), "max_neighbours_grow" (1 = after an
- * evaluation whose lists overflowed, a later one raises max_neighbours to
- * 1.25 times the largest count seen, in steps of 32, up to 4096 and 16 GiB
- * of lists; the count travels to the host behind an event and is read
- * without waiting, and not during graph capture; the default; 0 = fixed),
+ * single-instance evaluation whose lists overflowed, a later one raises
+ * max_neighbours to 1.25 times the largest count seen, in steps of 32, up
+ * to 4096 and 16 GiB of lists; the count travels to the host behind an
+ * event and is read without waiting, and not during graph capture; the
+ * default; 0 = fixed; batches keep the capacity last set),
  * "reorder_interval"
  * (Morton re-sort every k derivative evaluations, default 8; 0 = never;
  * the order changes only the order of compensated sums, not which pairs
