@@ -750,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
     int* __restrict__ nbr_cnt, const int kmax, const int ntotal, float4* __restrict__ near_part,
     const DevParams* __restrict__ prm, const float* __restrict__ umass)
 {
-    const int4 bl = blk[blockIdx.x];
+    const int4 bl = blk[xcd_block((int)blockIdx.x, (int)gridDim.x)];  // XCD-contiguous (xcd_block)
     const int lane = threadIdx.x & 63;
     const int first = __builtin_amdgcn_readfirstlane(bl.y + 64 * (int)(threadIdx.x >> 6));
     if (first >= bl.z) return;
