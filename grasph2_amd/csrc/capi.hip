@@ -314,7 +314,7 @@ int sym_near_shares(const Ctx& c, int64_t n)
 int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s, SymLaunch& S,
                     hipStream_t& ws)
 {
-    const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.nslices);
+    const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.near_shares);
     int r;
     if ((r = grow(c, c.sym_box, c.sym_box_cap, G.box_elems, "symmetric sweep boxes")) ||
         (r = grow(c, c.sym_bits, c.sym_bits_cap, G.bits_elems, "symmetric sweep near bits")) ||
@@ -329,7 +329,8 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
     S.grav = L.grav;
     S.want_nbr = L.want_nbr;
     S.finish = L.finish;
-    S.nshares = L.nslices;
+    S.nshares = L.near_shares;
+    S.list_shares = L.nslices;
     S.acc = L.acc;
     S.nbr = L.nbr;
     S.nbr_cnt = L.nbr_cnt;
@@ -488,7 +489,15 @@ int derivatives(Ctx& c, const grasph_particles* p, const grasph_params* prm, voi
     const bool sym = sym_on && (sweep_grav || need_list) && sym_fits(c, n, need_list);
     int slices = 1, slice_len = (int)n;
     if ((sweep_grav || need_list) && !sym) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
-    if (sym) slices = sym_near_shares(c, n);  // the near field's shares take the slices' list planes
+    int near_shares = 1;
+    if (sym) {
+        // the near field's list shares take the slices' list planes; with
+        // long lists (after a collision's impact) one share: the merge of
+        // shares would move every entry again (1e8 entries: 1.4 ms), and
+        // the one list kernel still ends inside the far field's window
+        near_shares = sym_near_shares(c, n);
+        slices = (int64_t)n * c.max_neighbours >= (int64_t(1) << 26) ? 1 : near_shares;
+    }
     if (need_list) {
         grow_list_capacity(c, n, slices, s);
         if ((r = ensure_nbr(c, n, slices))) return r;
@@ -530,6 +539,7 @@ int derivatives(Ctx& c, const grasph_particles* p, const grasph_params* prm, voi
     L.want_pot = false;
     L.finish = accel_pass && !sph;
     L.nslices = slices;
+    L.near_shares = near_shares;
     L.slice_len = slice_len;
     L.part = c.part;
     if (sweep_grav) {

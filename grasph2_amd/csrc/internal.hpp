@@ -180,7 +180,9 @@ struct SweepLaunch {
     float4* xvel;
     float* maxvsig;
     int ni;             // i particles per lane (1 or 2)
-    int nslices;        // j-split: slices of slice_len j (lists and counts per slice)
+    int nslices;        // j-split: slices of slice_len j (lists and counts per slice); the
+                        // symmetric sweep: its near field's list shares (planes)
+    int near_shares;    // the symmetric sweep: shares of its in-group gravity
     int slice_len;
     float4* part;       // [nslices][n][2] partial sums when nslices > 1
     const float* umass; // {uniform, mass} from launch_mass_check, or null
@@ -213,6 +215,7 @@ struct SymLaunch {
     bool add_acc;       // acc already holds the SPH terms: the combine adds the gravity to them
     bool list_prio;     // the lists' wavefronts take issue priority over the far field's beside them
     bool ingroup_done;  // the in-group gravity is already enqueued
+    int list_shares;    // its list kernel cut into this many shares (own list planes, merged after)
     int nshares;        // near field cut into this many shares per block (own list planes, merged after)
     float4* acc;
     int* nbr;           // a list per slot and share

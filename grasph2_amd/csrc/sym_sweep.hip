@@ -927,14 +927,16 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
 hipError_t launch_sym_lists(const SymLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0 || !L.want_nbr) return hipSuccess;
-    if (L.nshares < 1) return hipErrorInvalidValue;
-    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
+    if (L.list_shares < 1) return hipErrorInvalidValue;
+    SymLaunch M = L;
+    M.nshares = L.list_shares;  // the list kernel's shares (list planes), not the in-group gravity's
+    const SymGeom G = sym_geometry(M.n, M.wgs, M.want_nbr, M.nshares);
     hipLaunchKernelGGL(block_box_kernel, dim3((G.nb + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, L.xs, L.n,
                        G.nb, L.box);
     const int64_t words = (int64_t)G.nb * G.nwords;
     hipLaunchKernelGGL(near_bits_kernel, dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, L.box,
                        G.nb, G.nwords, P.H2, L.bits);
-    if (P.H2 > 0.0f) launch_near<false, true, true>(L, G, P, s); else launch_near<false, true, false>(L, G, P, s);
+    if (P.H2 > 0.0f) launch_near<false, true, true>(M, G, P, s); else launch_near<false, true, false>(M, G, P, s);
     return hipGetLastError();
 }
 
