@@ -264,6 +264,9 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
     L.nbr = c.nbr;
     L.nbr_cnt = c.nbr_cnt;
     L.n = (int)p->n;
+    // long lists (a capacity grown past 512 after contact): 4 lanes per
+    // particle, 4 times the wavefronts to hide the gathers' latency
+    L.sub = c.max_neighbours > 512 ? 4 : 1;
     L.kmax = c.max_neighbours;
     L.balsara = p->balsara;
     L.density_dt = p->density_dt;

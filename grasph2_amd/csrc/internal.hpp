@@ -356,7 +356,7 @@ struct SlotMap {
 };
 
 template <bool BATCH>
-__device__ inline SlotMap slot_map(const BlockTables& T, int n)
+__device__ inline SlotMap slot_map(const BlockTables& T, int n, int slots_per_block = 256)
 {
     SlotMap m;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -370,7 +370,7 @@ __device__ inline SlotMap slot_map(const BlockTables& T, int n)
         m.n = I.n;
     } else {
         m.inst = 0;
-        m.slot0 = blk * 256;
+        m.slot0 = blk * slots_per_block;
         m.slot_end = n;
         m.off = 0;
         m.n = n;
@@ -448,6 +448,7 @@ struct NbrLaunch {
     float4* xvel;
     float* maxvsig;
     bool acc_has_gravity;
+    int sub;        // lanes per particle in the passes (one instance only): 1 or 4
     BlockTables T;  // batch tables (T.nblocks == 0 for one instance)
 };
 hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P, hipStream_t s);

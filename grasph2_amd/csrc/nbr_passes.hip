@@ -80,48 +80,93 @@ struct Lists {
 // latency where few wavefronts share a SIMD (one instance: 2048 wavefronts
 // at 131072 particles), shallower ones keep the registers low where many
 // do (batches).
-template <int D, typename L, typename F>
+template <int D, int SUB, typename L, typename F>
 __device__ __forceinline__ void for_each_neighbour(const float4* __restrict__ xs, const Lists& Ls,
                                                    const size_t nstride, const int t, const int kmax,
-                                                   const float4 pi, const SlotMap& m, const float H2, L&& load,
-                                                   F&& body)
+                                                   const float4 pi, const SlotMap& m, const float H2, const int sub,
+                                                   L&& load, F&& body)
 {
     const int cnt = Ls.cnt[t];
     if (cnt <= kmax) {
-        if (cnt <= 0) return;
-        const int* list = Ls.nbr + list_entry(t, 0, kmax);
-        const int last = cnt - 1;
+        // this lane's share of the list: entries sub, sub + SUB, ...
+        const int cl = cnt > sub ? (cnt - 1 - sub) / SUB + 1 : 0;
+        if (cl <= 0) return;
+        const int* list = Ls.nbr + list_entry(t, sub, kmax);
+        constexpr size_t kStride = 64 * SUB;
+        const int last = cl - 1;
         int jcur[D], jnx[D];
         decltype(load(0)) dd[D];
 #pragma unroll
-        for (int u = 0; u < D; ++u) jcur[u] = list[64 * (size_t)min(u, last)];
+        for (int u = 0; u < D; ++u) jcur[u] = list[kStride * min(u, last)];
 #pragma unroll
-        for (int u = 0; u < D; ++u) jnx[u] = list[64 * (size_t)min(u + D, last)];
+        for (int u = 0; u < D; ++u) jnx[u] = list[kStride * min(u + D, last)];
 #pragma unroll
         for (int u = 0; u < D; ++u) dd[u] = load(jcur[u]);
-        for (int k = 0; k < cnt; k += D) {
+        for (int k = 0; k < cl; k += D) {
 #pragma unroll
             for (int u = 0; u < D; ++u) {
                 const int e = k + u;
-                if (e < cnt) body(jcur[u], dd[u]);
+                if (e < cl) body(jcur[u], dd[u]);
                 jcur[u] = jnx[u];
                 dd[u] = load(jcur[u]);  // partner e + D (its index came D partners ago)
-                jnx[u] = list[64 * (size_t)min(e + 2 * D, last)];
+                jnx[u] = list[kStride * min(e + 2 * D, last)];
             }
         }
     } else {
+        // the same share of the partners in slot order: the partner count
+        // picks the lane, so the sums split as over the list
+        int idx = 0;
         for (int j = m.off; j < m.off + m.n; ++j) {
             const float4 pj = xs[j];
-            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) body(j, load(j));
+            if (is_neighbour(dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z), H2)) {
+                if (SUB == 1 || idx % SUB == sub) body(j, load(j));
+                ++idx;
+            }
         }
     }
+}
+
+// SUB lanes share a particle when its list is long (one instance, SUB = 4):
+// lane l of a wavefront serves particle l mod (64 / SUB) of the wavefront's
+// 64 / SUB and walks its neighbours sub, sub + SUB, ... (sub = l div (64 /
+// SUB)); the lanes' sums are then added in a fixed order (the lane with
+// sub = 0 adds sub 1's, then the sum of subs 2 and 3) and that lane writes.
+template <int SUB>
+__device__ __forceinline__ int pass_sub() { return (int)(threadIdx.x & 63) / (64 / SUB); }
+template <int SUB>
+__device__ __forceinline__ int pass_slot(const SlotMap& m)
+{
+    constexpr int kPPW = 64 / SUB;
+    return m.slot0 + (int)(threadIdx.x >> 6) * kPPW + (int)(threadIdx.x & 63) % kPPW;
+}
+template <int SUB>
+__device__ __forceinline__ void merge_subs(CSum& a)
+{
+#pragma unroll
+    for (int o = 64 / SUB; o < 64; o <<= 1) {
+        const float s = __shfl_xor(a.s, o, 64), c = __shfl_xor(a.c, o, 64);
+        a.add(s);
+        a.c += c;
+    }
+}
+template <int SUB, typename T>
+__device__ __forceinline__ void sum_subs(T& a)
+{
+#pragma unroll
+    for (int o = 64 / SUB; o < 64; o <<= 1) a = a + __shfl_xor(a, o, 64);
+}
+template <int SUB>
+__device__ __forceinline__ void max_subs(float& a)
+{
+#pragma unroll
+    for (int o = 64 / SUB; o < 64; o <<= 1) a = fmaxf(a, __shfl_xor(a, o, 64));
 }
 
 struct PV {  // position/mass and velocity/density of a partner
     float4 pm, vr;
 };
 
-template <bool BATCH>
+template <bool BATCH, int SUB>
 __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restrict__ xs, SlotRec* __restrict__ rec,
                                                          const int* __restrict__ perm,
                                                          const Lists Ls,
@@ -129,32 +174,41 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
                                                          float* __restrict__ density, const BlockTables T,
                                                          const DevParams P0)
 {
-    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
-    const int t = m.slot0 + threadIdx.x;
-    if (t >= m.slot_end) return;
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
+    const int t = pass_slot<SUB>(m), sub = pass_sub<SUB>();
+    const bool act = t < m.slot_end;
+    if (SUB == 1 && !act) return;
+    const int tc = act ? t : m.slot0;  // lanes past the end take part in the lanes' sums only
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
-    const int i = perm ? perm[t] : t;
-    const float4 pi = xs[t];  // positions from the compact slot-ordered array (16 B per partner)
+    const int i = perm ? perm[tc] : tc;
+    const float4 pi = xs[tc];  // positions from the compact slot-ordered array (16 B per partner)
     CSum rho{0.0f, 0.0f};
-    for_each_neighbour<BATCH ? 2 : 4>(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return xs[j]; },
-                       [&](int, const float4 pj) {
-        rho.add(pj.w * kernel_w_f64(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, P));
-    });
+    if (act)
+        for_each_neighbour<SUB == 1 && !BATCH ? 4 : 2, SUB>(xs, Ls, nstride, t, kmax, pi, m, P.H2, sub,
+                                                           [&](int j) { return xs[j]; }, [&](int, const float4 pj) {
+            rho.add(pj.w * kernel_w_f64(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, P));
+        });
+    if (SUB > 1) {
+        merge_subs<SUB>(rho);
+        if (!act || sub != 0) return;
+    }
     float r = rho.value();
     if ((P.flags & GRASPH_DEAL_WITH_NO_PARTNERS) && r < 0.001f) r = P.rho0 * 0.01f;  // :85-88
     density[i] = r;
     rec[t].vr.w = r;  // the strain / record passes read densities from the slot records
 }
 
-template <bool BATCH, bool SOLIDS, bool BALS, bool INTD>
+template <bool BATCH, int SUB, bool SOLIDS, bool BALS, bool INTD>
 __global__ __launch_bounds__(kBlock) void strain_kernel(
     const float4* __restrict__ xs, SlotRec* __restrict__ rec, const float* __restrict__ dstress,
     const int* __restrict__ perm, const Lists Ls, const size_t nstride, const int kmax, float* __restrict__ balsara,
     float* __restrict__ density_dt, float* __restrict__ dstress_dt, const BlockTables T, const DevParams P0)
 {
-    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
-    const int t = m.slot0 + threadIdx.x;
-    if (t >= m.slot_end) return;
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
+    const int t0 = pass_slot<SUB>(m), sub = pass_sub<SUB>();
+    const bool act = t0 < m.slot_end;
+    if (SUB == 1 && !act) return;
+    const int t = act ? t0 : m.slot0;  // lanes past the end take part in the lanes' sums only
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
     const float4 pi = rec[t].pm;
@@ -169,8 +223,10 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     double rxy = 0, rxz = 0, ryz = 0;                              // rotation rate (antisymmetric)
     double divv = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
 
-    for_each_neighbour<BATCH ? 2 : 4>(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return PV{rec[j].pm, rec[j].vr}; },
-                       [&](int, const PV& q) {
+    if (act)
+    for_each_neighbour<SUB == 1 && !BATCH ? 4 : 2, SUB>(xs, Ls, nstride, t, kmax, pi, m, P.H2, sub,
+                                                       [&](int j) { return PV{rec[j].pm, rec[j].vr}; },
+                                                       [&](int, const PV& q) {
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const double s = dw_over_r_f64(rx, ry, rz, P);
@@ -198,6 +254,13 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
         }
         if (BALS || INTD) divv -= vol * (vx * gx + vy * gy + vz * gz);  // :137
     });
+    if (SUB > 1) {
+        sum_subs<SUB>(exx); sum_subs<SUB>(eyy); sum_subs<SUB>(ezz);
+        sum_subs<SUB>(exy); sum_subs<SUB>(exz); sum_subs<SUB>(eyz);
+        sum_subs<SUB>(rxy); sum_subs<SUB>(rxz); sum_subs<SUB>(ryz);
+        sum_subs<SUB>(divv); sum_subs<SUB>(cx); sum_subs<SUB>(cy); sum_subs<SUB>(cz);
+        if (!act || sub != 0) return;
+    }
 
     // do_after, computeDerivatives.h:144-161
     Tens S = sym_to_full(0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
@@ -272,16 +335,18 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SlotRec* __restrict__ rec,
     }
 }
 
-template <bool BATCH, bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
+template <bool BATCH, int SUB, bool SOLIDS, bool BALS, bool AV, bool XSPH, bool ARTS>
 __global__ __launch_bounds__(kBlock) void accel_kernel(
     const float4* __restrict__ xs, const SlotRec* __restrict__ rec, const int* __restrict__ perm, const Lists Ls,
     const size_t nstride,
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
     const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
-    const SlotMap m = slot_map<BATCH>(T, (int)nstride);
-    const int t = m.slot0 + threadIdx.x;
-    if (t >= m.slot_end) return;
+    const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
+    const int t0 = pass_slot<SUB>(m), sub = pass_sub<SUB>();
+    const bool act = t0 < m.slot_end;
+    if (SUB == 1 && !act) return;
+    const int t = act ? t0 : m.slot0;  // lanes past the end take part in the lanes' sums only
     const DevParams P = BATCH ? T.prm[m.inst] : P0;
     const int i = perm ? perm[t] : t;
     const float4 pi = rec[t].pm;
@@ -293,15 +358,17 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     const float arts_i = (ARTS && !SOLIDS) ? ((sxx > 0) ? (-P.mateps * sxx) : 0.0f) : 0.0f;
 
     CSum ax{0.0f, 0.0f}, ay{0.0f, 0.0f}, az{0.0f, 0.0f};
-    if (acc_has_gravity) {
+    if (acc_has_gravity && sub == 0) {
         const float4 g = acc[i];
         ax.s = g.x; ay.s = g.y; az.s = g.z;
     }
     float mv = 0.0f;
     float xvx = 0.0f, xvy = 0.0f, xvz = 0.0f;
 
-    for_each_neighbour<BATCH ? 2 : 4>(xs, Ls, nstride, t, kmax, pi, m, P.H2, [&](int j) { return rec[j]; },
-                       [&](int, const SlotRec& q) {
+    if (act)
+    for_each_neighbour<SUB == 1 && !BATCH ? 4 : 2, SUB>(xs, Ls, nstride, t, kmax, pi, m, P.H2, sub,
+                                                       [&](int j) { return rec[j]; },
+                                                       [&](int, const SlotRec& q) {
         const float4 pj = q.pm;
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         const float r2 = dist2_exact(rx, ry, rz);
@@ -350,6 +417,12 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
             xvz += f * (vj.z - vi.z) * w;
         }
     });
+    if (SUB > 1) {
+        merge_subs<SUB>(ax); merge_subs<SUB>(ay); merge_subs<SUB>(az);
+        if (BALS && AV) max_subs<SUB>(mv);
+        if (XSPH) { sum_subs<SUB>(xvx); sum_subs<SUB>(xvy); sum_subs<SUB>(xvz); }
+        if (!act || sub != 0) return;
+    }
 
     float fx = ax.value(), fy = ay.value(), fz = az.value();
     environment_acceleration(P, fx, fy, fz);
@@ -382,18 +455,24 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const float4* __restrict
 
 inline Lists lists_of(const NbrLaunch& L) { return Lists{L.nbr, L.nbr_cnt}; }
 
-inline dim3 grid_of(const NbrLaunch& L)
+inline int sub_of(const NbrLaunch& L) { return L.T.nblocks > 0 || L.sub <= 1 ? 1 : 4; }
+inline dim3 grid_plain(const NbrLaunch& L)  // per-particle kernels without a list walk
 {
     return dim3(L.T.nblocks > 0 ? L.T.nblocks : (L.n + kBlock - 1) / kBlock);
 }
+inline dim3 grid_of(const NbrLaunch& L)
+{
+    const int per = kBlock / sub_of(L);  // particles per block
+    return dim3(L.T.nblocks > 0 ? L.T.nblocks : (L.n + per - 1) / per);
+}
 
-template <bool BATCH>
+template <bool BATCH, int SUB>
 hipError_t strain_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
                in = P.flags & GRASPH_INTEGRATE_DENSITY;
 #define GRASPH_STRAIN(A, B, C)                                                                                     \
-    hipLaunchKernelGGL((strain_kernel<BATCH, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.dstress,    \
+    hipLaunchKernelGGL((strain_kernel<BATCH, SUB, A, B, C>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.dstress, \
                        L.perm, lists_of(L), (size_t)L.n, L.kmax, L.balsara, L.density_dt, L.dstress_dt, L.T, P)
     const int key = (so ? 4 : 0) | (ba ? 2 : 0) | (in ? 1 : 0);
     switch (key) {
@@ -410,14 +489,15 @@ hipError_t strain_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
     return hipGetLastError();
 }
 
-template <bool BATCH>
+template <bool BATCH, int SUB>
 hipError_t accel_b(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     const bool so = P.flags & GRASPH_SOLIDS, ba = P.flags & GRASPH_BALSARA_SWITCH,
                av = P.flags & GRASPH_ARTIFICIAL_VISCOSITY, xs = P.flags & GRASPH_XSPH,
                ar = P.flags & GRASPH_ARTIFICIAL_STRESS;
 #define GRASPH_ACCEL(A, B, C, D, E)                                                                                  \
-    hipLaunchKernelGGL((accel_kernel<BATCH, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm,   \
+    hipLaunchKernelGGL((accel_kernel<BATCH, SUB, A, B, C, D, E>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec,    \
+                       L.perm,                                                                                     \
                        lists_of(L), (size_t)L.n, L.kmax, L.acc, L.xvel, L.maxvsig, L.acc_has_gravity ? 1 : 0,  \
                        L.T, P)
     // SOLIDS + ARTIFICIAL_STRESS is rejected before launch (the reference does not compile it).
@@ -459,28 +539,32 @@ hipError_t launch_density(const NbrLaunch& L, float* density, const DevParams& P
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(density_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm, lists_of(L),
+        hipLaunchKernelGGL((density_kernel<true, 1>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm, lists_of(L),
                            (size_t)L.n, L.kmax, density, L.T, P);
+    else if (sub_of(L) == 4)
+        hipLaunchKernelGGL((density_kernel<false, 4>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm,
+                           lists_of(L), (size_t)L.n, L.kmax, density, L.T, P);
     else
-        hipLaunchKernelGGL(density_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm, lists_of(L),
-                           (size_t)L.n, L.kmax, density, L.T, P);
+        hipLaunchKernelGGL((density_kernel<false, 1>), grid_of(L), dim3(kBlock), 0, s, L.xs, L.rec, L.perm,
+                           lists_of(L), (size_t)L.n, L.kmax, density, L.T, P);
     return hipGetLastError();
 }
 
 hipError_t launch_strain(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
-    return L.T.nblocks > 0 ? strain_b<true>(L, P, s) : strain_b<false>(L, P, s);
+    if (L.T.nblocks > 0) return strain_b<true, 1>(L, P, s);
+    return sub_of(L) == 4 ? strain_b<false, 4>(L, P, s) : strain_b<false, 1>(L, P, s);
 }
 
 hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(prep_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.balsara_in, L.perm,
+        hipLaunchKernelGGL(prep_kernel<true>, grid_plain(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.balsara_in, L.perm,
                            (size_t)L.n, L.T, P);
     else
-        hipLaunchKernelGGL(prep_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.balsara_in,
+        hipLaunchKernelGGL(prep_kernel<false>, grid_plain(L), dim3(kBlock), 0, s, L.rec, L.dstress, L.balsara_in,
                            L.perm, (size_t)L.n, L.T, P);
     return hipGetLastError();
 }
@@ -488,17 +572,18 @@ hipError_t launch_prep(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 hipError_t launch_accel(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
-    return L.T.nblocks > 0 ? accel_b<true>(L, P, s) : accel_b<false>(L, P, s);
+    if (L.T.nblocks > 0) return accel_b<true, 1>(L, P, s);
+    return sub_of(L) == 4 ? accel_b<false, 4>(L, P, s) : accel_b<false, 1>(L, P, s);
 }
 
 hipError_t launch_finish(const NbrLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0) return hipSuccess;
     if (L.T.nblocks > 0)
-        hipLaunchKernelGGL(finish_kernel<true>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.acc, L.xvel,
+        hipLaunchKernelGGL(finish_kernel<true>, grid_plain(L), dim3(kBlock), 0, s, L.posm, L.vel, L.acc, L.xvel,
                            L.maxvsig, L.n, L.T, P);
     else
-        hipLaunchKernelGGL(finish_kernel<false>, grid_of(L), dim3(kBlock), 0, s, L.posm, L.vel, L.acc, L.xvel,
+        hipLaunchKernelGGL(finish_kernel<false>, grid_plain(L), dim3(kBlock), 0, s, L.posm, L.vel, L.acc, L.xvel,
                            L.maxvsig, L.n, L.T, P);
     return hipGetLastError();
 }

@@ -303,6 +303,38 @@ def test_list_capacity_grows_after_overflow(gpu_ctx):
         ctx.close()
 
 
+def test_long_lists_four_lanes_per_particle():
+    """A list capacity above 512 (grown after a collision's impact) lets four
+    lanes share each particle's list.  Results: within FP32 rounding of the
+    one-lane walk and of the FP64 oracle, and the same bits whether a
+    particle's list overflowed into the exact scan or not (the scan splits
+    its partners over the lanes as the list does)."""
+    h, prm = solid_case(2500, seed=61)
+    prm = prm.copy()
+    prm.H *= 2.4  # several hundred to ~1000 neighbours per particle
+    runs = {}
+    for k in (4096, 640, 512):
+        ctx = g.Context(0)
+        try:
+            ctx.set_option("max_neighbours_grow", 0)
+            ctx.set_option("max_neighbours", k)
+            runs[k] = gpu_run(ctx, h, prm)
+            runs[k]["stats"] = ctx.neighbour_stats()
+        finally:
+            ctx.close()
+    pairs, mxn, over = runs[640]["stats"]
+    print("[parity] long lists: %d pairs, largest %d, %d overflowed at 640" % (pairs, mxn, over), flush=True)
+    assert mxn > 640 and over > 0 and runs[4096]["stats"][2] == 0
+    for f in ("acc", "density", "dstress_dt", "balsara", "maxvsig"):
+        assert np.array_equal(runs[640][f], runs[4096][f]), f  # list walk and scan agree bit for bit
+    mx, nr = vec_errors(runs[4096]["acc"], runs[512]["acc"])
+    assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)  # four lanes against one: FP32 rounding
+    ref = O.compute_derivatives(prm, h, "f64")
+    check_vec(runs[4096]["acc"], ref["acc"])
+    dmx, dnr = scalar_errors(runs[4096]["density"], ref["density"])
+    assert dnr <= TOL and dmx <= TOL, (dmx, dnr)
+
+
 def test_leapfrog_fp64_drift(gpu_ctx):
     h, prm = solid_case(2000, seed=51)
     prm.flags |= g.INTEGRATE_DENSITY
