@@ -146,7 +146,9 @@ int grasph_destroy(grasph_ctx* ctx);
 
 /* Tuning knobs.  Keys: "max_neighbours" (list capacity per particle,
  * default 160; a particle with more neighbours takes an exact scan of all
- * partners, with the same results), "max_neighbours_grow" (1 = after an
+ * partners, with the same results; above 512, four lanes share each
+ * particle's neighbours in the single-instance passes, which splits their
+ * sums four ways: equal to FP32 rounding), "max_neighbours_grow" (1 = after an
  * single-instance evaluation whose lists overflowed, a later one raises
  * max_neighbours to 1.25 times the largest count seen, in steps of 32, up
  * to 4096 and 16 GiB of lists; the count travels to the host behind an
