@@ -58,6 +58,7 @@ struct BatchState {
     int n_sblk = 0;
     int64_t sym_min = -1;    // the size threshold the tables were built with (-1: none)
     int sym_max_words = 0;
+    bool sym_plain = true;  // SymBatchTables::plain
     float4* d_sbox = nullptr;
     uint32_t* d_sbits = nullptr;
     float4* d_snear = nullptr;
@@ -129,6 +130,7 @@ void free_tables(BatchState& b)
     b.n_sblk = 0;
     b.sym_min = -1;
     b.sym_max_words = 0;
+    b.sym_plain = true;
     b.d_insts = nullptr; b.d_prm = nullptr; b.d_inst_of = nullptr; b.d_items = nullptr; b.d_blk = nullptr;
     b.d_queue = nullptr; b.d_bbox = nullptr; b.d_keys = nullptr; b.d_keys_alt = nullptr; b.d_vals = nullptr;
     b.d_perm = nullptr; b.d_sort_tmp = nullptr; b.d_ts = nullptr; b.d_um = nullptr;
@@ -432,6 +434,7 @@ int ensure_sym(Ctx& c, BatchState& b)
     b.n_sblk = (int)T.blk.size();
     b.sym_min = c.batch_symmetric_min;
     b.sym_max_words = T.max_words;
+    b.sym_plain = T.plain;
     // fresh arrays for the new tables (upload()-style: free, then allocate)
     void* old[] = {b.d_sbox, b.d_sbits, b.d_snear, b.d_sapart, b.d_sbpart};
     for (void* q : old) free_ptr(q);
@@ -483,6 +486,7 @@ int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_
     L.rows = b.d_srows;
     L.nrows = b.n_srows;
     L.max_words = b.sym_max_words;
+    L.plain = b.sym_plain;
     L.queue = b.d_queue + 1;
     L.wgs = c.sym_workgroups > 0 ? c.sym_workgroups : sym_default_workgroups(device_cus(c));
     L.acc = (float4*)p->acc;

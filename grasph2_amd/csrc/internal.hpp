@@ -201,7 +201,8 @@ struct SymGeom {
     int64_t box_elems;    // float4: bounding box per block of 64
     int64_t bits_elems;   // uint32: near-block bit matrix (lists wanted)
     int64_t near_elems;   // float4: near-field partial per slot and share
-    int64_t apart_elems;  // float4: far own-side fragments
+    int64_t apart_elems;  // float4: far own-side fragments (kFragElems each)
+    bool plain;           // no far workgroup flushes a fragment twice: no correction planes
     int64_t bpart_elems;  // float: far partner-side partials
 };
 SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares);
@@ -214,7 +215,6 @@ struct SymLaunch {
     bool finish;        // as SweepLaunch::finish
     bool add_acc;       // acc already holds the SPH terms: the combine adds the gravity to them
     bool list_prio;     // the lists' wavefronts take issue priority over the far field's beside them
-    bool ingroup_done;  // the in-group gravity is already enqueued
     int list_shares;    // its list kernel cut into this many shares (own list planes, merged after)
     int nshares;        // near field cut into this many shares per block (own list planes, merged after)
     float4* acc;
@@ -229,7 +229,7 @@ struct SymLaunch {
     float4* box;
     uint32_t* bits;
     float4* near_part;
-    float4* apart;
+    float4* apart;        // own-side fragments (kFragElems each)
     float* bpart;
 };
 // the symmetric sweep in four launches: the near field's lists and its
@@ -260,6 +260,7 @@ struct SymBatchTables {
     std::vector<int4> blk;       // 256-slot blocks (instance, first, end) of the instances in the sweep
     int max_words = 0;
     int64_t box_elems = 0, bits_elems = 0, apart_elems = 0, bpart_elems = 0;
+    bool plain = true;  // no row piece flushes its fragment twice
 };
 // instances of at least min_n particles take the symmetric sweep
 SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_nbr, int64_t min_n);
@@ -280,6 +281,7 @@ struct SymBatchLaunch {
     int max_words;
     int* queue;           // one work counter
     int wgs;
+    bool plain;           // SymBatchTables::plain
     bool ingroup_done;    // the in-group gravity was enqueued after the far field on its stream
     const float* umass;   // {uniform, mass} per instance
     float4* acc;
@@ -292,7 +294,7 @@ struct SymBatchLaunch {
     float4* box;
     uint32_t* bits;
     float4* near_part;
-    float4* apart;
+    float4* apart;        // own-side fragments (kFragElems each)
     float* bpart;
 };
 // the batched symmetric sweep in four launches, as the single-instance one

@@ -15,7 +15,7 @@ namespace grasph {
 
 // own slots per lane in the far field: the partner-side reduction and the
 // partners' DPP moves of a chunk are shared by the lane's slots (16: one
-// wavefront owns a group of 1024, ~210 VGPRs, 2 wavefronts per SIMD; 4 was
+// wavefront owns a group of 1024, 209-235 VGPRs, 2 wavefronts per SIMD; 4 was
 // measured 7% slower than 8)
 constexpr int kNI = 16;
 constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefront's slots)
@@ -96,5 +96,23 @@ __device__ __host__ inline int64_t batch_piece_begin(int64_t units, int h)
 }
 // first element of the partner-side partials of group h (the h groups before it contribute)
 __device__ __host__ inline int64_t bpart_group_base(int64_t h) { return (int64_t)kGroup * (h * (h - 1) / 2); }
+
+// Own-side fragments: one per (row, wavefront that worked on the row), in
+// float4 elements, 2 kGroup of them: for each 64 slots of the group, their
+// 64 sums (x y z, w unused), then those sums' 64 corrections (slot s: sums
+// at frag_elem(s), corrections 64 further).  A wavefront adds a unit's
+// own-side sums to LDS sums and those to the fragment every kFoldsPerFlush
+// units and at the end of its stay on the row.  The first flush writes the
+// sums (their corrections are exactly zero).  When no wavefront stays longer
+// than kFoldsPerFlush units on a row, every fragment is flushed once: it is
+// plain, and neither the far field nor the combine touch its corrections;
+// otherwise the first flush writes zero corrections too.
+constexpr int kFoldsPerFlush = 16;
+constexpr int kFragElems = 2 * kGroup;
+__device__ __host__ inline int frag_elem(int slot) { return 128 * (slot >> 6) + (slot & 63); }
+__device__ __host__ inline bool fragments_plain(int64_t max_units_per_visit)
+{
+    return max_units_per_visit <= kFoldsPerFlush;
+}
 
 }  // namespace grasph

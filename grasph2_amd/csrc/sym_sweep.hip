@@ -253,10 +253,9 @@ __global__ __launch_bounds__(kBlock) void near_kernel(const float4* __restrict__
 // the wavefront's LDS sums (lsum[1024 c + slot], component c: plain sums of
 // at most kFoldsPerFlush units), and every kFoldsPerFlush units and at the
 // end of the row the LDS sums go into the row's own-side fragment in global
-// memory (slot s: frag[2 s] sums x y z, frag[2 s + 1] their corrections),
-// compensated.  Neither level takes registers from the pair loop, and the
-// fragment is read and written about once per row.
-constexpr int kFoldsPerFlush = 16;
+// memory (float4 sums at frag[frag_elem(slot)], their corrections 64
+// further), compensated.  Neither level takes registers from the pair loop,
+// and the fragment is written about once per row.
 struct FarOwn {
     float x[kNI], y[kNI], z[kNI], m[kNI];
     float ax[kNI], ay[kNI], az[kNI];
@@ -264,6 +263,7 @@ struct FarOwn {
     float4* frag;
     int folds;   // units in lsum since the last flush (0: lsum holds nothing)
     bool fresh;  // nothing flushed into the fragment yet: the first flush writes it
+    bool plain;  // every fragment is flushed once: no correction planes
 };
 
 // ------------------------------------------------------------------ rotated far segment
@@ -421,7 +421,7 @@ __device__ __forceinline__ void far_segment_rot(const float4* __restrict__ xs, c
 // the wavefront's LDS sums lsum, and the row's fragment of workgroup k:
 // frag (fresh until the first flush).
 __device__ __forceinline__ void far_load_row(const float4* __restrict__ xs, const int n, const int64_t g,
-                                             const int lane, float* lsum, float4* frag, FarOwn& o)
+                                             const int lane, float* lsum, float4* frag, const bool plain, FarOwn& o)
 {
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
@@ -438,6 +438,7 @@ __device__ __forceinline__ void far_load_row(const float4* __restrict__ xs, cons
     o.frag = frag;
     o.folds = 0;
     o.fresh = true;
+    o.plain = plain;
 }
 
 // The lane's slot of own slot q, through an empty asm statement: indexed
@@ -452,25 +453,27 @@ __device__ __forceinline__ int own_lane(const int lane)
 }
 
 // The LDS sums of the lane's slots added (compensated) into the row's
-// fragment.
+// fragment; the first flush stores them (and zero corrections unless the
+// fragments are plain).
 __device__ __forceinline__ void far_flush_own(FarOwn& o, const int lane)
 {
     if (o.folds == 0) return;
     const int l = own_lane(lane);
+    float4* const fr = o.frag;
 #pragma unroll
     for (int q = 0; q < kNI; ++q) {
-        const int slot = 64 * q + l;
+        const int slot = 64 * q + l, e = 128 * q + l;  // e: frag_elem(slot)
         float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), cv = sv;
         if (!o.fresh) {
-            sv = o.frag[2 * slot];
-            cv = o.frag[2 * slot + 1];
+            sv = fr[e];
+            cv = fr[e + 64];
         }
         CSum x{sv.x, cv.x}, y{sv.y, cv.y}, z{sv.z, cv.z};
         x.add(o.lsum[slot]);
         y.add(o.lsum[kGroup + slot]);
         z.add(o.lsum[2 * kGroup + slot]);
-        o.frag[2 * slot] = make_float4(x.s, y.s, z.s, 0.0f);
-        o.frag[2 * slot + 1] = make_float4(x.c, y.c, z.c, 0.0f);
+        fr[e] = make_float4(x.s, y.s, z.s, 0.0f);
+        if (!o.plain) fr[e + 64] = make_float4(x.c, y.c, z.c, 0.0f);
     }
     o.fresh = false;
     o.folds = 0;
@@ -521,11 +524,13 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
 // Far field of one instance.  Workgroup k of W takes the units
 // [k U / W, (k + 1) U / W) of the rows g (group g against the later
 // groups, each cut into Q pieces) laid end to end; its own-side sums of row
-// g go to fragment g + k.
+// g go to fragment g + k (plain: no workgroup has more than kFoldsPerFlush
+// units).
 template <bool UM, bool SOFT, int QC>
 __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
                                          const int64_t U, const int W, const int Qr, float4* __restrict__ apart,
-                                         float* __restrict__ bpart, const float H2, const float m, float* lsum)
+                                         const bool plain, float* __restrict__ bpart, const float H2, const float m,
+                                         float* lsum)
 {
     const int Q = QC ? QC : Qr;
     const int k = blockIdx.x, lane = threadIdx.x;
@@ -533,13 +538,13 @@ __device__ __forceinline__ void far_work(const float4* __restrict__ xs, const in
     if (u0 >= u1) return;
     int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
     FarOwn o;
-    far_load_row(xs, n, g, lane, lsum, apart + 2 * (size_t)(g + k) * kGroup, o);
+    far_load_row(xs, n, g, lane, lsum, apart + (size_t)kFragElems * (g + k), plain, o);
     for (int64_t u = u0; u < u1; ++u) {
         if (s == row_units(g, nsb, Q)) {
             far_flush_own(o, lane);
             ++g;
             s = 0;
-            far_load_row(xs, n, g, lane, lsum, apart + 2 * (size_t)(g + k) * kGroup, o);
+            far_load_row(xs, n, g, lane, lsum, apart + (size_t)kFragElems * (g + k), plain, o);
         }
         far_unit<UM, SOFT, QC>(xs, n, g, (int)(g + 1 + s / Q), (int)(s % Q), Q, o, bpart, H2, m, lane);
         ++s;
@@ -551,21 +556,21 @@ template <bool SOFT>
 __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __restrict__ xs, const int n,
                                                             const int64_t nsb, const int64_t U, const int W,
                                                             const int Q, float4* __restrict__ apart,
-                                                            float* __restrict__ bpart, const float H2,
-                                                            const float* __restrict__ umass)
+                                                            const int plain, float* __restrict__ bpart,
+                                                            const float H2, const float* __restrict__ umass)
 {
     __shared__ float lsum[3 * kGroup];
     const bool um = umass[0] != 0.0f;
     if (Q == 2) {
         if (um)
-            far_work<true, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, umass[1], lsum);
+            far_work<true, SOFT, 2>(xs, n, nsb, U, W, 2, apart, plain, bpart, H2, umass[1], lsum);
         else
-            far_work<false, SOFT, 2>(xs, n, nsb, U, W, 2, apart, bpart, H2, 1.0f, lsum);
+            far_work<false, SOFT, 2>(xs, n, nsb, U, W, 2, apart, plain, bpart, H2, 1.0f, lsum);
     } else {
         if (um)
-            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, umass[1], lsum);
+            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, plain, bpart, H2, umass[1], lsum);
         else
-            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, bpart, H2, 1.0f, lsum);
+            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, plain, bpart, H2, 1.0f, lsum);
     }
 }
 
@@ -576,8 +581,9 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
 // (small counts: the slots alone would not fill the GPU, and a slot's row
 // has up to hundreds of fragments), PARTS threads share a slot: part p
 // takes every PARTS-th term of each list from the p-th on, and part 0 adds
-// the parts' compensated sums in part order.
-template <int PARTS>
+// the parts' compensated sums in part order.  PLAIN: the fragments hold
+// sums alone (their corrections are zero and were not stored).
+template <int PARTS, bool PLAIN>
 __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
     const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
     const int n, const int nshares, const int64_t nsb, const int64_t U, const int W, const int Q, const int grav,
@@ -604,25 +610,29 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
             const int64_t k0 = wg_of_unit(row_prefix(g, nsb, Q), U, W);
             const int64_t k1 = wg_of_unit(row_prefix(g + 1, nsb, Q) - 1, U, W);
             int64_t k = k0 + p;
+            const int el = frag_elem((int)local);
             for (; k + 7 * PARTS <= k1; k += 8 * PARTS) {
                 float4 f[8], fc[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    const size_t e = 2 * ((size_t)(g + k + u * PARTS) * kGroup + local);
-                    f[u] = apart[e];
-                    fc[u] = apart[e + 1];
+                    const float4* fr = apart + (size_t)kFragElems * (g + k + u * PARTS) + el;
+                    f[u] = fr[0];
+                    if (!PLAIN) fc[u] = fr[64];
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     x.add(f[u].x); y.add(f[u].y); z.add(f[u].z);
-                    x.c += fc[u].x; y.c += fc[u].y; z.c += fc[u].z;
+                    if (!PLAIN) { x.c += fc[u].x; y.c += fc[u].y; z.c += fc[u].z; }
                 }
             }
             for (; k <= k1; k += PARTS) {
-                const size_t e = 2 * ((size_t)(g + k) * kGroup + local);
-                const float4 f = apart[e], fc = apart[e + 1];
+                const float4* fr = apart + (size_t)kFragElems * (g + k) + el;
+                const float4 f = fr[0];
                 x.add(f.x); y.add(f.y); z.add(f.z);
-                x.c += fc.x; y.c += fc.y; z.c += fc.z;
+                if (!PLAIN) {
+                    const float4 fc = fr[64];
+                    x.c += fc.x; y.c += fc.y; z.c += fc.z;
+                }
             }
         }
         const int64_t base = bpart_group_base(g) + local;
@@ -776,22 +786,23 @@ __global__ __launch_bounds__(kBlock) void sym_batch_near_kernel(
 // R.units partner groups from j0 on, two pieces of 512 partners each
 template <bool UM, bool SOFT>
 __device__ __forceinline__ void batch_far_row(const float4* __restrict__ xk, const int n, const SymRow R, const int j0,
-                                              float* lsum, float4* frag, float* __restrict__ bk, const float H2,
-                                              const float m, const int lane)
+                                              float* lsum, float4* frag, const bool plain, float* __restrict__ bk,
+                                              const float H2, const float m, const int lane)
 {
     FarOwn o;
-    far_load_row(xk, n, R.g, lane, lsum, frag, o);
+    far_load_row(xk, n, R.g, lane, lsum, frag, plain, o);
     for (int s = 0; s < 2 * R.units; ++s) far_unit<UM, SOFT, 2>(xk, n, R.g, j0 + (s >> 1), s & 1, 2, o, bk, H2, m, lane);
     far_flush_own(o, lane);
 }
 
 // Persistent far field over the batch's rows; every wavefront leaves once
-// the counter passes nrows.
+// the counter passes nrows.  plain: no piece has more than kFoldsPerFlush
+// units.
 template <bool SOFT>
 __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
     const float4* __restrict__ xs, const SymInst* __restrict__ sinst, const BatchInst* __restrict__ insts,
     const SymRow* __restrict__ rows, const int nrows, int* __restrict__ queue, float4* __restrict__ apart,
-    float* __restrict__ bpart, const float* __restrict__ umass)
+    const int plain, float* __restrict__ bpart, const float* __restrict__ umass)
 {
     __shared__ float lsum[3 * kGroup];
     const int lane = threadIdx.x;
@@ -808,17 +819,18 @@ __global__ __launch_bounds__(kFarThreads, 2) void sym_batch_far_kernel(
         const bool um = umass[2 * R.inst] != 0.0f;
         const float m = umass[2 * R.inst + 1];
         const int j0 = R.g + 1 + (int)batch_piece_begin(row_units(R.g, S.nsb), R.piece);
-        float4* const frag = apart + 2 * ((size_t)kBatchPieces * (S.row0 + R.g) + R.piece) * kGroup;
+        float4* const frag = apart + (size_t)kFragElems * ((size_t)kBatchPieces * (S.row0 + R.g) + R.piece);
         if (um)
-            batch_far_row<true, SOFT>(xk, S.n, R, j0, lsum, frag, bk, H2, m, lane);
+            batch_far_row<true, SOFT>(xk, S.n, R, j0, lsum, frag, plain, bk, H2, m, lane);
         else
-            batch_far_row<false, SOFT>(xk, S.n, R, j0, lsum, frag, bk, H2, 1.0f, lane);
+            batch_far_row<false, SOFT>(xk, S.n, R, j0, lsum, frag, plain, bk, H2, 1.0f, lane);
     }
 }
 
 // Per slot of the instances in the sweep (256-slot block table), as
 // sym_combine_kernel: near partial, the row's one own-side fragment, the
 // partner-side partials (group order, negated).
+template <bool PLAIN>
 __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
     const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
     const SymInst* __restrict__ sinst, const int4* __restrict__ blk, const int grav, const int* __restrict__ perm,
@@ -841,10 +853,13 @@ __global__ __launch_bounds__(kBlock) void sym_batch_combine_kernel(
         if (g < S.nrows) {
             const int pieces = batch_row_pieces(row_units(g, S.nsb));
             for (int h = 0; h < pieces; ++h) {
-                const size_t e = 2 * (((size_t)kBatchPieces * (S.row0 + g) + h) * kGroup + lg);
-                const float4 f = apart[e], fc = apart[e + 1];
+                const float4* fr = apart + (size_t)kFragElems * ((size_t)kBatchPieces * (S.row0 + g) + h) + frag_elem(lg);
+                const float4 f = fr[0];
                 x.add(f.x); y.add(f.y); z.add(f.z);
-                x.c += fc.x; y.c += fc.y; z.c += fc.z;
+                if (!PLAIN) {
+                    const float4 fc = fr[64];
+                    x.c += fc.x; y.c += fc.y; z.c += fc.z;
+                }
             }
         }
         const float* bk = bpart + S.bpart0;
@@ -918,7 +933,8 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
     G.box_elems = 2 * (int64_t)G.nb;
     G.bits_elems = want_nbr ? (int64_t)G.nb * G.nwords : 0;
     G.near_elems = 2 * n * nshares;
-    G.apart_elems = 2 * (int64_t)kGroup * (G.groups + wgs);
+    G.apart_elems = (int64_t)kFragElems * (G.groups + wgs);
+    G.plain = fragments_plain((G.units + G.wgs - 1) / G.wgs);
     G.bpart_elems = 3 * bpart_group_base(G.groups);
     return G;
 }
@@ -962,10 +978,10 @@ hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s)
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
     if (P.H2 > 0.0f)
         hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
-                           G.split, L.apart, L.bpart, P.H2, L.umass);
+                           G.split, L.apart, G.plain ? 1 : 0, L.bpart, P.H2, L.umass);
     else
         hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
-                           G.split, L.apart, L.bpart, P.H2, L.umass);
+                           G.split, L.apart, G.plain ? 1 : 0, L.bpart, P.H2, L.umass);
     return hipGetLastError();
 }
 
@@ -976,16 +992,20 @@ hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_
     const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
     // threads per slot: about 2^17 threads or more in all
     const int parts = L.n >= 98304 ? 1 : L.n >= 49152 ? 2 : L.n >= 24576 ? 4 : 8;
-#define GRASPH_COMBINE(PARTS)                                                                                         \
-    hipLaunchKernelGGL((sym_combine_kernel<PARTS>), dim3((L.n + kBlock / PARTS - 1) / (kBlock / PARTS)), dim3(kBlock), \
-                       0, s, L.near_part, L.apart, L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs, G.split,            \
-                       L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.vel, L.xvel,       \
-                       L.maxvsig, P)
-    switch (parts) {
-    case 1: GRASPH_COMBINE(1); break;
-    case 2: GRASPH_COMBINE(2); break;
-    case 4: GRASPH_COMBINE(4); break;
-    default: GRASPH_COMBINE(8); break;
+#define GRASPH_COMBINE(PARTS, PLAIN)                                                                                  \
+    hipLaunchKernelGGL((sym_combine_kernel<PARTS, PLAIN>), dim3((L.n + kBlock / PARTS - 1) / (kBlock / PARTS)),        \
+                       dim3(kBlock), 0, s, L.near_part, L.apart, L.bpart, L.n, L.nshares, G.nsb, G.units, L.wgs,       \
+                       G.split, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.vel,       \
+                       L.xvel, L.maxvsig, P)
+    if (G.plain) {
+        switch (parts) {
+        case 1: GRASPH_COMBINE(1, true); break;
+        case 2: GRASPH_COMBINE(2, true); break;
+        case 4: GRASPH_COMBINE(4, true); break;
+        default: GRASPH_COMBINE(8, true); break;
+        }
+    } else {
+        GRASPH_COMBINE(1, false);  // more than kFoldsPerFlush units per workgroup: large counts (or few workgroups)
     }
 #undef GRASPH_COMBINE
     return hipGetLastError();
@@ -1016,8 +1036,11 @@ SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_n
         S.bpart0 = bpart0;
         for (int g = 0; g < S.nrows; ++g) {
             const int64_t u = row_units(g, S.nsb);
-            for (int h = 0; h < batch_row_pieces(u); ++h)
-                T.rows.push_back(SymRow{k, g, h, (int)(batch_piece_begin(u, h + 1) - batch_piece_begin(u, h))});
+            for (int h = 0; h < batch_row_pieces(u); ++h) {
+                const int units = (int)(batch_piece_begin(u, h + 1) - batch_piece_begin(u, h));
+                T.rows.push_back(SymRow{k, g, h, units});
+                T.plain = T.plain && fragments_plain(2 * (int64_t)units);  // two far units per partner group
+            }
         }
         T.max_words = std::max(T.max_words, S.nb * S.nwords);
         box0 += S.nb;
@@ -1029,7 +1052,7 @@ SymBatchTables sym_batch_tables(const std::vector<int64_t>& offsets, bool want_n
     std::stable_sort(T.rows.begin(), T.rows.end(), [](const SymRow& a, const SymRow& b) { return a.units > b.units; });
     T.box_elems = 2 * (int64_t)box0;
     T.bits_elems = bits0;
-    T.apart_elems = 2 * (int64_t)kGroup * kBatchPieces * row0;
+    T.apart_elems = (int64_t)kFragElems * kBatchPieces * row0;
     T.bpart_elems = bpart0;
     return T;
 }
@@ -1059,19 +1082,24 @@ hipError_t launch_sym_batch_far(const SymBatchLaunch& L, hipStream_t s)
     if ((e = launch_reset_queue(L.queue, 1, s)) != hipSuccess) return e;
     if (L.soft)
         hipLaunchKernelGGL((sym_batch_far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst, L.insts,
-                           L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
+                           L.rows, L.nrows, L.queue, L.apart, L.plain ? 1 : 0, L.bpart, L.umass);
     else
         hipLaunchKernelGGL((sym_batch_far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.sinst, L.insts,
-                           L.rows, L.nrows, L.queue, L.apart, L.bpart, L.umass);
+                           L.rows, L.nrows, L.queue, L.apart, L.plain ? 1 : 0, L.bpart, L.umass);
     return hipGetLastError();
 }
 
 hipError_t launch_sym_batch_combine(const SymBatchLaunch& L, hipStream_t s)
 {
     if (L.ntotal <= 0 || L.n_blk <= 0 || !(L.grav || L.finish)) return hipSuccess;
-    hipLaunchKernelGGL(sym_batch_combine_kernel, dim3(L.n_blk), dim3(kBlock), 0, s, L.near_part, L.apart, L.bpart,
-                       L.sinst, L.blk, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0, L.add_acc ? 1 : 0,
-                       L.vel, L.xvel, L.maxvsig, L.prm);
+    if (L.plain)
+        hipLaunchKernelGGL(sym_batch_combine_kernel<true>, dim3(L.n_blk), dim3(kBlock), 0, s, L.near_part, L.apart,
+                           L.bpart, L.sinst, L.blk, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0,
+                           L.add_acc ? 1 : 0, L.vel, L.xvel, L.maxvsig, L.prm);
+    else
+        hipLaunchKernelGGL(sym_batch_combine_kernel<false>, dim3(L.n_blk), dim3(kBlock), 0, s, L.near_part, L.apart,
+                           L.bpart, L.sinst, L.blk, L.grav ? 1 : 0, L.perm, L.xs, L.acc, L.finish ? 1 : 0,
+                           L.add_acc ? 1 : 0, L.vel, L.xvel, L.maxvsig, L.prm);
     return hipGetLastError();
 }
 

@@ -9,12 +9,16 @@
 //  - every own-side fragment and partner-side partial index is in bounds of
 //    the arrays sym_geometry() sizes, and no partial is written twice;
 //  - every fragment and partial the combine reads was written, and every
-//    fragment written is read.
+//    fragment written is read;
+//  - every fragment is flushed at least once, and exactly once when the
+//    geometry calls the fragments plain (the combine then reads no
+//    correction planes).
 // Also the batch form: the rows table covers each (instance, row) once, and
 // each row's pieces cover its partner groups once.
 // Prints "ok <cases>" and exits 0, or prints the first failure and exits 1.
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <set>
 #include <utility>
 #include <vector>
@@ -40,13 +44,16 @@ static int check(int64_t n, int64_t wanted, int Q)
     if (sum != U) return fail("unit total", n, wanted, sum, U);
     if (U == 0) return 0;  // no far field: the near field covers the one group
     const int64_t W = far_workgroups(wanted, U);
-    const int64_t apart_elems = 2 * (int64_t)kGroup * (G + wanted);
+    const int64_t apart_elems = (int64_t)kFragElems * (G + wanted);
     const int64_t bpart_elems = 3 * bpart_group_base(G);
 
     std::vector<char> seen((size_t)(nsb * nsb * Q), 0);
     std::vector<char> bw((size_t)(bpart_elems / 3), 0);
     std::set<std::pair<int64_t, int64_t>> frag;  // (row, workgroup)
+    std::map<std::pair<int64_t, int64_t>, int> flushes;  // as far_work flushes its LDS sums
+    const bool plain = fragments_plain((U + W - 1) / W);
     for (int64_t k = 0; k < W; ++k) {
+        int folds = 0;
         const int64_t u0 = k * U / W, u1 = (k + 1) * U / W;
         if (u0 >= u1) return fail("empty share", n, W, k);
         int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
@@ -54,6 +61,8 @@ static int check(int64_t n, int64_t wanted, int Q)
         for (int64_t u = u0; u < u1; ++u) {
             if (s == row_units(g, nsb, Q)) {
                 frag.insert({g, k});
+                if (folds > 0) ++flushes[{g, k}];
+                folds = 0;
                 ++g;
                 s = 0;
                 if (g >= npos) return fail("row past the last", n, W, g);
@@ -76,8 +85,18 @@ static int check(int64_t n, int64_t wanted, int Q)
                 bw[(size_t)x] = 1;
             }
             ++s;
+            if (++folds == kFoldsPerFlush) {
+                ++flushes[{g, k}];
+                folds = 0;
+            }
         }
         frag.insert({g, k});
+        if (folds > 0) ++flushes[{g, k}];
+    }
+    for (const auto& f : frag) {
+        const int c = flushes.count(f) ? flushes[f] : 0;
+        if (c < 1) return fail("fragment never flushed", n, W, f.first, f.second);
+        if (plain && c != 1) return fail("plain fragment flushed twice", n, W, f.first, f.second);
     }
     for (int64_t I = 0; I < nsb; ++I)
         for (int64_t J = 0; J < nsb; ++J)
@@ -85,7 +104,7 @@ static int check(int64_t n, int64_t wanted, int Q)
                 if ((I / kGroupSB < J / kGroupSB) != (seen[(size_t)((I * nsb + J) * Q + p)] != 0))
                     return fail("coverage", n, W, I, J);
     for (const auto& f : frag)
-        if (2 * ((f.first + f.second + 1) * (int64_t)kGroup) > apart_elems) return fail("fragment bounds", n, W, f.first, f.second);
+        if ((f.first + f.second + 1) * (int64_t)kFragElems > apart_elems) return fail("fragment bounds", n, W, f.first, f.second);
     // what the combine reads
     size_t reads = 0;
     for (int64_t g = 0; g < G; ++g) {

@@ -67,6 +67,29 @@ def test_matches_one_sided_batch_and_fp64(sym_batch_ctx, gravity_only):
     assert nr <= 1e-5 and mx <= 1e-5, (mx, nr)
 
 
+def test_large_instance_flushes_fragments_twice(sym_batch_ctx):
+    """An instance of 40000 particles (40 groups): its first rows' pieces
+    hold 10 partner groups, 20 far units, more than a wavefront folds
+    between flushes (16), so the batch's fragments carry corrections."""
+    print("batch symmetric, instance of 40000", flush=True)
+    ctx = sym_batch_ctx
+    sizes = (40000, 3000, 500)
+    insts, params, host, offsets = make_batch(sizes=sizes, seed=9)
+    for p in params:
+        p.flags = g.SELF_GRAVITY
+    got = run_batch(ctx, host, offsets, params)
+    ctx.set_option("batch_symmetric", 0)
+    ref = run_batch(ctx, host, offsets, params)
+    ctx.set_option("batch_symmetric", 1)
+    for k, (a, r) in enumerate(zip(got, ref)):
+        mx, nr = vec_errors(a["acc"], r["acc"])
+        assert nr <= 1e-6 and mx <= 1e-5, (k, mx, nr)
+    idx = np.random.default_rng(1).choice(sizes[0], 2000, replace=False)  # FP64 on a sample
+    o = O.calc_acceleration(params[0], insts[0], "f64", sample=idx)
+    mx, nr = vec_errors(got[0]["acc"][idx], o["acc"][idx])
+    assert nr <= 1e-5 and mx <= 1e-5, (mx, nr)
+
+
 def test_graph_steps_match_eager_and_repeat(sym_batch_ctx):
     print("batch symmetric graph", flush=True)
     ctx = sym_batch_ctx

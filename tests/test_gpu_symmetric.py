@@ -40,6 +40,27 @@ def test_gravity_matches_fp64(sym_ctx, n):
     check_vec(out["acc"], ref["acc"])
 
 
+# Few far workgroups: a workgroup then stays on a row for more units than it
+# folds into its LDS sums between flushes (16), so its fragment is flushed
+# several times, compensated, with corrections (3 workgroups: ~500 units
+# each, several rows; 40: 19 units, two flushes); 25000 on 40 workgroups
+# (15 units each) keeps the plain fragments of one flush.
+@pytest.mark.parametrize("n,wgs", [(20037, 3), (20037, 40), (25000, 40)])
+def test_gravity_few_far_workgroups(sym_ctx, n, wgs):
+    print("symmetric gravity n=%d sym_workgroups=%d" % (n, wgs), flush=True)
+    h, prm = ics.plummer(n, seed=n + wgs)
+    base = gpu_run(sym_ctx, h, prm)
+    sym_ctx.set_option("sym_workgroups", wgs)
+    try:
+        out = gpu_run(sym_ctx, h, prm)
+    finally:
+        sym_ctx.set_option("sym_workgroups", 0)
+    ref = O.calc_acceleration(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
+    mx, nr = vec_errors(out["acc"], base["acc"])
+    assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
+
+
 @pytest.mark.parametrize("uniform", [True, False])
 def test_gravity_equal_and_unequal_masses(sym_ctx, uniform):
     print("symmetric masses uniform=%s" % uniform, flush=True)
