@@ -9,7 +9,10 @@ particles.  Default workload: the two-sphere solid collision, N = 131072
 FP64 position drift).  Prints one JSON line.
 
 value = pair interactions per second = N (N - 1) / (seconds per step): every
-step evaluates softened gravity on all ordered pairs i != j.
+step evaluates softened gravity on all ordered pairs i != j.  The timed
+pass launches the steps one by one, or, for gravity alone (whose step holds
+no Morton re-sort), replays them from a HIP graph of config.graph_steps
+steps captured on a stream of its own (the same kernels and arguments).
 roofline: the dominant kernel, FP32 vector ALU bound, at 20 algorithmic
 FLOPs per ordered gravity pair (FMA = 2, rsqrt = 1), divided by its mean
 duration from HIP events recorded around every launch on the stream it
@@ -62,12 +65,17 @@ INTEGRATOR_BYTES_NBODY = 96    # x64 24 + v 12 + a 12 read; x64 24 + xf 12 + v 1
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (default 10; 100 for the Plummer workload, whose steps take ~0.1 ms)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="collision", choices=["collision", "contact", "plummer", "liquid", "large"])
     ap.add_argument("--advance", type=int, default=0,
                     help="untimed steps before the warm-up (e.g. to carry the collision into contact)")
     ap.add_argument("--n", type=int, default=0, help="override the particle count")
+    ap.add_argument("--graph", type=int, default=-1, metavar="G",
+                    help="steps per captured HIP graph in the timed pass (0: plain launches); default 10 for "
+                         "gravity alone, whose step holds no Morton re-sort, else 0 (the re-sort every "
+                         "reorder_interval steps stays out of graphs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
     ap.add_argument("--ensemble", type=int, default=512,
@@ -128,6 +136,8 @@ def ensemble_measure(ctx, count, steps, loop_steps):
 
 def main():
     args = parse()
+    if args.steps <= 0:
+        args.steps = 100 if args.workload == "plummer" else 10
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -161,24 +171,45 @@ def main():
         ctx.set_option(key, int(val))
     d = g.DeviceParticles.from_host(h)
     g.hip.synchronize()
+    graph_steps = args.graph if args.graph >= 0 else (0 if sph else 10)
+    if graph_steps > 0 and args.steps % graph_steps:
+        graph_steps = 1
+    st = g.hip.Stream() if graph_steps > 0 else None  # a graph is captured from a stream of its own
 
-    def step():
-        ctx.compute_derivatives(d, prm)
-        ctx.integrate(d, prm, True)
+    def step(stream=st):
+        ctx.compute_derivatives(d, prm, stream)
+        ctx.integrate(d, prm, True, stream)
+
+    def sync():
+        if st is not None:
+            st.synchronize()
+        g.hip.synchronize()
 
     for k in range(args.advance):
         step()
         if (k + 1) % 200 == 0:
-            g.hip.synchronize()
+            sync()
             print("advanced %d of %d steps" % (k + 1, args.advance), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
-    g.hip.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    g.hip.synchronize()
-    t1 = time.perf_counter()
+    sync()
+    if graph_steps > 0:
+        # the same steps, replayed: graph_steps steps per launch
+        graph = g.hip.Graph(st, lambda s: [step(s) for _ in range(graph_steps)])
+        graph.launch(st)  # a replay's worth of warm-up
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps // graph_steps):
+            graph.launch(st)
+        sync()
+        t1 = time.perf_counter()
+        del graph
+    else:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        sync()
+        t1 = time.perf_counter()
     # per-kernel times from HIP events in a second timed pass of as many
     # steps (recording events around every launch lengthens a step, so
     # ms_per_step comes from the pass above, which records none)
@@ -186,7 +217,7 @@ def main():
     ctx.kernel_times(reset=True)
     for _ in range(args.steps):
         step()
-    g.hip.synchronize()
+    sync()
     kt = ctx.kernel_times(reset=True)
     ctx.set_option("profile", 0)
     nb_pairs, nb_max, nb_over = ctx.neighbour_stats() if sph else (0, 0, 0)
@@ -249,7 +280,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded initial conditions of the workload)",
         "config": {"workload": args.workload, "n": n, "flags": int(prm.flags), "H": prm.H,
-                   "dt": prm.fixed_timestep, "fp64_drift": True, "advance_steps": args.advance},
+                   "dt": prm.fixed_timestep, "fp64_drift": True, "advance_steps": args.advance,
+                   "graph_steps": graph_steps},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": dom_name, "kernel_ms": dom_ms, "flops_per_launch": dom_flops,
@@ -296,9 +328,12 @@ def plastic_fraction(ctx, d, prm):
 
 
 def cpu_baseline(d, prm, n, target_s):
-    """Time the FP32 oracle's three passes for a sample of i against all j."""
+    """Time the FP32 oracle's passes of computeDerivatives for a sample of i
+    against all j: the ones the flags select (computeDerivatives.h:315-328):
+    density and strain with SPH, acceleration always."""
     import numpy as np
 
+    import grasph2_amd as g
     from oracle import pyoracle as O
 
     host = host_cpu()
@@ -306,12 +341,20 @@ def cpu_baseline(d, prm, n, target_s):
     O.set_threads(threads)
     state = d.download()
     rng = np.random.default_rng(0)
+    f = int(prm.flags)
+    sph = bool(f & g.SPH)
+    dens = sph and not (f & g.INTEGRATE_DENSITY)
+    bal = sph and bool(f & (g.SOLIDS | g.INTEGRATE_DENSITY | g.BALSARA_SWITCH))
+    passes_run = (["calcDensity"] if dens else []) + (["calcBalsaraDensityDTDStressDT"] if bal else []) + \
+        ["calcAcceleration"]
 
     def run(count):
         sample = np.sort(rng.choice(n, size=count, replace=False))
         t = time.perf_counter()
-        O.calc_density(prm, state, "f32", sample)
-        O.calc_balsara(prm, state, "f32", sample)
+        if dens:
+            O.calc_density(prm, state, "f32", sample)
+        if bal:
+            O.calc_balsara(prm, state, "f32", sample)
         O.calc_acceleration(prm, state, "f32", sample)
         return time.perf_counter() - t
 
@@ -334,8 +377,8 @@ def cpu_baseline(d, prm, n, target_s):
         count = n
         what = "%d pass(es) over all %d target particles" % (passes, n)
     out = {"value": passes * count * (n - 1) / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
-           "sample": "FP32 oracle, calcDensity + calcBalsaraDensityDTDStressDT + calcAcceleration for %s "
-                     "against all %d partners, %.1f s, %d OpenMP threads" % (what, n, t, threads)}
+           "sample": "FP32 oracle, %s for %s against all %d partners, %.1f s, %d OpenMP threads"
+                     % (" + ".join(passes_run), what, n, t, threads)}
     out.update(host)
     return out
 

@@ -584,6 +584,10 @@ template <typename T> int pass(int which, const grasph_params* prm, const oracle
 {
     int v = validate<T>(prm);
     if (v) return v;
+    // the SPH passes read and write the SPH arrays, which a state of
+    // gravity alone does not have: an error, not a write through null
+    if (which < 2 && (!p->density || !p->vel)) return -1;
+    if (which == 1 && (prm->flags & GRASPH_SOLIDS) && (!p->dstress || !p->dstress_dt)) return -1;
     Consts<T> C = make_consts<T>(prm);
     Parts<T> P = parts_from<T>(p);
     Range R{s, s ? ns : P.n};

@@ -696,3 +696,15 @@ def test_fp32_oracle_tracks_fp64():
     a64, a32 = o64["acc"][:, :3], o32["acc"][:, :3]
     assert np.linalg.norm(a32 - a64) / np.linalg.norm(a64) < 1e-5
     assert np.allclose(o32["density"], o64["density"], rtol=1e-5)
+
+
+def test_sph_passes_need_sph_arrays():
+    """A state of gravity alone (positions, velocities, accelerations) has no
+    SPH arrays: the oracle's SPH passes refuse it instead of writing through
+    null pointers; its acceleration pass runs."""
+    h, prm = ics.plummer(256, seed=2)
+    s = np.arange(16)
+    O.calc_acceleration(prm, h, "f32", s)
+    for fn in (O.calc_density, O.calc_balsara):
+        with pytest.raises(RuntimeError):
+            fn(prm, h, "f32", s)
