@@ -61,13 +61,18 @@ def test_gravity_few_far_workgroups(sym_ctx, n, wgs):
     assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
 
 
-@pytest.mark.parametrize("uniform", [True, False])
+# Equal masses (the mass factored out of both sides' sums), random masses,
+# and equal masses but for the last 50 particles (the equal-mass check
+# sees the exception at the end).
+@pytest.mark.parametrize("uniform", [True, False, "mixed"])
 def test_gravity_equal_and_unequal_masses(sym_ctx, uniform):
     print("symmetric masses uniform=%s" % uniform, flush=True)
     h, prm = ics.plummer(6000, seed=11)
     h["posm"][:, 3] = np.float32(1.0 / 6000)
-    if not uniform:
+    if uniform is False:
         h["posm"][:, 3] *= np.random.default_rng(5).uniform(0.5, 1.5, size=6000).astype(np.float32)
+    elif uniform == "mixed":
+        h["posm"][-50:, 3] *= np.float32(1.5)
     for H in (0.0, 0.01):
         prm.H = H
         out = gpu_run(sym_ctx, h, prm)
