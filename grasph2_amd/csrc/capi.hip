@@ -277,10 +277,6 @@ NbrLaunch nbr_base(Ctx& c, const grasph_particles* p)
     return L;
 }
 
-// The symmetric sweep keeps one partner-side partial per (group of 1024,
-// later particle): 6 n^2 / 1024 bytes.  Past this size the j-split sweep
-// runs instead.
-constexpr int64_t kSymMaxPartialBytes = int64_t(12) << 30;
 // The symmetric sweep's default thresholds (option sweep_symmetric = 2),
 // measured step times against the one-sided sweep: with neighbour lists
 // (collision) 0.319 against 0.322 ms at 16384 particles, 0.269 against
@@ -290,12 +286,6 @@ constexpr int64_t kSymAutoMinLists = 16384;
 constexpr int64_t kSymAutoMinGravity = 16384;
 
 int sym_wgs(const Ctx& c) { return c.sym_workgroups > 0 ? c.sym_workgroups : sym_default_workgroups(c.cus); }
-
-bool sym_fits(const Ctx& c, int64_t n, bool want_nbr)
-{
-    const SymGeom G = sym_geometry(n, sym_wgs(c), want_nbr, 1);
-    return G.bpart_elems * (int64_t)sizeof(float) <= kSymMaxPartialBytes;
-}
 
 // shares of the symmetric sweep's near field: ~16 wavefronts per SIMD, at most 16
 int sym_near_shares(const Ctx& c, int64_t n)
@@ -317,13 +307,16 @@ int sym_near_shares(const Ctx& c, int64_t n)
 int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_t s, SymLaunch& S,
                     hipStream_t& ws)
 {
-    const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.near_shares);
+    const int64_t col_elems = c.sym_partial_bytes / (int64_t)sizeof(float);
+    const SymGeom G = sym_geometry(L.n, sym_wgs(c), L.want_nbr, L.near_shares, col_elems);
     int r;
     if ((r = grow(c, c.sym_box, c.sym_box_cap, G.box_elems, "symmetric sweep boxes")) ||
         (r = grow(c, c.sym_bits, c.sym_bits_cap, G.bits_elems, "symmetric sweep near bits")) ||
         (r = grow(c, c.sym_near, c.sym_near_cap, G.near_elems, "symmetric sweep near sums")) ||
         (r = grow(c, c.sym_apart, c.sym_apart_cap, G.apart_elems, "symmetric sweep own-side sums")) ||
-        (r = grow(c, c.sym_bpart, c.sym_bpart_cap, G.bpart_elems, "symmetric sweep partner-side sums")))
+        (r = grow(c, c.sym_bpart, c.sym_bpart_cap, G.bpart_elems, "symmetric sweep partner-side sums")) ||
+        (G.ncols > 1 && L.grav &&
+         (r = grow(c, c.sym_accsum, c.sym_accsum_cap, 2 * (int64_t)L.n, "symmetric sweep column sums"))))
         return r;
     std::memset(&S, 0, sizeof S);
     S.xs = L.xs ? L.xs : L.posm;
@@ -348,6 +341,8 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
     S.near_part = c.sym_near;
     S.apart = c.sym_apart;
     S.bpart = c.sym_bpart;
+    S.col_elems = col_elems;
+    S.accsum = c.sym_accsum;
     hipError_t e;
     ws = s;
     // With gravity alone, only the in-group gravity (enqueued by
@@ -481,7 +476,7 @@ int derivatives(Ctx& c, const grasph_particles* p, const grasph_params* prm, voi
     const bool sweep_grav = accel_pass && grav;
     const bool sym_on = c.sweep_symmetric == 1 ||
                         (c.sweep_symmetric == 2 && n >= (need_list ? kSymAutoMinLists : kSymAutoMinGravity));
-    const bool sym = sym_on && (sweep_grav || need_list) && sym_fits(c, n, need_list);
+    const bool sym = sym_on && (sweep_grav || need_list);
     int slices = 1, slice_len = (int)n;
     if ((sweep_grav || need_list) && !sym) choose_slices(c, n, c.sweep_ni, &slices, &slice_len);
     int near_shares = 1;
@@ -832,7 +827,7 @@ int grasph_destroy(grasph_ctx* ctx)
     c.batch = nullptr;
     void* bufs[] = {c.perm, c.keys,   c.keys_alt, c.vals, c.sort_tmp, c.bbox,
                     c.nbr,  c.nbr_cnt, c.rec,     c.pot,  c.red,      c.part, c.ts, c.umass,
-                    c.xs,   c.sym_box, c.sym_bits, c.sym_near, c.sym_apart, c.sym_bpart, c.mass_flags};
+                    c.xs,   c.sym_box, c.sym_bits, c.sym_near, c.sym_apart, c.sym_bpart, c.sym_accsum, c.mass_flags};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& ev : c.ev_pending) {
@@ -875,6 +870,9 @@ int grasph_set_option(grasph_ctx* ctx, const char* key, int64_t value)
     } else if (k == "sweep_symmetric") {
         if (value < 0 || value > 2) return grasph::fail(c, GRASPH_E_INVALID, "sweep_symmetric must be 0, 1 or 2");
         c.sweep_symmetric = (int)value;
+    } else if (k == "sym_partial_bytes") {
+        if (value < (int64_t(1) << 20)) return grasph::fail(c, GRASPH_E_INVALID, "sym_partial_bytes must be at least 1 MiB");
+        c.sym_partial_bytes = value;
     } else if (k == "sym_workgroups") {
         if (value < 0 || value > 65536) return grasph::fail(c, GRASPH_E_INVALID, "sym_workgroups must be in [0, 65536]");
         c.sym_workgroups = (int)value;
@@ -918,6 +916,7 @@ int grasph_get_option(const grasph_ctx* ctx, const char* key, int64_t* value)
     else if (k == "sweep_slices") *value = c.sweep_slices;
     else if (k == "sweep_symmetric") *value = c.sweep_symmetric;
     else if (k == "sym_workgroups") *value = c.sym_workgroups;
+    else if (k == "sym_partial_bytes") *value = c.sym_partial_bytes;
     else if (k == "batch_symmetric") *value = c.batch_symmetric;
     else if (k == "batch_symmetric_min") *value = c.batch_symmetric_min;
     else if (k == "sym_near_shares") *value = c.sym_near_shares;

@@ -57,6 +57,8 @@ struct Ctx {
     int sweep_symmetric = 2;  // single instance, far pairs once for both sides (sym_sweep.hip): 0 off, 1 on, 2 from 16384 particles
     int sym_workgroups = 0;   // workgroups of its far kernel (0: 2 wavefronts per SIMD)
     int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
+    // bytes of partner-side partials one column batch of its far field may take (6 n^2 / 1024 in all)
+    int64_t sym_partial_bytes = int64_t(12) << 30;
     int sym_list_priority = 2;  // its near-field lists ahead of the far field: 0 no, 1 yes, 2 below 98304 particles
     int batch_symmetric = 1;  // batches: the symmetric sweep for instances of batch_symmetric_min particles or more
     int batch_symmetric_min = 0;
@@ -109,6 +111,8 @@ struct Ctx {
     int64_t sym_apart_cap = 0;
     float* sym_bpart = nullptr;
     int64_t sym_bpart_cap = 0;
+    float4* sym_accsum = nullptr;  // running sums between the far field's column batches
+    int64_t sym_accsum_cap = 0;
 
     // slot-ordered positions for the sweep, and the slot records of the passes
     float4* xs = nullptr;
@@ -203,9 +207,18 @@ struct SymGeom {
     int64_t near_elems;   // float4: near-field partial per slot and share
     int64_t apart_elems;  // float4: far own-side fragments (kFragElems each)
     bool plain;           // no far workgroup flushes a fragment twice: no correction planes
-    int64_t bpart_elems;  // float: far partner-side partials
+    int64_t bpart_elems;  // float: far partner-side partials (of the widest column batch)
+    int ncols;            // column batches of the far field (1: the whole far field at once)
 };
-SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares);
+SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares, int64_t col_elems = INT64_MAX);
+// one column batch [c0, c1) of the far field's partner groups
+struct SymCol {
+    int c0, c1;
+    int64_t units;
+    int wgs;
+    bool plain;
+};
+SymCol sym_column(int64_t nsb, int split, int wanted, int c0, int c1);
 int sym_default_workgroups(int cus);  // far-field workgroups for 2 wavefronts per SIMD
 struct SymLaunch {
     const float4* xs;   // slot-ordered positions + masses
@@ -231,6 +244,8 @@ struct SymLaunch {
     float4* near_part;
     float4* apart;        // own-side fragments (kFragElems each)
     float* bpart;
+    int64_t col_elems;    // partner-side partials (floats) a column batch of the far field may take
+    float4* accsum;       // running sums between column batches: 2 float4 per slot (column-batched far field)
 };
 // the symmetric sweep in four launches: the near field's lists and its
 // in-group gravity, the far field (no dependence among them: they may run

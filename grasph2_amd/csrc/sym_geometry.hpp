@@ -11,6 +11,8 @@
 
 #include <stdint.h>
 
+#include <vector>
+
 namespace grasph {
 
 // own slots per lane in the far field: the partner-side reduction and the
@@ -21,6 +23,65 @@ constexpr int kNI = 16;
 constexpr int kSB = 64 * kNI;           // slots per super-block (a far wavefront's slots)
 constexpr int kGroup = 1024;            // slots per group
 constexpr int kGroupSB = kGroup / kSB;  // super-blocks per group (wavefronts per far workgroup)
+
+static_assert(kGroupSB == 1, "the row geometry below takes one far wavefront per group");
+
+// Column batches: the far field may run in batches of partner groups
+// [c0, c1) (1 <= c0 < c1 <= nsb), so the partner-side partials, which grow
+// with n^2, need only the batch's share.  Row g of a batch holds its
+// partner groups [max(g + 1, c0), c1), each cut into Q pieces: rows
+// 0 .. c1 - 2 have units.  The unbatched far field is the batch [1, nsb).
+__device__ __host__ inline int64_t col_row_units(int64_t g, int64_t c0, int64_t c1, int Q)
+{
+    const int64_t first = g + 1 > c0 ? g + 1 : c0;
+    return first < c1 ? (c1 - first) * Q : 0;
+}
+// units before row g (g <= c1 - 1): rows below c0 hold c1 - c0 groups
+// each, row g' >= c0 holds c1 - 1 - g'
+__device__ __host__ inline int64_t col_row_prefix(int64_t g, int64_t c0, int64_t c1, int Q)
+{
+    const int64_t a = g < c0 ? g : c0, m = g > c0 ? g - c0 : 0;
+    return Q * (a * (c1 - c0) + m * (c1 - 1 - c0) - m * (m - 1) / 2);
+}
+__device__ __host__ inline int64_t col_rows(int64_t c1) { return c1 >= 2 ? c1 - 1 : 0; }
+// the row holding unit u of the batch: the largest g with col_row_prefix(g) <= u
+__device__ __host__ inline int64_t col_row_of_unit(int64_t u, int64_t c0, int64_t c1, int Q)
+{
+    int64_t lo = 0, hi = col_rows(c1) - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (col_row_prefix(mid, c0, c1, Q) <= u) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// first partner group of row g in the batch
+__device__ __host__ inline int64_t col_first_group(int64_t g, int64_t c0) { return g + 1 > c0 ? g + 1 : c0; }
+// first element (of 3 per partner and row) of the batch's partner-side
+// partials of group J: the groups c0 .. J - 1 before it take J' rows each
+__device__ __host__ inline int64_t col_bpart_base(int64_t J, int64_t c0)
+{
+    return (int64_t)kGroup * (J * (J - 1) / 2 - c0 * (c0 - 1) / 2);
+}
+
+// Column batches of the far field: boundaries c[0] = 1 < c[1] < ... <
+// c[last] = nsb, each batch [c[b], c[b + 1]) as wide as keeps its
+// partner-side partials within max_elems floats (one group at least).
+__host__ inline std::vector<int> sym_columns(int64_t nsb, int64_t max_elems)
+{
+    std::vector<int> c{1};
+    if (nsb <= 1) {
+        c.push_back(1);
+        return c;
+    }
+    int64_t c0 = 1;
+    while (c0 < nsb) {
+        int64_t c1 = c0 + 1;
+        while (c1 < nsb && 3 * col_bpart_base(c1 + 1, c0) <= max_elems) ++c1;
+        c.push_back((int)c1);
+        c0 = c1;
+    }
+    return c;
+}
 
 // far units of row g: each partner super-block of [kGroupSB (g + 1), nsb)
 // cut into Q pieces of kSB / Q partners (Q = 1, 2, 4, 8 or 16; more pieces

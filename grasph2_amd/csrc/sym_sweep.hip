@@ -501,7 +501,9 @@ __device__ __forceinline__ void far_fold_own(FarOwn& o, const float m, const int
 }
 
 // One far unit of row g: piece `piece` of Q (kSB / Q partners, Q >= 2) of
-// partner group J against the wavefront's 1024 slots.  The partner side is
+// partner group J against the wavefront's 1024 slots; the partials of group
+// J start at element 3 (bpart_group_base(J) - boff) of bpart (boff: the
+// groups before the column batch).  The partner side is
 // stored per chunk as row g's partials; the own side is folded at the end,
 // so a slot adds at most 512 terms in plain FP32 before they join its LDS
 // sum (and at most kFoldsPerFlush such sums before they join the
@@ -509,11 +511,11 @@ __device__ __forceinline__ void far_fold_own(FarOwn& o, const float m, const int
 template <bool UM, bool SOFT, int QC = 0>
 __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const int n, const int64_t g, const int J,
                                          const int piece, const int Qr, FarOwn& o, float* __restrict__ bpart,
-                                         const float H2, const float m, const int lane)
+                                         const int64_t boff, const float H2, const float m, const int lane)
 {
     const int Q = QC ? QC : Qr;  // QC: the split known at compile time (2 for the largest counts)
     const int len = kSB / Q, b0 = kSB * J + len * piece, nchunks = len / kRow;
-    float* const bp = bpart + 3 * (bpart_group_base(J) + g * kGroup);
+    float* const bp = bpart + 3 * (bpart_group_base(J) - boff + g * kGroup);
     if (b0 + len > n)
         far_segment_rot<UM, SOFT, true>(xs, n, b0, nchunks, J, o, H2, lane, bp, m);
     else
@@ -521,35 +523,61 @@ __device__ __forceinline__ void far_unit(const float4* __restrict__ xs, const in
     far_fold_own<UM>(o, m, lane);
 }
 
-// Far field of one instance.  Workgroup k of W takes the units
-// [k U / W, (k + 1) U / W) of the rows g (group g against the later
-// groups, each cut into Q pieces) laid end to end; its own-side sums of row
-// g go to fragment g + k (plain: no workgroup has more than kFoldsPerFlush
-// units).
-template <bool UM, bool SOFT, int QC>
-__device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb,
-                                         const int64_t U, const int W, const int Qr, float4* __restrict__ apart,
-                                         const bool plain, float* __restrict__ bpart, const float H2, const float m,
-                                         float* lsum)
+// Far field of one instance, or (COL) column batch [c0, c1) of its partner
+// groups.  Workgroup k of W takes the units [k U / W, (k + 1) U / W) of the
+// rows g (group g against its later groups, or those in the batch, each cut
+// into Q pieces) laid end to end; its own-side sums of row g go to fragment
+// g + k (plain: no workgroup has more than kFoldsPerFlush units).  The
+// whole far field keeps a kernel of its own (far_kernel), so the row
+// arithmetic of the batches does not change its code.
+template <bool UM, bool SOFT, int QC, bool COL>
+__device__ __forceinline__ void far_work(const float4* __restrict__ xs, const int n, const int64_t nsb, const int c0,
+                                         const int c1, const int64_t U, const int W, const int Qr,
+                                         float4* __restrict__ apart, const bool plain, float* __restrict__ bpart,
+                                         const float H2, const float m, float* lsum)
 {
     const int Q = QC ? QC : Qr;
     const int k = blockIdx.x, lane = threadIdx.x;
     const int64_t u0 = (int64_t)k * U / W, u1 = (int64_t)(k + 1) * U / W;
     if (u0 >= u1) return;
-    int64_t g = row_of_unit(u0, nsb, Q), s = u0 - row_prefix(g, nsb, Q);
+    const int64_t boff = COL ? bpart_group_base(c0) : 0;
+    int64_t g = COL ? col_row_of_unit(u0, c0, c1, Q) : row_of_unit(u0, nsb, Q);
+    int64_t s = u0 - (COL ? col_row_prefix(g, c0, c1, Q) : row_prefix(g, nsb, Q));
     FarOwn o;
     far_load_row(xs, n, g, lane, lsum, apart + (size_t)kFragElems * (g + k), plain, o);
     for (int64_t u = u0; u < u1; ++u) {
-        if (s == row_units(g, nsb, Q)) {
+        if (s == (COL ? col_row_units(g, c0, c1, Q) : row_units(g, nsb, Q))) {
             far_flush_own(o, lane);
             ++g;
             s = 0;
             far_load_row(xs, n, g, lane, lsum, apart + (size_t)kFragElems * (g + k), plain, o);
         }
-        far_unit<UM, SOFT, QC>(xs, n, g, (int)(g + 1 + s / Q), (int)(s % Q), Q, o, bpart, H2, m, lane);
+        const int J = (int)((COL ? col_first_group(g, c0) : g + 1) + s / Q);
+        far_unit<UM, SOFT, QC>(xs, n, g, J, (int)(s % Q), Q, o, bpart, boff, H2, m, lane);
         ++s;
     }
     far_flush_own(o, lane);
+}
+
+template <bool SOFT, bool COL>
+__device__ __forceinline__ void far_dispatch(const float4* __restrict__ xs, const int n, const int64_t nsb,
+                                             const int c0, const int c1, const int64_t U, const int W, const int Q,
+                                             float4* __restrict__ apart, const int plain,
+                                             float* __restrict__ bpart, const float H2,
+                                             const float* __restrict__ umass, float* lsum)
+{
+    const bool um = umass[0] != 0.0f;
+    if (Q == 2) {
+        if (um)
+            far_work<true, SOFT, 2, COL>(xs, n, nsb, c0, c1, U, W, 2, apart, plain, bpart, H2, umass[1], lsum);
+        else
+            far_work<false, SOFT, 2, COL>(xs, n, nsb, c0, c1, U, W, 2, apart, plain, bpart, H2, 1.0f, lsum);
+    } else {
+        if (um)
+            far_work<true, SOFT, 0, COL>(xs, n, nsb, c0, c1, U, W, Q, apart, plain, bpart, H2, umass[1], lsum);
+        else
+            far_work<false, SOFT, 0, COL>(xs, n, nsb, c0, c1, U, W, Q, apart, plain, bpart, H2, 1.0f, lsum);
+    }
 }
 
 template <bool SOFT>
@@ -560,18 +588,20 @@ __global__ __launch_bounds__(kFarThreads, 2) void far_kernel(const float4* __res
                                                             const float H2, const float* __restrict__ umass)
 {
     __shared__ float lsum[3 * kGroup];
-    const bool um = umass[0] != 0.0f;
-    if (Q == 2) {
-        if (um)
-            far_work<true, SOFT, 2>(xs, n, nsb, U, W, 2, apart, plain, bpart, H2, umass[1], lsum);
-        else
-            far_work<false, SOFT, 2>(xs, n, nsb, U, W, 2, apart, plain, bpart, H2, 1.0f, lsum);
-    } else {
-        if (um)
-            far_work<true, SOFT, 0>(xs, n, nsb, U, W, Q, apart, plain, bpart, H2, umass[1], lsum);
-        else
-            far_work<false, SOFT, 0>(xs, n, nsb, U, W, Q, apart, plain, bpart, H2, 1.0f, lsum);
-    }
+    far_dispatch<SOFT, false>(xs, n, nsb, 1, (int)nsb, U, W, Q, apart, plain, bpart, H2, umass, lsum);
+}
+
+// one column batch [c0, c1) of the far field
+template <bool SOFT>
+__global__ __launch_bounds__(kFarThreads, 2) void far_column_kernel(const float4* __restrict__ xs, const int n,
+                                                                   const int c0, const int c1, const int64_t U,
+                                                                   const int W, const int Q,
+                                                                   float4* __restrict__ apart, const int plain,
+                                                                   float* __restrict__ bpart, const float H2,
+                                                                   const float* __restrict__ umass)
+{
+    __shared__ float lsum[3 * kGroup];
+    far_dispatch<SOFT, true>(xs, n, 0, c0, c1, U, W, Q, apart, plain, bpart, H2, umass, lsum);
 }
 
 // Per slot, in a fixed order: the near partials (share order), the far own-side fragments
@@ -674,6 +704,88 @@ __global__ __launch_bounds__(kBlock) void sym_combine_kernel(
         }
     }
     if (t >= n) return;
+    const int i = perm ? perm[t] : t;
+    if (add_acc) {  // the SPH terms (and environment), summed before the far field was done
+        const float4 a = acc[i];
+        x.add(a.x); y.add(a.y); z.add(a.z);
+    }
+    float ax = x.value(), ay = y.value(), az = z.value();
+    if (finish) {
+        environment_acceleration(P, ax, ay, az);
+        if (xvel) xvel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (maxvsig) maxvsig[i] = 0.0f;
+    }
+    acc[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// After column batch [c0, c1) of the far field (column-batched far field,
+// one thread per slot): the slot's running compensated sums (accsum: 2
+// float4 per slot, sums then corrections; none before the first batch)
+// take the batch's own-side fragments of the slot's row and, for slots in
+// the batch's groups, their partner-side partials, complete now (every row
+// before the group was in the batch).  After the last batch the near
+// partials join them and the acceleration is finished as in the combine.
+__global__ __launch_bounds__(kBlock) void sym_column_combine_kernel(
+    const float4* __restrict__ near_part, const float4* __restrict__ apart, const float* __restrict__ bpart,
+    float4* __restrict__ accsum, const int n, const int nshares, const int c0, const int c1, const int64_t U,
+    const int W, const int Q, const int plain, const int first, const int last, const int* __restrict__ perm,
+    float4* __restrict__ acc, const int finish, const int add_acc, float4* __restrict__ xvel,
+    float* __restrict__ maxvsig, const DevParams P)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    CSum x{0.0f, 0.0f}, y{0.0f, 0.0f}, z{0.0f, 0.0f};
+    if (!first) {
+        const float4 a = accsum[2 * (size_t)t], c = accsum[2 * (size_t)t + 1];
+        x = CSum{a.x, c.x}; y = CSum{a.y, c.y}; z = CSum{a.z, c.z};
+    }
+    const int g = t / kGroup, local = t - g * kGroup;
+    if (g < col_rows(c1) && U > 0) {
+        // the fragments of the workgroups that worked on row g in this batch
+        const int64_t k0 = wg_of_unit(col_row_prefix(g, c0, c1, Q), U, W);
+        const int64_t k1 = wg_of_unit(col_row_prefix(g + 1, c0, c1, Q) - 1, U, W);
+        const int el = frag_elem(local);
+        for (int64_t k = k0; k <= k1; ++k) {
+            const float4* fr = apart + (size_t)kFragElems * (g + k) + el;
+            const float4 f = fr[0];
+            x.add(f.x); y.add(f.y); z.add(f.z);
+            if (!plain) {
+                const float4 fc = fr[64];
+                x.c += fc.x; y.c += fc.y; z.c += fc.z;
+            }
+        }
+    }
+    if (g >= c0 && g < c1) {
+        const int64_t base = col_bpart_base(g, c0) + local;
+        int gp = 0;  // eight rows' loads in flight before their (ordered) adds
+        for (; gp + 8 <= g; gp += 8) {
+            float v[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const size_t e = 3 * (size_t)(base + (int64_t)(gp + u) * kGroup);
+                v[u][0] = bpart[e]; v[u][1] = bpart[e + 1]; v[u][2] = bpart[e + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                x.add(-v[u][0]); y.add(-v[u][1]); z.add(-v[u][2]);
+            }
+        }
+        for (; gp < g; ++gp) {
+            const size_t e = 3 * (size_t)(base + (int64_t)gp * kGroup);
+            x.add(-bpart[e]); y.add(-bpart[e + 1]); z.add(-bpart[e + 2]);
+        }
+    }
+    if (!last) {
+        accsum[2 * (size_t)t] = make_float4(x.s, y.s, z.s, 0.0f);
+        accsum[2 * (size_t)t + 1] = make_float4(x.c, y.c, z.c, 0.0f);
+        return;
+    }
+    for (int sl = 0; sl < nshares; ++sl) {
+        const size_t e = 2 * ((size_t)sl * n + t);
+        const float4 a = near_part[e], c = near_part[e + 1];
+        x.add(a.x); y.add(a.y); z.add(a.z);
+        x.c += c.x; y.c += c.y; z.c += c.z;
+    }
     const int i = perm ? perm[t] : t;
     if (add_acc) {  // the SPH terms (and environment), summed before the far field was done
         const float4 a = acc[i];
@@ -791,7 +903,8 @@ __device__ __forceinline__ void batch_far_row(const float4* __restrict__ xk, con
 {
     FarOwn o;
     far_load_row(xk, n, R.g, lane, lsum, frag, plain, o);
-    for (int s = 0; s < 2 * R.units; ++s) far_unit<UM, SOFT, 2>(xk, n, R.g, j0 + (s >> 1), s & 1, 2, o, bk, H2, m, lane);
+    for (int s = 0; s < 2 * R.units; ++s)
+        far_unit<UM, SOFT, 2>(xk, n, R.g, j0 + (s >> 1), s & 1, 2, o, bk, 0, H2, m, lane);
     far_flush_own(o, lane);
 }
 
@@ -920,7 +1033,20 @@ int sym_default_workgroups(int cus)
     return std::max(1, 8 * cus / kGroupSB);  // 2 wavefronts per SIMD
 }
 
-SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
+// one column batch's units, far workgroups (of `wanted`) and fragment kind
+SymCol sym_column(int64_t nsb, int split, int wanted, int c0, int c1)
+{
+    SymCol C;
+    C.c0 = c0;
+    C.c1 = c1;
+    C.units = col_row_prefix(col_rows(c1), c0, c1, split);
+    C.wgs = far_workgroups(wanted, C.units);
+    C.plain = fragments_plain((C.units + C.wgs - 1) / C.wgs);
+    (void)nsb;
+    return C;
+}
+
+SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares, int64_t col_elems)
 {
     SymGeom G;
     G.nb = (int)((n + 63) / 64);
@@ -936,6 +1062,13 @@ SymGeom sym_geometry(int64_t n, int wgs, bool want_nbr, int nshares)
     G.apart_elems = (int64_t)kFragElems * (G.groups + wgs);
     G.plain = fragments_plain((G.units + G.wgs - 1) / G.wgs);
     G.bpart_elems = 3 * bpart_group_base(G.groups);
+    G.ncols = 1;
+    if (G.bpart_elems > col_elems) {
+        const std::vector<int> c = sym_columns(G.nsb, col_elems);
+        G.ncols = (int)c.size() - 1;
+        G.bpart_elems = 0;
+        for (int b = 0; b < G.ncols; ++b) G.bpart_elems = std::max(G.bpart_elems, 3 * col_bpart_base(c[b + 1], c[b]));
+    }
     return G;
 }
 
@@ -971,17 +1104,44 @@ hipError_t launch_sym_ingroup(const SymLaunch& L, const DevParams& P, hipStream_
 
 bool sym_has_far(const SymLaunch& L) { return L.n > 0 && L.grav && sym_geometry(L.n, L.wgs, false, 1).units > 0; }
 
-// far field: gravity between groups, once per pair (needs only xs and umass)
+template <bool SOFT>
+void launch_far_column(const SymLaunch& L, const SymCol& C, const int split, const DevParams& P, hipStream_t s)
+{
+    hipLaunchKernelGGL((far_column_kernel<SOFT>), dim3(C.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, C.c0, C.c1,
+                       C.units, C.wgs, split, L.apart, C.plain ? 1 : 0, L.bpart, P.H2, L.umass);
+}
+
+void launch_column_combine(const SymLaunch& L, const SymCol& C, const int split, const bool first, const bool last,
+                           const DevParams& P, hipStream_t s)
+{
+    hipLaunchKernelGGL(sym_column_combine_kernel, dim3((L.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, L.near_part,
+                       L.apart, L.bpart, L.accsum, L.n, L.nshares, C.c0, C.c1, C.units, C.wgs, split, C.plain ? 1 : 0,
+                       first ? 1 : 0, last ? 1 : 0, L.perm, L.acc, L.finish ? 1 : 0, L.add_acc ? 1 : 0, L.xvel,
+                       L.maxvsig, P);
+}
+
+// far field: gravity between groups, once per pair (needs only xs and
+// umass); in column batches when the partials of the whole would not fit,
+// each batch but the last followed by its column combine
 hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (!sym_has_far(L)) return hipSuccess;
-    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
-    if (P.H2 > 0.0f)
-        hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
-                           G.split, L.apart, G.plain ? 1 : 0, L.bpart, P.H2, L.umass);
-    else
-        hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units, L.wgs,
-                           G.split, L.apart, G.plain ? 1 : 0, L.bpart, P.H2, L.umass);
+    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares, L.col_elems);
+    if (G.ncols == 1) {
+        if (P.H2 > 0.0f)
+            hipLaunchKernelGGL((far_kernel<true>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units,
+                               L.wgs, G.split, L.apart, G.plain ? 1 : 0, L.bpart, P.H2, L.umass);
+        else
+            hipLaunchKernelGGL((far_kernel<false>), dim3(L.wgs), dim3(kFarThreads), 0, s, L.xs, L.n, G.nsb, G.units,
+                               L.wgs, G.split, L.apart, G.plain ? 1 : 0, L.bpart, P.H2, L.umass);
+        return hipGetLastError();
+    }
+    const std::vector<int> c = sym_columns(G.nsb, L.col_elems);
+    for (int b = 0; b < G.ncols; ++b) {
+        const SymCol C = sym_column(G.nsb, G.split, L.wgs, c[b], c[b + 1]);
+        if (P.H2 > 0.0f) launch_far_column<true>(L, C, G.split, P, s); else launch_far_column<false>(L, C, G.split, P, s);
+        if (b + 1 < G.ncols) launch_column_combine(L, C, G.split, b == 0, false, P, s);
+    }
     return hipGetLastError();
 }
 
@@ -989,7 +1149,14 @@ hipError_t launch_sym_far(const SymLaunch& L, const DevParams& P, hipStream_t s)
 hipError_t launch_sym_combine(const SymLaunch& L, const DevParams& P, hipStream_t s)
 {
     if (L.n <= 0 || !(L.grav || L.finish)) return hipSuccess;
-    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares);
+    const SymGeom G = sym_geometry(L.n, L.wgs, L.want_nbr, L.nshares, L.col_elems);
+    if (G.ncols > 1 && sym_has_far(L)) {
+        // the last column batch's combine, which finishes
+        const std::vector<int> c = sym_columns(G.nsb, L.col_elems);
+        const SymCol C = sym_column(G.nsb, G.split, L.wgs, c[G.ncols - 1], c[G.ncols]);
+        launch_column_combine(L, C, G.split, false, true, P, s);
+        return hipGetLastError();
+    }
     // threads per slot: about 2^17 threads or more in all
     const int parts = L.n >= 98304 ? 1 : L.n >= 49152 ? 2 : L.n >= 24576 ? 4 : 8;
 #define GRASPH_COMBINE(PARTS, PLAIN)                                                                                  \

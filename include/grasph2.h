@@ -167,7 +167,12 @@ int grasph_destroy(grasph_ctx* ctx);
  * 16384 particles on, the default; results agree to FP32 rounding, the
  * neighbour lists are identical), "sym_workgroups" (workgroups of the
  * symmetric sweep's far-field kernel, single instances and batches, 0 = two
- * wavefronts per SIMD),
+ * wavefronts per SIMD), "sym_partial_bytes" (single instance: device memory
+ * the symmetric far field's partner-side partials may take, at least 1 MiB,
+ * default 12 GiB; they need 6 n^2 / 1024 bytes in all, and past this the far
+ * field runs in column batches of partner groups, each with its share of
+ * the partials, the running sums kept between batches in 32 B per particle;
+ * results agree with the unbatched far field to FP32 rounding),
  * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
  * 0 = chosen from the occupancy), "sym_list_priority" (its near-field list
  * kernel takes issue priority over the far field running beside it: 1 = yes,

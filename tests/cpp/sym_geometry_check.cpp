@@ -13,8 +13,10 @@
 //  - every fragment is flushed at least once, and exactly once when the
 //    geometry calls the fragments plain (the combine then reads no
 //    correction planes).
-// Also the batch form: the rows table covers each (instance, row) once, and
-// each row's pieces cover its partner groups once.
+// Also the column-batched far field (partner groups in batches, each with
+// its own partials and fragments), and the batch form: the rows table
+// covers each (instance, row) once, and each row's pieces cover its
+// partner groups once.
 // Prints "ok <cases>" and exits 0, or prints the first failure and exits 1.
 #include <cstdio>
 #include <cstdlib>
@@ -126,6 +128,104 @@ static int check(int64_t n, int64_t wanted, int Q)
     return 0;
 }
 
+// Column batches: the far field of n particles in batches of partner groups
+// (sym_columns with max_elems floats of partials each), each replayed as
+// far_work does it with its own workgroup count, and the column combine's
+// reads after each batch.  Every pair of groups is computed once over all
+// batches, and each batch's partials and fragments stay in its arrays.
+static int check_columns(int64_t n, int64_t wanted, int Q, int64_t max_elems)
+{
+    const int64_t nsb = (n + kSB - 1) / kSB, G = nsb;
+    const std::vector<int> c = sym_columns(nsb, max_elems);
+    if (c.front() != 1 || c.back() != (nsb > 1 ? nsb : 1)) return fail("column ends", n, wanted, c.front(), c.back());
+    std::vector<char> seen((size_t)(nsb * nsb * Q), 0);
+    const int64_t apart_elems = (int64_t)kFragElems * (G + wanted);
+    for (size_t b = 0; b + 1 < c.size(); ++b) {
+        const int64_t c0 = c[b], c1 = c[b + 1];
+        if (c1 <= c0 && nsb > 1) return fail("empty column", n, wanted, c0, c1);
+        const int64_t elems = 3 * col_bpart_base(c1, c0);
+        if (elems > max_elems && c1 > c0 + 1) return fail("column partials", n, wanted, c0, c1);
+        const int64_t U = col_row_prefix(col_rows(c1), c0, c1, Q);
+        int64_t sum = 0;
+        for (int64_t g = 0; g < nsb; ++g) sum += col_row_units(g, c0, c1, Q);
+        if (sum != U) return fail("column unit total", n, wanted, sum, U);
+        if (U == 0) continue;
+        const int64_t W = far_workgroups(wanted, U);
+        const bool plain = fragments_plain((U + W - 1) / W);
+        std::vector<char> bw((size_t)(elems / 3), 0);
+        std::set<std::pair<int64_t, int64_t>> frag;
+        std::map<std::pair<int64_t, int64_t>, int> flushes;
+        for (int64_t k = 0; k < W; ++k) {
+            const int64_t u0 = k * U / W, u1 = (k + 1) * U / W;
+            if (u0 >= u1) return fail("column empty share", n, W, k);
+            int64_t g = col_row_of_unit(u0, c0, c1, Q), s = u0 - col_row_prefix(g, c0, c1, Q);
+            if (s < 0 || s >= col_row_units(g, c0, c1, Q)) return fail("column row search", n, W, u0, g);
+            int folds = 0;
+            for (int64_t u = u0; u < u1; ++u) {
+                if (s == col_row_units(g, c0, c1, Q)) {
+                    frag.insert({g, k});
+                    if (folds > 0) ++flushes[{g, k}];
+                    folds = 0;
+                    ++g;
+                    s = 0;
+                    if (g >= col_rows(c1)) return fail("column row past the last", n, W, g);
+                }
+                const int64_t J = col_first_group(g, c0) + s / Q, piece = s % Q;
+                if (J < c0 || J >= c1 || J <= g) return fail("column partner group", n, W, g, J);
+                char& sc = seen[(size_t)((g * nsb + J) * Q + piece)];
+                if (sc) return fail("column pair twice", n, W, g, J);
+                sc = 1;
+                const int64_t len = kSB / Q, b0 = kSB * J + len * piece;
+                for (int64_t B = b0; B < b0 + len && B < n; ++B) {
+                    // as far_unit indexes it: bpart_group_base(J) - bpart_group_base(c0) + g kGroup + local
+                    const int64_t x = bpart_group_base(J) - bpart_group_base(c0) + g * kGroup + (B - J * kGroup);
+                    if (x < 0 || 3 * x + 2 >= elems) return fail("column partial bounds", n, W, g, B);
+                    if (bw[(size_t)x]) return fail("column partial twice", n, W, g, B);
+                    bw[(size_t)x] = 1;
+                }
+                ++s;
+                if (++folds == kFoldsPerFlush) {
+                    ++flushes[{g, k}];
+                    folds = 0;
+                }
+            }
+            frag.insert({g, k});
+            if (folds > 0) ++flushes[{g, k}];
+        }
+        for (const auto& f : frag) {
+            const int cnt = flushes.count(f) ? flushes[f] : 0;
+            if (cnt < 1 || (plain && cnt != 1)) return fail("column flushes", n, W, f.first, f.second);
+            if ((f.first + f.second + 1) * (int64_t)kFragElems > apart_elems) return fail("column fragment bounds", n, W);
+        }
+        // what sym_column_combine_kernel reads after this batch
+        size_t reads = 0;
+        for (int64_t g = 0; g < G; ++g) {
+            if (g < col_rows(c1)) {
+                const int64_t k0 = wg_of_unit(col_row_prefix(g, c0, c1, Q), U, W);
+                const int64_t k1 = wg_of_unit(col_row_prefix(g + 1, c0, c1, Q) - 1, U, W);
+                for (int64_t k = k0; k <= k1; ++k) {
+                    if (!frag.count({g, k})) return fail("column fragment read, not written", n, W, g, k);
+                    ++reads;
+                }
+            }
+            if (g >= c0 && g < c1)
+                for (int64_t t = g * kGroup; t < (g + 1) * kGroup && t < n; ++t)
+                    for (int64_t gp = 0; gp < g; ++gp)
+                        if (!bw[(size_t)(col_bpart_base(g, c0) + gp * kGroup + (t - g * kGroup))])
+                            return fail("column partial read, not written", n, W, gp, t);
+        }
+        if (reads != frag.size()) return fail("column fragment written, not read", n, W, (int64_t)reads, (int64_t)frag.size());
+    }
+    for (int64_t I = 0; I < nsb; ++I)
+        for (int64_t J = 0; J < nsb; ++J)
+            for (int p = 0; p < Q; ++p)
+                if ((I < J) != (seen[(size_t)((I * nsb + J) * Q + p)] != 0)) return fail("column coverage", n, wanted, I, J);
+    // one batch is the unbatched far field
+    for (int64_t g = 0; g <= rows_with_units(nsb); ++g)
+        if (col_row_prefix(g, 1, nsb, Q) != row_prefix(g, nsb, Q)) return fail("column prefix", n, wanted, g);
+    return 0;
+}
+
 // batch form: one row per (instance, g < rows_with_units), fragments numbered row0 + g
 static int check_batch(const std::vector<int64_t>& sizes)
 {
@@ -183,6 +283,16 @@ int main()
             if (U > 0 && far_split(n, far_workgroups(w, U)) != Q) return fail("split not stable", n, w, Q);
             ++cases;
         }
+    // column batches: from one group per batch to the whole far field
+    const int64_t cns[] = {1, 1025, 3000, 20037, 40000, 131072};
+    const int64_t cel[] = {1, 3 * 1024 * 3, 3 * 1024 * 20, 3 * 1024 * 300, int64_t(1) << 40};
+    for (int64_t n : cns)
+        for (int64_t e : cel)
+            for (int64_t w : {1, 7, 64, 2048})
+                for (int Q : {2, 4, 16}) {
+                    if (check_columns(n, w, Q, e)) return 1;
+                    ++cases;
+                }
     std::vector<int64_t> sizes;
     for (int i = 0; i < 200; ++i) sizes.push_back(rand() % 17000);
     if (check_batch(sizes)) return 1;

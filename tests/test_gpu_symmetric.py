@@ -61,6 +61,50 @@ def test_gravity_few_far_workgroups(sym_ctx, n, wgs):
     assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
 
 
+# Column batches: with 1 MiB for the partner-side partials, 20037 particles
+# (20 groups) take three batches of partner groups, [1, 13), [13, 18) and
+# [18, 20), with running sums between them; with 3 far workgroups each
+# batch's fragments are also flushed several times (with corrections).
+@pytest.mark.parametrize("wgs", [0, 3])
+def test_gravity_column_batches(sym_ctx, wgs):
+    print("symmetric gravity in column batches, sym_workgroups=%d" % wgs, flush=True)
+    h, prm = ics.plummer(20037, seed=21)
+    base = gpu_run(sym_ctx, h, prm)
+    sym_ctx.set_option("sym_partial_bytes", 1 << 20)
+    sym_ctx.set_option("sym_workgroups", wgs)
+    try:
+        out = gpu_run(sym_ctx, h, prm)
+        again = gpu_run(sym_ctx, h, prm)
+    finally:
+        sym_ctx.set_option("sym_partial_bytes", 12 << 30)
+        sym_ctx.set_option("sym_workgroups", 0)
+    assert np.array_equal(out["acc"], again["acc"])  # run to run
+    ref = O.calc_acceleration(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
+    mx, nr = vec_errors(out["acc"], base["acc"])
+    assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
+
+
+def test_sph_column_batches(sym_ctx):
+    """The solid case in column batches: the acceleration pass's SPH terms
+    join the gravity in the last batch's combine; the neighbour sums are the
+    unbatched run's bit for bit."""
+    print("symmetric SPH in column batches", flush=True)
+    h, prm = solid_case(20000, seed=23)
+    base = gpu_run(sym_ctx, h, prm)
+    sym_ctx.set_option("sym_partial_bytes", 1 << 20)
+    try:
+        out = gpu_run(sym_ctx, h, prm)
+    finally:
+        sym_ctx.set_option("sym_partial_bytes", 12 << 30)
+    for f in ("density", "dstress_dt", "balsara", "maxvsig"):
+        assert np.array_equal(out[f], base[f]), f
+    mx, nr = vec_errors(out["acc"], base["acc"])
+    assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
+    ref = O.compute_derivatives(prm, h, "f64")
+    check_vec(out["acc"], ref["acc"])
+
+
 # Equal masses (the mass factored out of both sides' sums), random masses,
 # and equal masses but for the last 50 particles (the equal-mass check
 # sees the exception at the end).
