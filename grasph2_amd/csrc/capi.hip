@@ -365,12 +365,15 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
             if ((e = launch_sym_far(S, P, s)) != hipSuccess) return hip_fail(c, e, "far-field launch");
         }
         ws = c.side;
-        // Below ~100k particles the far field's window is short next to the
+        // Below ~50k particles the far field's window is short next to the
         // lists' work: there the lists take issue priority, so they and the
-        // passes after them end inside it (liquid N = 65536: 1.12 -> 1.05 ms
-        // per step); above, their VALU work would lengthen the far field by
-        // more than it saves (collision N = 131072: 3.79 -> 3.86 ms).
-        S.list_prio = c.sym_list_priority == 1 || (c.sym_list_priority == 2 && L.n < 98304);
+        // passes after them end inside it (collision N = 32768: 0.435 ->
+        // 0.410 ms per step, N = 16384: 0.290 -> 0.283; liquid N = 32768:
+        // 0.420 -> 0.387); from 65536 on the side stream ends inside the
+        // window without it, and its VALU work would lengthen the far field
+        // (collision N = 65536: 0.985 -> 1.008 ms, liquid N = 65536: 0.945
+        // -> 1.005, collision N = 131072: 3.79 -> 3.86).
+        S.list_prio = c.sym_list_priority == 1 || (c.sym_list_priority == 2 && L.n < 49152);
     }
     ProfScope ps(&c, ws, KC_SWEEP);
     if ((e = launch_sym_lists(S, P, ws)) != hipSuccess) return hip_fail(c, e, "near-field list launch");
