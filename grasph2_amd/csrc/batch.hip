@@ -91,7 +91,7 @@ struct BatchState {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     // the symmetric sweep's far field: its own stream, forked and joined
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_lists = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string graph_key;
@@ -299,7 +299,8 @@ int prepare(Ctx& c, const grasph_particles* p, const grasph_batch* bt)
             hipEventCreateWithFlags(&b.ev_out, hipEventDisableTiming) != hipSuccess ||
             hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b.ev_lists, hipEventDisableTiming) != hipSuccess)
             return hip_fail(c, hipGetLastError(), "batch stream");
     }
     return 0;
@@ -520,19 +521,27 @@ int sym_batch_begin(Ctx& c, BatchState& b, const grasph_particles* p, hipStream_
             ProfScope ps(&c, fs, KC_FAR);
             if ((e = launch_sym_batch_far(L, fs)) != hipSuccess) return hip_fail(c, e, "batched far-field launch");
         }
-        if (fork) {
-            // the gravity inside the groups needs only the positions: after the
-            // far field on its stream, beside the neighbour passes on s
-            ProfScope ps(&c, fs, KC_SWEEP);
-            if ((e = launch_sym_batch_ingroup(L, fs)) != hipSuccess)
+    }
+    {
+        ProfScope ps(&c, s, KC_SWEEP);
+        if ((e = launch_sym_batch_lists(L, s)) != hipSuccess) return hip_fail(c, e, "batched near-field list launch");
+    }
+    if (fork && sym_batch_has_far(L)) {
+        // the gravity inside the groups needs only the positions: after the
+        // far field on its stream, once the lists are done, so that it shares
+        // the vector units with the memory-bound passes on s rather than with
+        // the list kernel, which the passes wait for
+        if (L.want_nbr && ((e = hipEventRecord(b.ev_lists, s)) != hipSuccess ||
+                           (e = hipStreamWaitEvent(b.side, b.ev_lists, 0)) != hipSuccess))
+            return hip_fail(c, e, "batched list event");
+        {
+            ProfScope ps(&c, b.side, KC_SWEEP);
+            if ((e = launch_sym_batch_ingroup(L, b.side)) != hipSuccess)
                 return hip_fail(c, e, "batched near-field gravity launch");
             L.ingroup_done = true;
         }
-        if (fork && (e = hipEventRecord(b.ev_join, b.side)) != hipSuccess)
-            return hip_fail(c, e, "batched far-field join event");
+        if ((e = hipEventRecord(b.ev_join, b.side)) != hipSuccess) return hip_fail(c, e, "batched far-field join event");
     }
-    ProfScope ps(&c, s, KC_SWEEP);
-    if ((e = launch_sym_batch_lists(L, s)) != hipSuccess) return hip_fail(c, e, "batched near-field list launch");
     return 0;
 }
 
@@ -799,6 +808,7 @@ void destroy_batch_state(BatchState* b)
     if (b->gs) (void)hipStreamDestroy(b->gs);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev_lists) (void)hipEventDestroy(b->ev_lists);
     if (b->side) (void)hipStreamDestroy(b->side);
     for (grasph_ctx* x : b->loop)
         if (x) grasph_destroy(x);
