@@ -174,6 +174,8 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
                                                          float* __restrict__ density, const BlockTables T,
                                                          const DevParams P0)
 {
+    // batches: issue ahead of the far field's wavefronts that run beside
+    if (BATCH) __builtin_amdgcn_s_setprio(3);
     const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
     const int t = pass_slot<SUB>(m), sub = pass_sub<SUB>();
     const bool act = t < m.slot_end;
@@ -204,6 +206,8 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     const int* __restrict__ perm, const Lists Ls, const size_t nstride, const int kmax, float* __restrict__ balsara,
     float* __restrict__ density_dt, float* __restrict__ dstress_dt, const BlockTables T, const DevParams P0)
 {
+    // batches: issue ahead of the far field's wavefronts that run beside
+    if (BATCH) __builtin_amdgcn_s_setprio(3);
     const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
     const int t0 = pass_slot<SUB>(m), sub = pass_sub<SUB>();
     const bool act = t0 < m.slot_end;
@@ -342,6 +346,8 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
     const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
+    // batches: issue ahead of the far field's wavefronts that run beside
+    if (BATCH) __builtin_amdgcn_s_setprio(3);
     const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
     const int t0 = pass_slot<SUB>(m), sub = pass_sub<SUB>();
     const bool act = t0 < m.slot_end;
