@@ -140,8 +140,9 @@ def main():
         args.steps = 100 if args.workload == "plummer" else 10
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the sweep would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the sweep would shard by "
+              "target particle, with an all-gather of the positions per step); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -5,14 +5,15 @@ acceleration: the HIP path's acc after one computeDerivatives against the
 FP64 oracle evaluated on a sample of particles through the whole chain
 (density on the sample's neighbours' neighbours, strain / Balsara on its
 neighbours, acceleration on the sample), errors as in tests/_util.py, bound
-1e-5 (north_star).  The FP32 oracle on the same sample is reported beside it:
+1e-5 (the accuracy bound).  The FP32 oracle on the same sample is reported beside it:
 that is the reference's own FP32 arithmetic.
 energy: FP64 invariants (kinetic with the half-kick velocities synchronised,
 softened gravitational potential, compressional + elastic) over the
 workload's step count with integrate(pb, true) as the reference's main loop;
 the drift is reported and bounded loosely (viscosity and plasticity
 dissipate, so SPH runs do not conserve it).  Results go to stdout and, when
-the folder exists, gpurun_out/accuracy.json.
+the environment variable GRASPH_RESULTS_DIR names a folder, to
+accuracy.json there.
 """
 import json
 import os
@@ -34,8 +35,9 @@ RESULTS = {}
 def record(name, **kv):
     RESULTS.setdefault(name, {}).update(kv)
     print("[accuracy]", name, json.dumps(kv), flush=True)
-    if os.path.isdir("gpurun_out"):
-        with open("gpurun_out/accuracy.json", "w") as fh:
+    out = os.environ.get("GRASPH_RESULTS_DIR")
+    if out and os.path.isdir(out):
+        with open(os.path.join(out, "accuracy.json"), "w") as fh:
             json.dump(RESULTS, fh, indent=1)
 
 

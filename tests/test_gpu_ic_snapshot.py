@@ -8,7 +8,7 @@ The program checks the snapshot read back against the state downloaded when
 printToFile was called (the simulation goes on meanwhile).  Here the same
 initial conditions, read from what the program generated, are stepped by the
 FP64 oracle, and the state after the steps is compared: the acceleration of
-the last derivative evaluation at 1e-5 (north_star), positions and
+the last derivative evaluation at 1e-5 (the accuracy bound), positions and
 velocities to the FP32 rounding the steps accumulate.
 """
 import os

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle.
 
 Tolerance: relative 1e-5 against the FP64 oracle for acceleration and
-density (north_star's bound), computed with tests/_util.py's metrics:
+density (the accuracy bound), computed with tests/_util.py's metrics:
   - norm-wise relative error <= 1e-5;
   - every particle's error relative to max(|a|, 0.1 median |a|) <= 1e-5
     (vec_errors_floored: the relative bound wherever |a| is at least a
