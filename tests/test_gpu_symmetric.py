@@ -85,6 +85,16 @@ def test_gravity_column_batches(sym_ctx, wgs):
     assert nr <= 1e-6 and mx <= 1e-5, (mx, nr)
 
 
+def test_partial_budget_option(sym_ctx):
+    """sym_partial_bytes: read back as set, at least 1 MiB."""
+    assert sym_ctx.get_option("sym_partial_bytes") == 12 << 30
+    with pytest.raises(g.GrasphError):
+        sym_ctx.set_option("sym_partial_bytes", (1 << 20) - 1)
+    sym_ctx.set_option("sym_partial_bytes", 1 << 20)
+    assert sym_ctx.get_option("sym_partial_bytes") == 1 << 20
+    sym_ctx.set_option("sym_partial_bytes", 12 << 30)
+
+
 def test_sph_column_batches(sym_ctx):
     """The solid case in column batches: the acceleration pass's SPH terms
     join the gravity in the last batch's combine; the neighbour sums are the
