@@ -372,8 +372,13 @@ int sym_sweep_begin(Ctx& c, const SweepLaunch& L, const DevParams& P, hipStream_
         // 0.420 -> 0.387); from 65536 on the side stream ends inside the
         // window without it, and its VALU work would lengthen the far field
         // (collision N = 65536: 0.985 -> 1.008 ms, liquid N = 65536: 0.945
-        // -> 1.005, collision N = 131072: 3.79 -> 3.86).
-        S.list_prio = c.sym_list_priority == 1 || (c.sym_list_priority == 2 && L.n < 49152);
+        // -> 1.005, collision N = 131072: 3.79 -> 3.86).  Once the lists have
+        // grown long (capacity past 512: bodies in contact), the lists and
+        // the passes after them outlast the far field at any size, and the
+        // priority shortens the step (collision N = 131072 after impact:
+        // 5.87 -> 5.57 ms).
+        S.list_prio = c.sym_list_priority == 1 ||
+                      (c.sym_list_priority == 2 && (L.n < 49152 || c.max_neighbours > 512));
     }
     ProfScope ps(&c, ws, KC_SWEEP);
     if ((e = launch_sym_lists(S, P, ws)) != hipSuccess) return hip_fail(c, e, "near-field list launch");

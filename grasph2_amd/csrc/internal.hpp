@@ -59,7 +59,7 @@ struct Ctx {
     int sym_near_shares = 0;  // wavefronts per block of its near field (0: chosen from the occupancy)
     // bytes of partner-side partials one column batch of its far field may take (6 n^2 / 1024 in all)
     int64_t sym_partial_bytes = int64_t(12) << 30;
-    int sym_list_priority = 2;  // its near-field lists ahead of the far field: 0 no, 1 yes, 2 below 49152 particles
+    int sym_list_priority = 2;  // its near-field lists ahead of the far field: 0 no, 1 yes, 2 below 49152 particles or past 512 list entries
     int batch_symmetric = 1;  // batches: the symmetric sweep for instances of batch_symmetric_min particles or more
     int batch_symmetric_min = 0;
     int cus = 256;         // compute units of the device

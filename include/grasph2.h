@@ -176,7 +176,8 @@ int grasph_destroy(grasph_ctx* ctx);
  * "sym_near_shares" (wavefronts per 64 particles in its near field, 1-16,
  * 0 = chosen from the occupancy), "sym_list_priority" (its near-field list
  * kernel takes issue priority over the far field running beside it: 1 = yes,
- * 0 = no, 2 = below 49152 particles, the default), "batch_symmetric" (1 = batches use the
+ * 0 = no, 2 = below 49152 particles or once max_neighbours has grown past
+ * 512, the default), "batch_symmetric" (1 = batches use the
  * symmetric sweep for their instances of at least "batch_symmetric_min"
  * particles (default 0), the smaller ones the one-sided persistent sweep,
  * the default; gravity then equals the single-instance path's to FP32
