@@ -174,8 +174,11 @@ __global__ __launch_bounds__(kBlock) void density_kernel(const float4* __restric
                                                          float* __restrict__ density, const BlockTables T,
                                                          const DevParams P0)
 {
-    // batches: issue ahead of the far field's wavefronts that run beside
-    if (BATCH) __builtin_amdgcn_s_setprio(3);
+    // batches, and the long lists of bodies in contact (SUB > 1): issue
+    // ahead of the far field's wavefronts that run beside (ensemble of 512:
+    // 16.25-16.30 -> 15.98-16.16 ms per step; collision N = 131072 after
+    // impact: 5.59-5.62 -> 5.03-5.23)
+    if (BATCH || SUB > 1) __builtin_amdgcn_s_setprio(3);
     const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
     const int t = pass_slot<SUB>(m), sub = pass_sub<SUB>();
     const bool act = t < m.slot_end;
@@ -206,8 +209,11 @@ __global__ __launch_bounds__(kBlock) void strain_kernel(
     const int* __restrict__ perm, const Lists Ls, const size_t nstride, const int kmax, float* __restrict__ balsara,
     float* __restrict__ density_dt, float* __restrict__ dstress_dt, const BlockTables T, const DevParams P0)
 {
-    // batches: issue ahead of the far field's wavefronts that run beside
-    if (BATCH) __builtin_amdgcn_s_setprio(3);
+    // batches, and the long lists of bodies in contact (SUB > 1): issue
+    // ahead of the far field's wavefronts that run beside (ensemble of 512:
+    // 16.25-16.30 -> 15.98-16.16 ms per step; collision N = 131072 after
+    // impact: 5.59-5.62 -> 5.03-5.23)
+    if (BATCH || SUB > 1) __builtin_amdgcn_s_setprio(3);
     const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
     const int t0 = pass_slot<SUB>(m), sub = pass_sub<SUB>();
     const bool act = t0 < m.slot_end;
@@ -346,8 +352,11 @@ __global__ __launch_bounds__(kBlock) void accel_kernel(
     const int kmax, float4* __restrict__ acc, float4* __restrict__ xvel, float* __restrict__ maxvsig,
     const int acc_has_gravity, const BlockTables T, const DevParams P0)
 {
-    // batches: issue ahead of the far field's wavefronts that run beside
-    if (BATCH) __builtin_amdgcn_s_setprio(3);
+    // batches, and the long lists of bodies in contact (SUB > 1): issue
+    // ahead of the far field's wavefronts that run beside (ensemble of 512:
+    // 16.25-16.30 -> 15.98-16.16 ms per step; collision N = 131072 after
+    // impact: 5.59-5.62 -> 5.03-5.23)
+    if (BATCH || SUB > 1) __builtin_amdgcn_s_setprio(3);
     const SlotMap m = slot_map<BATCH>(T, (int)nstride, kBlock / SUB);
     const int t0 = pass_slot<SUB>(m), sub = pass_sub<SUB>();
     const bool act = t0 < m.slot_end;
