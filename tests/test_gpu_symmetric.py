@@ -203,20 +203,23 @@ def test_deterministic(sym_ctx):
         assert np.array_equal(a[f], b[f]), f
 
 
-def test_step_captured_in_graph():
+@pytest.mark.parametrize("partial_bytes", [12 << 30, 1 << 20])
+def test_step_captured_in_graph(partial_bytes):
     """computeDerivatives + integrate of the symmetric sweep (the far field on
     the caller's stream, the rest forked onto the context's second stream)
     captured into a HIP graph on the caller's stream: every replay, with
     eager work and synchronisation between replays, equals the same step
     run eagerly, bit for bit.  The Morton order is computed by the warm-up
     step and kept (a long re-sort interval), so both paths sum in the same
-    order and the captured step holds no sort."""
-    print("symmetric step in a graph", flush=True)
+    order and the captured step holds no sort.  With 1 MiB for the partner
+    partials the far field runs in column batches inside the graph."""
+    print("symmetric step in a graph, partial bytes %d" % partial_bytes, flush=True)
     h, prm = solid_case(20000, seed=77)
     ctx = g.Context(0)
     try:
         ctx.set_option("sweep_symmetric", 1)
         ctx.set_option("reorder_interval", 1000)
+        ctx.set_option("sym_partial_bytes", partial_bytes)
         s = g.hip.Stream()
         de, dg = g.DeviceParticles.from_host(h), g.DeviceParticles.from_host(h)
 
@@ -229,6 +232,7 @@ def test_step_captured_in_graph():
         ctx2 = g.Context(0)  # the eager run, with the same options and its own order
         ctx2.set_option("sweep_symmetric", 1)
         ctx2.set_option("reorder_interval", 1000)
+        ctx2.set_option("sym_partial_bytes", partial_bytes)
         ctx2.compute_derivatives(de, prm, s)  # the warm-up step
         ctx2.integrate(de, prm, True, s)
         for rep in range(3):
