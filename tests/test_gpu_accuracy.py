@@ -110,22 +110,30 @@ def plastic_share(st, prm):
 def test_large_column_batches(gpu_ctx):
     """The large sphere (N = 1048576) with 1 GiB for the far field's partner
     partials (of the 6.4 GB the whole far field takes): seven column batches
-    of partner groups with running sums between them.  A compact sample
-    against the FP64 oracle at 1e-5, and the whole acceleration against the
-    unbatched far field's to FP32 rounding (same order, same lists)."""
+    of partner groups with running sums between them.  Two compact samples
+    against the FP64 oracle (1e-5 norm-wise; per particle 1e-5 on the first,
+    the unbatched far field's error on the second), and the whole
+    acceleration against the unbatched far field's to FP32 rounding (same
+    order, same lists)."""
     print("large sphere, far field in column batches", flush=True)
     h, prm = ics.large_sphere(1048576)
     O.set_threads(min(16, os.cpu_count() or 1))
     d = g.DeviceParticles.from_host(h)
-    gpu_ctx.compute_derivatives(d, prm)
-    g.hip.synchronize()
-    base = d.download(["acc", "density"])
-    gpu_ctx.set_option("sym_partial_bytes", 1 << 30)
+    # both evaluations sort the same positions into the same slot order, so
+    # the neighbour sums agree bit for bit whatever the calls before were
+    interval = gpu_ctx.get_option("reorder_interval")
+    budget = gpu_ctx.get_option("sym_partial_bytes")
+    gpu_ctx.set_option("reorder_interval", 1)
     try:
         gpu_ctx.compute_derivatives(d, prm)
         g.hip.synchronize()
+        base = d.download(["acc", "density"])
+        gpu_ctx.set_option("sym_partial_bytes", 1 << 30)
+        gpu_ctx.compute_derivatives(d, prm)
+        g.hip.synchronize()
     finally:
-        gpu_ctx.set_option("sym_partial_bytes", 12 << 30)
+        gpu_ctx.set_option("sym_partial_bytes", budget)
+        gpu_ctx.set_option("reorder_interval", interval)
     got = d.download(["acc", "density"])
     d.free()
     assert np.array_equal(got["density"], base["density"])
@@ -136,6 +144,7 @@ def test_large_column_batches(gpu_ctx):
     # second cluster, where a particle of small |a| has a per-particle error
     # of 1.3e-5 with the unbatched far field too: held there to the norm-wise
     # 1e-5 and to the unbatched per-particle error
+    bounds = []
     for key, ns, seed in (("sample1", 4096, 1), ("sample2", 1024, 2)):
         sample = compact_sample(pos, ns, seed)
         ref64 = sampled_oracle(prm, h, sample, "f64")
@@ -144,9 +153,10 @@ def test_large_column_batches(gpu_ctx):
         mxu, _ = vec_errors(base["acc"][sample], ref64["acc"][sample])
         res.update({key + "_size": ns, key + "_acc_max_rel": mx, key + "_acc_norm_rel": nr,
                     key + "_acc_floored": fl, key + "_unbatched_acc_max_rel": mxu})
-        assert nr <= 1e-5, res
-        assert mx <= (1e-5 if key == "sample1" else max(1e-5, 1.1 * mxu)), res
+        bounds.append((nr, mx, 1e-5 if key == "sample1" else max(1e-5, 1.1 * mxu)))
     record("large_column_batches", **res)
+    for nr, mx, mx_bound in bounds:
+        assert nr <= 1e-5 and mx <= mx_bound, res
     assert nrb <= 1e-6 and mxb <= 1e-5, res
 
 
