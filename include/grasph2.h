@@ -157,7 +157,9 @@ int grasph_destroy(grasph_ctx* ctx);
  * "reorder_interval"
  * (Morton re-sort every k derivative evaluations, default 8; 0 = never;
  * the order changes only the order of compensated sums, not which pairs
- * are found),
+ * are found; a new particle count or position array re-sorts at once,
+ * and an array allocated at a freed one's address counts as the same
+ * particles until the next re-sort: slower, never wrong),
  * "profile" (1 = record HIP events around each kernel), "sweep_i_per_lane"
  * (1 or 2), "sweep_slices" (j-split of the sweep, 0 = automatic),
  * "batch_waves_per_block", "batch_blocks_per_cu" (batched sweep launch

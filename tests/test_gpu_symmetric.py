@@ -21,6 +21,10 @@ pytestmark = pytest.mark.gpu
 def sym_ctx():
     ctx = g.Context(0)
     ctx.set_option("sweep_symmetric", 1)
+    # a fresh Morton order at every evaluation: the slot order, and so the
+    # rounding of the sums these tests compare bit for bit, depends on the
+    # positions alone, not on the evaluations before
+    ctx.set_option("reorder_interval", 1)
     yield ctx
     ctx.close()
 
