@@ -293,6 +293,22 @@ int main()
                     if (check_columns(n, w, Q, e)) return 1;
                     ++cases;
                 }
+    // known answers (partials of batch [c0, c1): 3 kGroup (c1 (c1 - 1) - c0 (c0 - 1)) / 2 floats):
+    // 12 GiB holds the whole far field up to 1448 groups (1448 * 1447 / 2 * 3072 =
+    // 3218313216 <= 3221225472 floats; 1449 groups need 3222761472); 2048 groups
+    // take two batches, [1, 1448) and [1448, 2048); 1 GiB and 1024 groups take
+    // seven, the first [1, 418) (418 * 417 / 2 * 3072 = 267734016 <= 268435456 <
+    // 269018112 = 419 * 418 / 2 * 3072)
+    {
+        const int64_t gib12 = (int64_t(12) << 30) / 4, gib1 = (int64_t(1) << 30) / 4;
+        if (sym_columns(1448, gib12).size() != 2) return fail("1448 groups in one batch", 1448, 0, 0);
+        if (sym_columns(1449, gib12).size() != 3) return fail("1449 groups in two batches", 1449, 0, 0);
+        const std::vector<int> two = sym_columns(2048, gib12);
+        if (two.size() != 3 || two[1] != 1448) return fail("2048 groups in two batches", 2048, (int64_t)two.size(), 0);
+        const std::vector<int> seven = sym_columns(1024, gib1);
+        if (seven.size() != 8 || seven[1] != 418) return fail("1024 groups, 1 GiB: seven batches", 1024, (int64_t)seven.size(), 0);
+        cases += 4;
+    }
     std::vector<int64_t> sizes;
     for (int i = 0; i < 200; ++i) sizes.push_back(rand() % 17000);
     if (check_batch(sizes)) return 1;
